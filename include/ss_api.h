@@ -1,0 +1,256 @@
+/*
+ * ss_api.h -- C ABI of the MI355X-native per-timestep hot loop of
+ * stochasticsampling (active-suspension Smoluchowski / Stokes solver).
+ *
+ * Every entry point takes plain pointers and sizes.  Host buffers are owned by
+ * the caller; device memory is owned by the library.  Return codes: SS_OK (0)
+ * on success, a negative SS_ERR_* on failure; ss_last_error() then returns a
+ * thread-local message.  One handle per host thread; handles are independent
+ * and the library takes no internal locks.
+ *
+ * Reference citations name files of clamydo/stochasticsampling v2.1.1.
+ */
+#ifndef SS_API_H
+#define SS_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SS_API_VERSION 1
+
+/* Return codes. */
+#define SS_OK 0
+#define SS_ERR_INVALID (-1)     /* bad argument or parameter combination */
+#define SS_ERR_DEVICE (-2)      /* HIP runtime error or no GPU           */
+#define SS_ERR_STATE (-3)       /* call out of order (e.g. step before init) */
+#define SS_ERR_UNSUPPORTED (-4) /* size or mode outside what is built     */
+#define SS_ERR_ALLOC (-5)       /* device or host allocation failed       */
+
+/* Arithmetic of particle states and FFTs. */
+#define SS_FP64 0 /* plain FP64 throughout                                  */
+#define SS_FP32 1 /* FP32 states with compensated positions, FP32 FFT       */
+
+/* Source of the per-particle Gaussian / Rayleigh increments. */
+#define SS_NOISE_PHILOX 0   /* counter-based Philox4x32-10 keyed by (seed, step, particle id) */
+#define SS_NOISE_EXTERNAL 1 /* caller-supplied RandomVector buffer, see ss_set_noise */
+
+/* Binning strategy for particle -> (x,y,z,phi,theta) histogram. */
+#define SS_BIN_AUTO 0      /* SORTED for FP32, HISTOGRAM for FP64                        */
+#define SS_BIN_HISTOGRAM 1 /* plain: full 5-D u32 histogram by global atomics           */
+#define SS_BIN_SORTED 2    /* particles kept sorted by cell; LDS-privatised counts      */
+
+/* FFT variant used by the spectral solve. */
+#define SS_FFT_AUTO 0  /* autotuned among the variants that pass the accuracy gate */
+#define SS_FFT_RADIX 1 /* LDS-staged mixed-radix Stockham                          */
+#define SS_FFT_DFT 2   /* dense DFT matrix product (3-pass split-bf16 on MFMA)     */
+
+/* Flags. */
+#define SS_FLAG_QUASI2D 1u            /* z = 0, theta = pi/2 after every step (feature "quasi2d") */
+#define SS_FLAG_MAGNETIC_ALWAYS 2u    /* run the magnetic solve even when its coefficients are 0  */
+#define SS_FLAG_KEEP_DISTRIBUTION 4u  /* keep the full count grid of every step's binning         */
+
+/* Size of the simulation box (lib.rs BoxSize). */
+typedef struct ss_box {
+    double x, y, z;
+} ss_box;
+
+/* Size of the discrete grid (lib.rs GridSize). */
+typedef struct ss_grid {
+    int32_t x, y, z, phi, theta;
+} ss_grid;
+
+/*
+ * Everything Simulation::new reads (src/bin/simulation/mod.rs:91-160): the
+ * SimulationSettings fields used by the step (settings/mod.rs:108-118) and all
+ * Parameters fields (settings/mod.rs:51-67), plus the build's own knobs.
+ */
+typedef struct ss_params {
+    ss_box box_size;
+    ss_grid grid_size;
+    uint64_t number_of_particles;
+    double timestep;
+    uint64_t seed;
+    /* Parameters */
+    double magnetic_drag;
+    double shape;
+    double hydro_screening;
+    double volume_exclusion;
+    double magnetic_reorientation;
+    double diffusion_translational;
+    double diffusion_rotational;
+    double stress_active;
+    double stress_magnetic;
+    double magnetic_dipole_dipole;
+    /* Direction of the external field; (0,0,0) selects the reference's y axis
+       (modifiers.rs:108-110, stress.rs:121-141). */
+    double field_direction[3];
+    /* Build knobs. */
+    int32_t precision;   /* SS_FP64 | SS_FP32          */
+    int32_t noise_mode;  /* SS_NOISE_*                 */
+    int32_t binning;     /* SS_BIN_*                   */
+    int32_t fft_variant; /* SS_FFT_*                   */
+    uint32_t flags;      /* SS_FLAG_*                  */
+    int32_t device;      /* HIP device ordinal          */
+} ss_params;
+
+/* Fills p with the reference's defaults: zero optional parameters, ss y-field,
+   FP64, Philox noise, automatic binning and FFT, device 0. */
+void ss_params_default(ss_params* p);
+
+/* Thread-local message of the last failing call ("" if none). */
+const char* ss_last_error(void);
+
+/* SS_API_VERSION of the loaded library. */
+int ss_version(void);
+
+/* Number of visible GPUs (0 without a GPU; never an error in that case). */
+int ss_device_count(int* count);
+
+/* ---------------------------------------------------------------------------
+ * Simulation handle -- replaces struct Simulation (src/bin/simulation/mod.rs:60-67).
+ * ------------------------------------------------------------------------- */
+typedef struct ss_sim ss_sim;
+
+/* Simulation::new (src/bin/simulation/mod.rs:91-160).  All planning and device
+   allocation happens here; ss_step allocates nothing. */
+int ss_create(const ss_params* params, ss_sim** out);
+
+/* Drop for Simulation (src/bin/simulation/mod.rs:377-381). */
+int ss_destroy(ss_sim* sim);
+
+/* Simulation::init (src/bin/simulation/mod.rs:163-196).  particles is N x 5
+   doubles in serde order [x, y, z, phi, theta] (particle.rs:363-395).  Every
+   particle is sanitised like Particle::new (particle.rs:186-201) and the
+   distribution is sampled once.  n must equal number_of_particles. */
+int ss_init(ss_sim* sim, const double* particles, uint64_t n);
+
+/* Replaces the noise draw of do_timestep (src/bin/simulation/mod.rs:259-286)
+   for noise_mode SS_NOISE_EXTERNAL: rv is N x 5 doubles per particle, in the
+   order of RandomVector (mod.rs:51-57): [x, y, z, axis_angle, rotate_angle],
+   already scaled like the reference (rotate_angle = dr*sqrt(-2 ln(1-U))).
+   Every following step uses this buffer until it is replaced. */
+int ss_set_noise(ss_sim* sim, const double* rv, uint64_t n);
+
+/* Simulation::do_timestep / Iterator::next (src/bin/simulation/mod.rs:251-393),
+   nsteps times.  timestep_out (may be NULL) receives the counter afterwards. */
+int ss_step(ss_sim* sim, uint64_t nsteps, uint64_t* timestep_out);
+
+/* Simulation::get_timestep (src/bin/simulation/mod.rs:246-248). */
+int ss_get_timestep(const ss_sim* sim, uint64_t* timestep);
+
+/* Simulation::get_particles / get_particles_head (mod.rs:221-228): the first
+   n particles (n <= N) as n x 5 doubles, in the order they were given to init. */
+int ss_get_particles(ss_sim* sim, double* out, uint64_t n);
+
+/* Simulation::get_distribution (mod.rs:231-233): the distribution sampled at
+   the start of the last step (or by init).  counts (may be NULL) receives the
+   integer bin counts [x][y][z][phi][theta]; dist (may be NULL) the normalised
+   values exactly as Distribution::sample_from times the box volume
+   (distribution/mod.rs:164-184, mod.rs:253-257).  With SS_BIN_SORTED this
+   needs SS_FLAG_KEEP_DISTRIBUTION. */
+int ss_get_distribution(ss_sim* sim, uint32_t* counts, double* dist);
+
+/* Simulation::get_flow_field (mod.rs:236-238) and the gradient the step used:
+   u is [3][nx][ny][nz], grad_u (may be NULL) is [3][3][nx][ny][nz] with
+   grad_u[a][b] = d_a u_b (spectral_solver.rs:192-229), real parts. */
+int ss_get_flow_field(ss_sim* sim, double* u, double* grad_u);
+
+/* Simulation::get_magnetic_field (mod.rs:241-243) and its gradient
+   (magnetic_solver.rs:139-169), real parts, same layouts as above. */
+int ss_get_magnetic_field(ss_sim* sim, double* b, double* grad_b);
+
+/* Simulation::get_snapshot (mod.rs:210-217).  particles: N x 5; compensation
+   (may be NULL): N x 3 low-order position parts of the FP32 path (zeros in
+   FP64).  The noise generator state is (seed, timestep). */
+int ss_get_snapshot(ss_sim* sim, double* particles, double* compensation, uint64_t* timestep);
+
+/* Simulation::resume (mod.rs:199-207): init from the snapshot, restore the
+   compensation (may be NULL) and the timestep counter. */
+int ss_resume(ss_sim* sim, const double* particles, const double* compensation, uint64_t n,
+              uint64_t timestep);
+
+/* Per-stage device time of the last ss_profile_steps call, measured with HIP
+   events on the simulation's stream.  Stage names: "bin", "fft_forward",
+   "solve", "fft_inverse", "pack", "update", "sort".  Returns the number of
+   stages written (at most max_stages); ms_per_step[i] is the mean per step. */
+int ss_profile_steps(ss_sim* sim, uint64_t nsteps, double* ms_per_step, char* names,
+                     int name_stride, int max_stages);
+
+/* Wait for all work of the handle. */
+int ss_synchronize(ss_sim* sim);
+
+/* ---------------------------------------------------------------------------
+ * Initial conditions (no reference counterpart for the perturbed state; the
+ * uniform part mirrors Particle::create_isotropic, particle.rs:224-256).
+ * Positions uniform in the box; cos(theta) drawn by exact inverse CDF from
+ * 1 + amplitude*sin(2 pi x / Lx)*cos(theta); phi uniform.  Uniforms come
+ * from Philox4x32-10 with key (seed) and counter (index, 0xC0FFEE, 0, 0).
+ * ------------------------------------------------------------------------- */
+int ss_initial_condition(const ss_params* params, double amplitude, uint64_t seed,
+                         double* particles, uint64_t n);
+
+/* ---------------------------------------------------------------------------
+ * FFT plan -- replaces FFTPlan (crates/fftw3/src/fft.rs:33-244): unnormalised,
+ * in place, forward sign -1, backward +1, batched over `batch` contiguous
+ * [nx][ny][nz] complex arrays (interleaved re, im).
+ * ------------------------------------------------------------------------- */
+typedef struct ss_fft_plan ss_fft_plan;
+
+/* FFTPlan::new_c2c_inplace_3d (fft.rs:156-178).  direction: -1 forward, +1
+   backward. */
+int ss_fft_plan_create(int nx, int ny, int nz, int direction, int batch, int precision,
+                       int variant, int device, ss_fft_plan** out);
+
+/* FFTPlan::reexecute3d (fft.rs:226-231) on device memory, enqueued on the given
+   hipStream_t (NULL: the plan's own stream; the call then waits for it). */
+int ss_fft_execute(ss_fft_plan* plan, void* device_data, void* hip_stream);
+
+/* Same on host memory: copies in, executes, copies out. */
+int ss_fft_execute_host(ss_fft_plan* plan, void* host_data);
+
+/* Drop for FFTPlan (fft.rs:235-244). */
+int ss_fft_plan_destroy(ss_fft_plan* plan);
+
+/* ---------------------------------------------------------------------------
+ * Solver entry points for unit-level parity.
+ * ------------------------------------------------------------------------- */
+
+/* SpectralSolver::new + mean_flow_field (spectral_solver.rs:33-68,233-254) on a
+   given distribution dist [nx][ny][nz][phi][theta] (already multiplied by the
+   box volume, like the caller in src/bin/simulation/mod.rs:253-257).  Uses
+   params' grid, box, stress prefactors, shape, hydro_screening and precision.
+   Outputs real parts: u [3][nx][ny][nz], grad_u [3][3][nx][ny][nz] (may be NULL). */
+int ss_flow_solve(const ss_params* params, const double* dist, double* u, double* grad_u);
+
+/* MagneticSolver::mean_magnetic_field (magnetic_solver.rs:139-161) on a given
+   distribution; outputs real parts of B [3][..] and grad B [3][3][..]. */
+int ss_magnetic_solve(const ss_params* params, const double* dist, double* b, double* grad_b);
+
+/* Number of doubles per particle in the fields argument of ss_debug_update. */
+#define SS_DEBUG_FIELDS 34
+
+/* The device particle update -- LangevinBuilder chain of
+   src/bin/simulation/mod.rs:323-363 with modifiers.rs:39-231 -- applied to n
+   independent particles with explicitly given cell values.  fields per
+   particle: u[3], W[3][3], E[3][3], rho, B[3] (before the magnetic_dipole_dipole
+   factor), gradB[3][3] (row-major, [a][b]); rv per particle: RandomVector
+   [x,y,z,axis_angle,rotate_angle].  particles / out: n x 5.  Computed in
+   params->precision. */
+int ss_debug_update(const ss_params* params, uint64_t n, const double* particles,
+                    const double* fields, const double* rv, double* out);
+
+/* The device binning rule -- Distribution::coord_to_grid
+   (distribution/mod.rs:86-141) -- for n particles (n x 5, converted to
+   params->precision first); out receives n x 5 int32 grid coordinates. */
+int ss_debug_coord_to_grid(const ss_params* params, uint64_t n, const double* particles,
+                           int32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SS_API_H */

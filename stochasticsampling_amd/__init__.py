@@ -1,0 +1,29 @@
+"""MI355X-native hot loop of stochasticsampling: Python mirror of the
+reference's Simulation / SpectralSolver / FFTPlan API over libss (HIP)."""
+from .params import (  # noqa: F401
+    BIN_AUTO,
+    BIN_HISTOGRAM,
+    BIN_SORTED,
+    FFT_AUTO,
+    FFT_RADIX,
+    FLAG_KEEP_DISTRIBUTION,
+    FLAG_MAGNETIC_ALWAYS,
+    FLAG_QUASI2D,
+    FP32,
+    FP64,
+    NOISE_EXTERNAL,
+    NOISE_PHILOX,
+    Params,
+    make_params,
+)
+from .lib import SSError, device_count, load  # noqa: F401
+from .simulation import (  # noqa: F401
+    FFTPlan,
+    Simulation,
+    Snapshot,
+    debug_coord_to_grid,
+    debug_update,
+    initial_condition,
+    mean_flow_field,
+    mean_magnetic_field,
+)

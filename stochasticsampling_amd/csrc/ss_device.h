@@ -1,0 +1,315 @@
+// ss_device.h -- device-side building blocks shared by the kernels of libss:
+// Philox4x32-10 noise, the periodic wraps, the binning rule and the particle
+// update (the LangevinBuilder chain of src/bin/simulation/mod.rs:323-363).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ss {
+
+template <typename T>
+struct cplx {
+    T x, y;
+};
+
+template <typename T>
+struct Math;
+
+template <>
+struct Math<double> {
+    static constexpr double PI = 3.141592653589793;
+    static constexpr double TWOPI = 2.0 * 3.141592653589793;
+    __device__ static inline double sin(double a) { return ::sin(a); }
+    __device__ static inline double cos(double a) { return ::cos(a); }
+    __device__ static inline void sincos(double a, double* s, double* c) { ::sincos(a, s, c); }
+    __device__ static inline double atan2(double y, double x) { return ::atan2(y, x); }
+    __device__ static inline double sqrt(double a) { return ::sqrt(a); }
+    __device__ static inline double log(double a) { return ::log(a); }
+    __device__ static inline double floor(double a) { return ::floor(a); }
+    __device__ static inline double fmod(double a, double b) { return ::fmod(a, b); }
+    __device__ static inline double fabs(double a) { return ::fabs(a); }
+};
+
+template <>
+struct Math<float> {
+    static constexpr float PI = 3.14159265358979323846f;
+    static constexpr float TWOPI = 6.28318530717958647692f;
+    __device__ static inline float sin(float a) { return ::sinf(a); }
+    __device__ static inline float cos(float a) { return ::cosf(a); }
+    __device__ static inline void sincos(float a, float* s, float* c) { ::sincosf(a, s, c); }
+    __device__ static inline float atan2(float y, float x) { return ::atan2f(y, x); }
+    __device__ static inline float sqrt(float a) { return ::sqrtf(a); }
+    __device__ static inline float log(float a) { return ::logf(a); }
+    __device__ static inline float floor(float a) { return ::floorf(a); }
+    __device__ static inline float fmod(float a, float b) { return ::fmodf(a, b); }
+    __device__ static inline float fabs(float a) { return ::fabsf(a); }
+};
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+// ---------------------------------------------------------------------------
+__host__ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                              uint32_t k0, uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+// Per particle and step: counters (j, id, step lo, step hi), j = 0..2, key
+// (seed lo, seed hi).  Uniform k takes its high 32 bits from word k and (FP64)
+// 21 more bits from the top of word 6+k.  The FP32 path uses the high 24 bits
+// of word k only, so both precisions see the same increments to ~2^-24.
+// Result: RandomVector [x, y, z, axis_angle, rotate_angle] of
+// src/bin/simulation/mod.rs:51-57,259-286 (Box-Muller normals, Rayleigh angle).
+__device__ inline void philox_noise(uint64_t seed, uint64_t step, uint32_t id, double dr,
+                                    double out[5]) {
+    uint32_t w[12];
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    const uint32_t s0 = (uint32_t)step, s1 = (uint32_t)(step >> 32);
+    philox4x32_10(0u, id, s0, s1, k0, k1, w);
+    philox4x32_10(1u, id, s0, s1, k0, k1, w + 4);
+    philox4x32_10(2u, id, s0, s1, k0, k1, w + 8);
+    double u[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        uint64_t m = ((uint64_t)w[k] << 21) | (uint64_t)(w[6 + k] >> 11);
+        u[k] = ((double)m + 0.5) * (1.0 / 9007199254740992.0);
+    }
+    const double TWOPI = Math<double>::TWOPI;
+    double r0 = ::sqrt(-2.0 * ::log(u[0]));
+    double r1 = ::sqrt(-2.0 * ::log(u[2]));
+    out[0] = r0 * ::cos(TWOPI * u[1]);
+    out[1] = r0 * ::sin(TWOPI * u[1]);
+    out[2] = r1 * ::cos(TWOPI * u[3]);
+    out[3] = TWOPI * u[4];
+    out[4] = dr * ::sqrt(-2.0 * ::log(1.0 - u[5]));
+}
+
+__device__ inline void philox_noise(uint64_t seed, uint64_t step, uint32_t id, float dr,
+                                    float out[5]) {
+    uint32_t w[8];
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    const uint32_t s0 = (uint32_t)step, s1 = (uint32_t)(step >> 32);
+    philox4x32_10(0u, id, s0, s1, k0, k1, w);
+    philox4x32_10(1u, id, s0, s1, k0, k1, w + 4);
+    float u[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) u[k] = ((float)(w[k] >> 8) + 0.5f) * (1.0f / 16777216.0f);
+    const float TWOPI = Math<float>::TWOPI;
+    float r0 = sqrtf(-2.0f * logf(u[0]));
+    float r1 = sqrtf(-2.0f * logf(u[2]));
+    float s, c;
+    sincosf(TWOPI * u[1], &s, &c);
+    out[0] = r0 * c;
+    out[1] = r0 * s;
+    out[2] = r1 * cosf(TWOPI * u[3]);
+    out[3] = TWOPI * u[4];
+    out[4] = dr * sqrtf(-2.0f * logf(1.0f - u[5]));
+}
+
+// ---------------------------------------------------------------------------
+// Periodic wraps: modulo and ang_pbc (particle.rs:26-44).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline T modulo(T f, T m) {
+    T r = Math<T>::fmod(f, m);
+    return r < T(0) ? r + Math<T>::fabs(m) : r;
+}
+
+template <typename T>
+__device__ inline void ang_pbc(T phi, T theta, T& ophi, T& otheta) {
+    const T PI = Math<T>::PI, TWOPI = Math<T>::TWOPI;
+    T t = modulo(theta, TWOPI);
+    if (t > PI) {
+        ophi = modulo(phi + PI, TWOPI);
+        otheta = TWOPI - t;
+    } else {
+        ophi = modulo(phi, TWOPI);
+        otheta = t;
+    }
+}
+
+// Binning rule of Distribution::coord_to_grid / get_cell_index
+// (distribution/mod.rs:112-137, mesh/mod.rs:49-63): floor(q / g), an index equal
+// to n folds to n-1.  The lower clamp only guards memory; pbc keeps q >= 0.
+template <typename T>
+__device__ inline int grid_index(T q, T g, int n) {
+    T f = Math<T>::floor(q / g);
+    int i = (int)f;
+    if (f >= (T)n) i = n - 1;
+    if (i == n) i = n - 1;
+    if (f < T(0)) i = 0;
+    return i;
+}
+
+// ---------------------------------------------------------------------------
+// Field record at one cell, as the update reads it.
+// ---------------------------------------------------------------------------
+enum : int {
+    REC_U = 0,   // u[3]
+    REC_W = 3,   // W01, W02, W12 (W antisymmetric)
+    REC_E = 8,   // E00, E01, E02, E11, E12, E22 (E symmetric)
+    REC_RHO = 14,
+    REC_B = 15,  // Re B * magnetic_dipole_dipole
+    REC_GB = 18, // grad B [a][b]
+    REC_SMALL = 8,
+    REC_FULL = 32,
+};
+
+template <typename T>
+struct UpdateParams {
+    T dt;
+    T drag;            // magnetic_drag
+    T shape;
+    T vol_excl;        // volume_exclusion
+    T kappa;           // magnetic_reorientation
+    T dir[3];          // external field direction
+    int default_dir;   // 1: reference's y axis, built as [0, kappa, 0]
+    T d_trans;         // diffusion.translational
+    T dr;              // sqrt(2 D_r dt)
+    T box[3];
+    int use_e, use_rho, use_b, use_gb;
+    int quasi2d;
+};
+
+// One particle: the LangevinBuilder chain (src/bin/simulation/mod.rs:348-363,
+// modifiers.rs).  rec: field record of the particle's cell.  rv: RandomVector.
+// Outputs the position increment dp (to be added with pbc) and the new
+// orientation angles after ang_pbc.
+template <typename T>
+__device__ inline void langevin(const UpdateParams<T>& P, T phi, T theta, const T* rec, const T rv[5],
+                                T dp[3], T& ophi, T& otheta) {
+    T sp, cp, st, ct;
+    Math<T>::sincos(phi, &sp, &cp);
+    Math<T>::sincos(theta, &st, &ct);
+    const T n0 = st * cp, n1 = st * sp, n2 = ct;
+    // self_propulsion + convection (modifiers.rs:39-55)
+    T p0 = n0 + rec[REC_U + 0];
+    T p1 = n1 + rec[REC_U + 1];
+    T p2 = n2 + rec[REC_U + 2];
+    T o0 = T(0), o1 = T(0), o2 = T(0);
+    // magnetic_dipole_dipole_force (modifiers.rs:59-69, magnetic_interaction/mod.rs:13-17)
+    if (P.use_gb) {
+        const T* g = rec + REC_GB;
+        T f0 = (g[0] * n0 + g[1] * n1) + g[2] * n2;
+        T f1 = (g[3] * n0 + g[4] * n1) + g[5] * n2;
+        T f2 = (g[6] * n0 + g[7] * n1) + g[8] * n2;
+        p0 = p0 + f0 * P.drag;
+        p1 = p1 + f1 * P.drag;
+        p2 = p2 + f2 * P.drag;
+    }
+    // external_field_alignment (modifiers.rs:103-118)
+    if (P.kappa != T(0)) {
+        T b0, b1, b2;
+        if (P.default_dir) {
+            b0 = T(0);
+            b1 = P.kappa;
+            b2 = T(0);
+        } else {
+            b0 = P.dir[0] * P.kappa;
+            b1 = P.dir[1] * P.kappa;
+            b2 = P.dir[2] * P.kappa;
+        }
+        T nb = (n0 * b0 + n1 * b1) + n2 * b2;
+        o0 = o0 + (b0 - n0 * nb);
+        o1 = o1 + (b1 - n1 * nb);
+        o2 = o2 + (b2 - n2 * nb);
+    }
+    // magnetic_dipole_dipole_rotation (modifiers.rs:173-186)
+    if (P.use_b) {
+        const T* b = rec + REC_B;
+        T nb = (n0 * b[0] + n1 * b[1]) + n2 * b[2];
+        o0 = o0 + (b[0] - n0 * nb);
+        o1 = o1 + (b[1] - n1 * nb);
+        o2 = o2 + (b[2] - n2 * nb);
+    }
+    // jeffrey_vorticity (modifiers.rs:123-142): -W n with W antisymmetric
+    {
+        const T w01 = rec[REC_W + 0], w02 = rec[REC_W + 1], w12 = rec[REC_W + 2];
+        T f0 = w01 * n1 + w02 * n2;
+        T f1 = -w01 * n0 + w12 * n2;
+        T f2 = -w02 * n0 - w12 * n1;
+        o0 = o0 - f0;
+        o1 = o1 - f1;
+        o2 = o2 - f2;
+    }
+    // jeffrey_strain (modifiers.rs:147-169)
+    if (P.use_e) {
+        const T* e = rec + REC_E;
+        T f0 = ((e[0] * n0 + e[1] * n1) + e[2] * n2) * P.shape;
+        T f1 = ((e[1] * n0 + e[3] * n1) + e[4] * n2) * P.shape;
+        T f2 = ((e[2] * n0 + e[4] * n1) + e[5] * n2) * P.shape;
+        T fn = (f0 * n0 + f1 * n1) + f2 * n2;
+        o0 = o0 + (f0 - n0 * fn);
+        o1 = o1 + (f1 - n1 * fn);
+        o2 = o2 + (f2 - n2 * fn);
+    }
+    // step (langevin_builder/mod.rs:114-119)
+    p0 *= P.dt;
+    p1 *= P.dt;
+    p2 *= P.dt;
+    o0 *= P.dt;
+    o1 *= P.dt;
+    o2 *= P.dt;
+    // translational_diffusion (modifiers.rs:89-99), std of mod.rs:346
+    {
+        T diff = P.use_rho ? Math<T>::sqrt(T(2) * P.dt * (P.d_trans + P.vol_excl * rec[REC_RHO]))
+                           : Math<T>::sqrt(T(2) * P.dt * P.d_trans);
+        p0 = p0 + rv[0] * diff;
+        p1 = p1 + rv[1] * diff;
+        p2 = p2 + rv[2] * diff;
+    }
+    // rotational_diffusion (modifiers.rs:189-231): q (0,n) q*, quaternion 0.3.1
+    {
+        T sa, ca;
+        Math<T>::sincos(rv[3], &sa, &ca);
+        const T ax0 = cp * ct * sa - ca * sp;
+        const T ax1 = ca * cp + ct * sa * sp;
+        const T ax2 = -sa * st;
+        T sh, q0;
+        Math<T>::sincos(rv[4] / T(2), &sh, &q0);
+        const T q1 = ax0 * sh, q2 = ax1 * sh, q3 = ax2 * sh;
+        // t = q * (0, n)
+        const T t0 = T(0) * q0 - (q1 * n0 + q2 * n1 + q3 * n2);
+        const T t1 = (n0 * q0 + q1 * T(0)) + (q2 * n2 - q3 * n1);
+        const T t2 = (n1 * q0 + q2 * T(0)) + (q3 * n0 - q1 * n2);
+        const T t3 = (n2 * q0 + q3 * T(0)) + (q1 * n1 - q2 * n0);
+        // r = t * conj(q), vector part
+        const T c1 = -q1, c2 = -q2, c3 = -q3;
+        const T r1 = (c1 * t0 + t1 * q0) + (t2 * c3 - t3 * c2);
+        const T r2 = (c2 * t0 + t2 * q0) + (t3 * c1 - t1 * c3);
+        const T r3 = (c3 * t0 + t3 * q0) + (t1 * c2 - t2 * c1);
+        o0 = o0 + (r1 - n0);
+        o1 = o1 + (r2 - n1);
+        o2 = o2 + (r3 - n2);
+    }
+    // finalize (langevin_builder/mod.rs:121-125, particle.rs:106-121)
+    const T v0 = n0 + o0, v1 = n1 + o1, v2 = n2 + o2;
+    const T rxy = Math<T>::sqrt(v0 * v0 + v1 * v1);
+    const T nphi = Math<T>::atan2(v1, v0);
+    const T ntheta = Math<T>::PI / T(2) - Math<T>::atan2(v2, rxy);
+    ang_pbc(nphi, ntheta, ophi, otheta);
+    if (P.quasi2d) otheta = Math<T>::PI / T(2);
+    dp[0] = p0;
+    dp[1] = p1;
+    dp[2] = p2;
+}
+
+}  // namespace ss

@@ -1,0 +1,49 @@
+// ss_internal.h -- host-side declarations shared by the translation units of
+// libss (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/ss_api.h"
+#include "ss_device.h"
+
+namespace ss {
+
+// Thread-local error message behind ss_last_error().
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define SS_HIP_CHECK(expr)                                                             \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return ::ss::fail(SS_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Batched 3-D c2c FFT over contiguous [nx][ny][nz] complex arrays.
+// ---------------------------------------------------------------------------
+struct FftAxis {
+    int n = 0;
+    int nf = 0;
+    int fac[16] = {0};
+    void* tw = nullptr;  // n complex twiddles exp(-2 pi i t / n) in the plan precision
+};
+
+struct Fft3d {
+    int nx = 0, ny = 0, nz = 0;
+    int precision = SS_FP64;
+    FftAxis ax[3];
+    int init(int nx, int ny, int nz, int precision);
+    void release();
+    // In-place, unnormalised; sign -1 forward, +1 backward.
+    int run(void* data, int batch, int sign, hipStream_t stream) const;
+};
+
+int factorize(int n, int* fac);
+
+}  // namespace ss

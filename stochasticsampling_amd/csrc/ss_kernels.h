@@ -1,0 +1,669 @@
+// ss_kernels.h -- the per-step kernels of the hot loop (binning, moment
+// contraction, k-space solve, field packing, particle update, counting sort).
+// Included by ss_sim.hip only.
+#pragma once
+
+#include "ss_device.h"
+
+namespace ss {
+
+constexpr uint32_t INVALID_KEY = 0xFFFFFFFFu;
+
+// Geometry and scaling constants of one system, in the working precision.
+template <typename T>
+struct Geo {
+    int nx, ny, nz, np, nt;
+    int ns, na;
+    T gx, gy, gz, gphi, gtheta;  // GridWidth (grid_width.rs:23-31)
+    T box[3];
+};
+
+template <typename T>
+__device__ inline int cell_of(const Geo<T>& g, T x, T y, T z) {
+    int ix = grid_index(x, g.gx, g.nx);
+    int iy = grid_index(y, g.gy, g.ny);
+    int iz = grid_index(z, g.gz, g.nz);
+    return (ix * g.ny + iy) * g.nz + iz;
+}
+
+template <typename T>
+__device__ inline int angle_of(const Geo<T>& g, T phi, T theta) {
+    return grid_index(phi, g.gphi, g.np) * g.nt + grid_index(theta, g.gtheta, g.nt);
+}
+
+// Wave64 run detection on a key that is mostly constant across neighbouring
+// lanes (particles sorted by cell).  A run is a maximal stretch of adjacent
+// lanes with equal keys.
+struct Run {
+    int head;     // lane of this lane's run head
+    int len;      // length of the run (valid on head lanes)
+    bool is_head;
+};
+
+__device__ inline Run wave_run(uint32_t key) {
+    const int lane = __lane_id();
+    uint32_t prev = __shfl_up(key, 1);
+    bool head = (lane == 0) || (prev != key);
+    unsigned long long heads = __ballot(head);
+    unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    unsigned long long above = heads & ~le;
+    Run r;
+    r.is_head = head;
+    r.len = (above ? __builtin_ctzll(above) : 64) - lane;
+    r.head = 63 - __builtin_clzll(heads & le);
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Binning, plain path: full 5-D count grid by global atomics
+// (Distribution::histogram_from, distribution/mod.rs:146-160).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_bin_hist(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z,
+               const T* __restrict__ phi, const T* __restrict__ theta, uint64_t n, Geo<T> g,
+               uint32_t* __restrict__ hist) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int c = cell_of(g, x[i], y[i], z[i]);
+    int a = angle_of(g, phi[i], theta[i]);
+    atomicAdd(hist + (size_t)c * g.na + a, 1u);
+}
+
+// Contraction of one cell's angular counts against the moment kernels
+// (average_stress, stress.rs:60-96; DirectorField, director.rs:36-61; density,
+// src/bin/simulation/mod.rs:312-316), one wavefront per cell.  kern: [nm][na]
+// in LDS; out: moments m (times scale) as complex with zero imaginary part.
+template <typename T, int NM>
+__device__ inline void contract_cell(const uint32_t* cnt, int na, const T* kern, T scale, T rho_scale,
+                                     int cell, int ns, cplx<T>* spec, T* rho) {
+    const int lane = __lane_id();
+    T acc[NM];
+    T tot = T(0);
+#pragma unroll
+    for (int q = 0; q < NM; ++q) acc[q] = T(0);
+    for (int a = lane; a < na; a += 64) {
+        const T c = (T)cnt[a];
+        tot += c;
+#pragma unroll
+        for (int q = 0; q < NM; ++q) acc[q] += kern[q * na + a] * c;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int q = 0; q < NM; ++q) acc[q] += __shfl_xor(acc[q], off);
+        tot += __shfl_xor(tot, off);
+    }
+    if (lane < NM) {
+        T v = acc[0];
+#pragma unroll
+        for (int q = 1; q < NM; ++q)
+            if (lane == q) v = acc[q];
+        spec[(size_t)lane * ns + cell] = cplx<T>{v * scale, T(0)};
+    }
+    if (rho && lane == 0) rho[cell] = tot * rho_scale;
+}
+
+template <typename T, int NM>
+__global__ void __launch_bounds__(256)
+    k_contract_hist(const uint32_t* __restrict__ hist, Geo<T> g, const T* __restrict__ kern_g, T scale,
+                    T rho_scale, cplx<T>* __restrict__ spec, T* __restrict__ rho) {
+    extern __shared__ unsigned char smem_raw[];
+    T* kern = reinterpret_cast<T*>(smem_raw);
+    for (int t = threadIdx.x; t < NM * g.na; t += blockDim.x) kern[t] = kern_g[t];
+    __syncthreads();
+    const int wave = threadIdx.x >> 6;
+    for (int k = 0; k < 16; ++k) {
+        const int cell = blockIdx.x * 64 + k * 4 + wave;
+        if (cell >= g.ns) return;
+        contract_cell<T, NM>(hist + (size_t)cell * g.na, g.na, kern, scale, rho_scale, cell, g.ns, spec,
+                             rho);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Binning, sorted path: a workgroup owns CB consecutive cells, counts the
+// angle bins of their particles (contiguous in memory) in LDS and contracts
+// them.  Reads 2 bytes per particle.
+// ---------------------------------------------------------------------------
+template <typename T, int NM, int CB>
+__global__ void __launch_bounds__(256)
+    k_bin_sorted(const uint16_t* __restrict__ ab, const uint32_t* __restrict__ off, Geo<T> g,
+                 const T* __restrict__ kern_g, T scale, T rho_scale, cplx<T>* __restrict__ spec,
+                 T* __restrict__ rho, uint32_t* __restrict__ hist_out) {
+    extern __shared__ unsigned char smem_raw[];
+    T* kern = reinterpret_cast<T*>(smem_raw);
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(kern + NM * g.na);
+    __shared__ uint32_t soff[CB + 1];
+    const int c0 = blockIdx.x * CB;
+    const int ncell = min(CB, g.ns - c0);
+    for (int t = threadIdx.x; t < NM * g.na; t += blockDim.x) kern[t] = kern_g[t];
+    for (int t = threadIdx.x; t < CB * g.na; t += blockDim.x) cnt[t] = 0u;
+    for (int t = threadIdx.x; t <= ncell; t += blockDim.x) soff[t] = off[c0 + t];
+    __syncthreads();
+    const uint32_t p0 = soff[0], p1 = soff[ncell];
+    for (uint32_t j = p0 + threadIdx.x; j < p1; j += blockDim.x) {
+        int lo = 0, hi = ncell;  // find c with soff[c] <= j < soff[c+1]
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (soff[mid] <= j)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        atomicAdd(cnt + lo * g.na + ab[j], 1u);
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6;
+    for (int c = wave; c < ncell; c += 4)
+        contract_cell<T, NM>(cnt + c * g.na, g.na, kern, scale, rho_scale, c0 + c, g.ns, spec, rho);
+    if (hist_out)
+        for (int t = threadIdx.x; t < ncell * g.na; t += blockDim.x)
+            hist_out[(size_t)c0 * g.na + t] = cnt[t];
+}
+
+// ---------------------------------------------------------------------------
+// k-space solve: fft_mean_flow_field + update_gradient
+// (spectral_solver.rs:133-229) and fft_mean_magnetic_field + update_gradient
+// (magnetic_solver.rs:71-135), one thread per wave vector.
+//
+// Input spec_in: sigma-hat in `nsig` components (6 symmetric or 9), then
+// P-hat (3) when `mag`.  Output layout:
+//   hot  (full == 0): u(3), W01 W02 W12 (3), [E 6], [b 3], [gradB 9]
+//   full (full == 1): u(3), G(9), [b 3, gradB 9]
+// W and E are formed in k-space: the inverse transform is linear, so
+// IFFT((G - G^T)/2) equals (IFFT(G) - IFFT(G)^T)/2 of
+// src/bin/simulation/mod.rs:302-308.
+// ---------------------------------------------------------------------------
+struct SolveFlags {
+    int nsig;
+    int mag;      // P-hat present
+    int full;
+    int want_e, want_b, want_gb;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_solve(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, Geo<T> g,
+            const T* __restrict__ kx, const T* __restrict__ ky, const T* __restrict__ kz,
+            const T* __restrict__ ik2, const T* __restrict__ kinv, T screening, T norm, SolveFlags f) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.ns) return;
+    const int ns = g.ns;
+    const int iz = idx % g.nz, iy = (idx / g.nz) % g.ny, ix = idx / (g.ny * g.nz);
+    const T k[3] = {kx[ix], ky[iy], kz[iz]};
+    const T ki = kinv[idx];
+    const T kn[3] = {k[0] * ki, k[1] * ki, k[2] * ki};
+    // sigma k, projection, Oseen factor, times i
+    T ur[3], ui[3];
+    {
+        T sr[3], si[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            T ar = T(0), ai = T(0);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int q = f.nsig == 6 ? (i <= j ? 3 * i + j - (i * (i + 1)) / 2
+                                                    : 3 * j + i - (j * (j + 1)) / 2)
+                                          : 3 * i + j;
+                const cplx<T> s = in[(size_t)q * ns + idx];
+                ar = ar + s.x * k[j];
+                ai = ai + s.y * k[j];
+            }
+            sr[i] = ar;
+            si[i] = ai;
+        }
+        T kr = (kn[0] * sr[0] + kn[1] * sr[1]) + kn[2] * sr[2];
+        T kim = (kn[0] * si[0] + kn[1] * si[1]) + kn[2] * si[2];
+        const T fac = (ik2[idx] + screening) / norm;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            T ar = (sr[i] - kn[i] * kr) * fac;
+            T ai = (si[i] - kn[i] * kim) * fac;
+            ur[i] = -ai;
+            ui[i] = ar;
+        }
+    }
+    int o = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) out[(size_t)(o + i) * ns + idx] = cplx<T>{ur[i], ui[i]};
+    o += 3;
+    // G_ab = i k_a u_b
+    auto G = [&](int a, int b) { return cplx<T>{-(k[a] * ui[b]), k[a] * ur[b]}; };
+    if (f.full) {
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) out[(size_t)(o + 3 * a + b) * ns + idx] = G(a, b);
+        o += 9;
+    } else {
+        const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+#pragma unroll
+        for (int w = 0; w < 3; ++w) {
+            cplx<T> gab = G(pa[w], pb[w]), gba = G(pb[w], pa[w]);
+            out[(size_t)(o + w) * ns + idx] = cplx<T>{(gab.x - gba.x) * T(0.5), (gab.y - gba.y) * T(0.5)};
+        }
+        o += 3;
+        if (f.want_e) {
+            const int ea[6] = {0, 0, 0, 1, 1, 2}, eb[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+            for (int w = 0; w < 6; ++w) {
+                cplx<T> gab = G(ea[w], eb[w]), gba = G(eb[w], ea[w]);
+                out[(size_t)(o + w) * ns + idx] =
+                    cplx<T>{(gab.x + gba.x) * T(0.5), (gab.y + gba.y) * T(0.5)};
+            }
+            o += 6;
+        }
+    }
+    if (f.mag && (f.full || f.want_b || f.want_gb)) {
+        // b = (P - kn (kn . P)) / N, k = 0 kept (magnetic_solver.rs:88-94)
+        T br[3], bi[3];
+        const cplx<T>* P = in + (size_t)f.nsig * ns;
+        T kr = T(0), kim = T(0);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const cplx<T> pq = P[(size_t)q * ns + idx];
+            kr = kr + kn[q] * pq.x;
+            kim = kim + kn[q] * pq.y;
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const cplx<T> pq = P[(size_t)q * ns + idx];
+            br[q] = (pq.x - kn[q] * kr) / norm;
+            bi[q] = (pq.y - kn[q] * kim) / norm;
+        }
+        if (f.full || f.want_b) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) out[(size_t)(o + q) * ns + idx] = cplx<T>{br[q], bi[q]};
+            o += 3;
+        }
+        if (f.full || f.want_gb) {
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b)
+                    out[(size_t)(o + 3 * a + b) * ns + idx] = cplx<T>{-(k[a] * bi[b]), k[a] * br[b]};
+            o += 9;
+        }
+    }
+}
+
+// Real parts of the inverse-transformed fields -> the per-cell record read by
+// the update (REC_* offsets), B pre-multiplied by magnetic_dipole_dipole like
+// src/bin/simulation/mod.rs:337-338.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_pack(const cplx<T>* __restrict__ out, const T* __restrict__ rho, int ns, int rs, SolveFlags f,
+           T mu_dd, T* __restrict__ rec) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ns) return;
+    T* r = rec + (size_t)c * rs;
+    int o = 0;
+    for (int q = 0; q < 6; ++q) r[q] = out[(size_t)q * ns + c].x;
+    o = 6;
+    if (f.want_e) {
+        for (int q = 0; q < 6; ++q) r[REC_E + q] = out[(size_t)(o + q) * ns + c].x;
+        o += 6;
+    }
+    if (rho) r[REC_RHO] = rho[c];
+    if (f.want_b) {
+        for (int q = 0; q < 3; ++q) r[REC_B + q] = out[(size_t)(o + q) * ns + c].x * mu_dd;
+        o += 3;
+    }
+    if (f.want_gb) {
+        for (int q = 0; q < 9; ++q) r[REC_GB + q] = out[(size_t)(o + q) * ns + c].x;
+        o += 9;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Particle state (structure of arrays).  lo* hold the low-order parts of the
+// FP32 positions (true position = x + lo), nullptr in FP64.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct State {
+    T *x, *y, *z, *phi, *theta;
+    float *lx, *ly, *lz;
+    uint32_t* id;  // original index (sorted path); nullptr: slot == index
+};
+
+template <typename T>
+__device__ inline T add_wrap(T x, T d, T L, float* lo) {
+    // FP64: Position::pbc after pos + delta (particle.rs:72-76)
+    return modulo(x + d, L);
+}
+
+template <>
+__device__ inline float add_wrap<float>(float x, float d, float L, float* lo) {
+    // Compensated (Kahan / Fast2Sum) increment, then the periodic wrap with the
+    // rounding error of the wrap folded into the low part.
+    float y = d + *lo;
+    float t = x + y;
+    float bp = t - x;
+    float l = (x - (t - bp)) + (y - bp);  // TwoSum: t + l == x + y exactly
+    if (t >= L) {
+        t = t - L;
+    } else if (t < 0.0f) {
+        float s = t + L;
+        l = l + (t - (s - L));
+        t = s;
+    }
+    *lo = l;
+    return t;
+}
+
+// Update of every particle: gather the cell record, draw the noise, run the
+// LangevinBuilder chain, move, wrap.  With SORTED, also emit the new
+// (cell, angle) key and count particles per new cell.
+template <typename T, bool SORTED, bool EXT_NOISE>
+__global__ void __launch_bounds__(256)
+    k_update(State<T> s, uint64_t n, Geo<T> g, UpdateParams<T> P, const T* __restrict__ rec, int rs,
+             uint64_t seed, uint64_t step, const T* __restrict__ noise, uint32_t* __restrict__ key,
+             uint32_t* __restrict__ cnt) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    uint32_t newcell = INVALID_KEY;
+    if (valid) {
+        const T x = s.x[i], y = s.y[i], z = s.z[i];
+        const uint32_t id = s.id ? s.id[i] : (uint32_t)i;
+        const int c = cell_of(g, x, y, z);
+        T rv[5];
+        if (EXT_NOISE) {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) rv[k] = noise[(size_t)k * n + id];
+        } else {
+            philox_noise(seed, step, id, P.dr, rv);
+        }
+        T dp[3], ophi, otheta;
+        langevin<T>(P, s.phi[i], s.theta[i], rec + (size_t)c * rs, rv, dp, ophi, otheta);
+        T nx, ny, nz;
+        if constexpr (sizeof(T) == 4) {
+            float lx = s.lx[i], ly = s.ly[i], lz = s.lz[i];
+            nx = add_wrap<T>(x, dp[0], P.box[0], &lx);
+            ny = add_wrap<T>(y, dp[1], P.box[1], &ly);
+            nz = add_wrap<T>(z, dp[2], P.box[2], &lz);
+            if (P.quasi2d) {
+                nz = T(0);
+                lz = 0.0f;
+            }
+            s.lx[i] = lx;
+            s.ly[i] = ly;
+            s.lz[i] = lz;
+        } else {
+            nx = add_wrap<T>(x, dp[0], P.box[0], nullptr);
+            ny = add_wrap<T>(y, dp[1], P.box[1], nullptr);
+            nz = add_wrap<T>(z, dp[2], P.box[2], nullptr);
+            if (P.quasi2d) nz = T(0);
+        }
+        s.x[i] = nx;
+        s.y[i] = ny;
+        s.z[i] = nz;
+        s.phi[i] = ophi;
+        s.theta[i] = otheta;
+        if (SORTED) {
+            newcell = (uint32_t)cell_of(g, nx, ny, nz);
+            key[i] = newcell * (uint32_t)g.na + (uint32_t)angle_of(g, ophi, otheta);
+        }
+    }
+    if (SORTED) {
+        Run r = wave_run(newcell);
+        if (valid && r.is_head) atomicAdd(cnt + newcell, (uint32_t)r.len);
+    }
+}
+
+// Keys and per-cell counts of the current state (initial sort).
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_keys(State<T> s, uint64_t n, Geo<T> g, uint32_t* __restrict__ key, uint32_t* __restrict__ cnt) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t cell = INVALID_KEY;
+    if (i < n) {
+        cell = (uint32_t)cell_of(g, s.x[i], s.y[i], s.z[i]);
+        key[i] = cell * (uint32_t)g.na + (uint32_t)angle_of(g, s.phi[i], s.theta[i]);
+    }
+    Run r = wave_run(cell);
+    if (i < n && r.is_head) atomicAdd(cnt + cell, (uint32_t)r.len);
+}
+
+// Exclusive scan of the per-cell counts, two kernels: chunk sums, then chunk
+// scans offset by the sum of the preceding chunks.  The second kernel also
+// clears the counts and the scatter cursors for the next step.
+constexpr int SCAN_CHUNK = 1024;
+
+__global__ void __launch_bounds__(256) k_scan_reduce(const uint32_t* __restrict__ cnt, int ns,
+                                                     uint32_t* __restrict__ part) {
+    __shared__ uint32_t red[256];
+    const int base = blockIdx.x * SCAN_CHUNK;
+    uint32_t acc = 0;
+    for (int t = threadIdx.x; t < SCAN_CHUNK; t += 256)
+        if (base + t < ns) acc += cnt[base + t];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256)
+    k_scan_apply(uint32_t* __restrict__ cnt, int ns, const uint32_t* __restrict__ part, int nparts,
+                 uint32_t* __restrict__ off, uint32_t* __restrict__ cursor) {
+    __shared__ uint32_t red[256];
+    __shared__ uint32_t sc[256];
+    // prefix of the preceding chunks
+    uint32_t acc = 0;
+    for (int t = threadIdx.x; t < (int)blockIdx.x; t += 256) acc += part[t];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    const uint32_t prefix = red[0];
+    // each thread owns 4 consecutive entries
+    const int base = blockIdx.x * SCAN_CHUNK + threadIdx.x * 4;
+    uint32_t v[4];
+    uint32_t tsum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v[k] = (base + k < ns) ? cnt[base + k] : 0u;
+        tsum += v[k];
+    }
+    sc[threadIdx.x] = tsum;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        uint32_t add = threadIdx.x >= (unsigned)o ? sc[threadIdx.x - o] : 0u;
+        __syncthreads();
+        sc[threadIdx.x] += add;
+        __syncthreads();
+    }
+    uint32_t run = prefix + sc[threadIdx.x] - tsum;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < ns) {
+            off[base + k] = run;
+            cnt[base + k] = 0u;
+            cursor[base + k] = 0u;
+        }
+        run += v[k];
+    }
+    if (blockIdx.x == nparts - 1 && threadIdx.x == 255) off[ns] = run;
+}
+
+// Scatter every particle to its slot in cell order: runs of equal cells
+// within a wavefront reserve contiguous slots with one atomic.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_permute(State<T> a, State<T> b, uint64_t n, int na, const uint32_t* __restrict__ key,
+              const uint32_t* __restrict__ off, uint32_t* __restrict__ cursor, uint16_t* __restrict__ ab) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    uint32_t k = valid ? key[i] : INVALID_KEY;
+    uint32_t cell = valid ? k / (uint32_t)na : INVALID_KEY;
+    Run r = wave_run(cell);
+    uint32_t base = 0;
+    if (valid && r.is_head) base = atomicAdd(cursor + cell, (uint32_t)r.len);
+    base = __shfl(base, r.head);
+    if (!valid) return;
+    const uint32_t dst = off[cell] + base + (uint32_t)(__lane_id() - r.head);
+    b.x[dst] = a.x[i];
+    b.y[dst] = a.y[i];
+    b.z[dst] = a.z[i];
+    b.phi[dst] = a.phi[i];
+    b.theta[dst] = a.theta[i];
+    if (a.lx) {
+        b.lx[dst] = a.lx[i];
+        b.ly[dst] = a.ly[i];
+        b.lz[dst] = a.lz[i];
+    }
+    b.id[dst] = a.id ? a.id[i] : (uint32_t)i;
+    ab[dst] = (uint16_t)(k - cell * (uint32_t)na);
+}
+
+// ---------------------------------------------------------------------------
+// Import / export between caller AoS doubles and the device state.
+// ---------------------------------------------------------------------------
+// Particle::new (particle.rs:186-201): modulo and ang_pbc, twice, in FP64.
+__device__ inline void sanitize(const double* in, const double* box, double* out) {
+    double a0, a1, b0, b1;
+    out[0] = modulo(in[0], box[0]);
+    out[1] = modulo(in[1], box[1]);
+    out[2] = modulo(in[2], box[2]);
+    ang_pbc(in[3], in[4], a0, a1);
+    out[0] = modulo(out[0], box[0]);
+    out[1] = modulo(out[1], box[1]);
+    out[2] = modulo(out[2], box[2]);
+    ang_pbc(a0, a1, b0, b1);
+    out[3] = b0;
+    out[4] = b1;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_import(const double* __restrict__ aos, const double* __restrict__ comp, uint64_t n, State<T> s,
+             double bx, double by, double bz, int do_sanitize) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double p[5];
+    const double box[3] = {bx, by, bz};
+    if (do_sanitize)
+        sanitize(aos + 5 * i, box, p);
+    else
+        for (int k = 0; k < 5; ++k) p[k] = aos[5 * i + k];
+    T* dst[5] = {s.x, s.y, s.z, s.phi, s.theta};
+    for (int k = 0; k < 5; ++k) dst[k][i] = (T)p[k];
+    if (s.lx) {
+        float* lo[3] = {s.lx, s.ly, s.lz};
+        for (int k = 0; k < 3; ++k) {
+            float hi = (float)p[k];
+            lo[k][i] = comp ? (float)comp[3 * i + k] : (float)(p[k] - (double)hi);
+        }
+    }
+    if (s.id) s.id[i] = (uint32_t)i;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_export(State<T> s, uint64_t n, double* __restrict__ aos, double* __restrict__ comp) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t j = s.id ? s.id[i] : i;
+    const T* src[5] = {s.x, s.y, s.z, s.phi, s.theta};
+    for (int k = 0; k < 5; ++k) aos[5 * j + k] = (double)src[k][i];
+    if (s.lx) {
+        const float* lo[3] = {s.lx, s.ly, s.lz};
+        if (comp) {
+            for (int k = 0; k < 3; ++k) comp[3 * j + k] = (double)lo[k][i];
+        } else {
+            for (int k = 0; k < 3; ++k) aos[5 * j + k] = (double)src[k][i] + (double)lo[k][i];
+        }
+    } else if (comp) {
+        for (int k = 0; k < 3; ++k) comp[3 * j + k] = 0.0;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_import_noise(const double* __restrict__ aos, uint64_t n,
+                                                      T* __restrict__ noise) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int k = 0; k < 5; ++k) noise[(size_t)k * n + i] = (T)aos[5 * i + k];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_real_parts(const cplx<T>* __restrict__ c, size_t cnt,
+                                                    double* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) out[i] = (double)c[i].x;
+}
+
+// Contraction of a caller-given distribution (values, not counts) for the
+// solver entry points ss_flow_solve / ss_magnetic_solve.
+template <typename T, int NM>
+__global__ void __launch_bounds__(256)
+    k_contract_dense(const double* __restrict__ dist, Geo<T> g, const T* __restrict__ kern_g, T measure,
+                     cplx<T>* __restrict__ spec) {
+    const int wave = threadIdx.x >> 6;
+    const int cell = blockIdx.x * 4 + wave;
+    if (cell >= g.ns) return;
+    const int lane = __lane_id();
+    T acc[NM];
+    for (int q = 0; q < NM; ++q) acc[q] = T(0);
+    for (int a = lane; a < g.na; a += 64) {
+        const T f = (T)dist[(size_t)cell * g.na + a];
+        for (int q = 0; q < NM; ++q) acc[q] += kern_g[q * g.na + a] * f;
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int q = 0; q < NM; ++q) acc[q] += __shfl_xor(acc[q], off);
+    if (lane == 0)
+        for (int q = 0; q < NM; ++q) spec[(size_t)q * g.ns + cell] = cplx<T>{acc[q] * measure, T(0)};
+}
+
+template <typename T>
+__global__ void k_coord_to_grid(const double* __restrict__ aos, uint64_t n, Geo<T> g,
+                                int32_t* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const T* dummy = nullptr;
+    (void)dummy;
+    out[5 * i + 0] = grid_index((T)aos[5 * i + 0], g.gx, g.nx);
+    out[5 * i + 1] = grid_index((T)aos[5 * i + 1], g.gy, g.ny);
+    out[5 * i + 2] = grid_index((T)aos[5 * i + 2], g.gz, g.nz);
+    out[5 * i + 3] = grid_index((T)aos[5 * i + 3], g.gphi, g.np);
+    out[5 * i + 4] = grid_index((T)aos[5 * i + 4], g.gtheta, g.nt);
+}
+
+// Debug update of n independent particles with explicit per-particle fields
+// (u[3] W[9] E[9] rho B[3] gradB[9]) and noise.
+template <typename T>
+__global__ void k_debug_update(const double* __restrict__ parts, const double* __restrict__ fields,
+                               const double* __restrict__ rvs, uint64_t n, UpdateParams<T> P, T mu_dd,
+                               double* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* f = fields + 34 * i;
+    T rec[REC_FULL];
+    for (int q = 0; q < REC_FULL; ++q) rec[q] = T(0);
+    for (int q = 0; q < 3; ++q) rec[REC_U + q] = (T)f[q];
+    rec[REC_W + 0] = (T)f[3 + 1];
+    rec[REC_W + 1] = (T)f[3 + 2];
+    rec[REC_W + 2] = (T)f[3 + 5];
+    rec[REC_E + 0] = (T)f[12 + 0];
+    rec[REC_E + 1] = (T)f[12 + 1];
+    rec[REC_E + 2] = (T)f[12 + 2];
+    rec[REC_E + 3] = (T)f[12 + 4];
+    rec[REC_E + 4] = (T)f[12 + 5];
+    rec[REC_E + 5] = (T)f[12 + 8];
+    rec[REC_RHO] = (T)f[21];
+    for (int q = 0; q < 3; ++q) rec[REC_B + q] = (T)f[22 + q] * mu_dd;
+    for (int q = 0; q < 9; ++q) rec[REC_GB + q] = (T)f[25 + q];
+    T rv[5];
+    for (int k = 0; k < 5; ++k) rv[k] = (T)rvs[5 * i + k];
+    const double* p = parts + 5 * i;
+    T dp[3], ophi, otheta;
+    langevin<T>(P, (T)p[3], (T)p[4], rec, rv, dp, ophi, otheta);
+    out[5 * i + 0] = (double)modulo((T)p[0] + dp[0], P.box[0]);
+    out[5 * i + 1] = (double)modulo((T)p[1] + dp[1], P.box[1]);
+    out[5 * i + 2] = P.quasi2d ? 0.0 : (double)modulo((T)p[2] + dp[2], P.box[2]);
+    out[5 * i + 3] = (double)ophi;
+    out[5 * i + 4] = (double)otheta;
+}
+
+}  // namespace ss

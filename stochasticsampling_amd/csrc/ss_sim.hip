@@ -1,0 +1,1094 @@
+// ss_sim.hip -- the simulation handle behind the C ABI: device memory, the
+// per-step launch sequence (Simulation::do_timestep,
+// src/bin/simulation/mod.rs:251-374) and the getters.
+//
+// One step:
+//   bin      particles -> per-cell moments (count grid or sorted LDS counts)
+//   fft      forward transform of the moment components
+//   solve    Oseen projection, gradient, W/E (and magnetic field) in k-space
+//   fft      inverse transform of the fields the update reads
+//   pack     real parts -> one record per cell
+//   update   gather + Langevin chain + move (+ new keys, sorted path)
+//   sort     scan of the per-cell counts + scatter into cell order (sorted path)
+#include <math.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ss_internal.h"
+#include "ss_kernels.h"
+
+namespace ss {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+static const double PI_D = 3.141592653589793;
+static const double TWOPI_D = 2.0 * 3.141592653589793;
+
+// Host copies of the reference's mesh helpers, evaluated in FP64 exactly as the
+// reference does, then rounded to the working precision.
+static void k_sampling(int n, double L, std::vector<double>& out) {
+    out.resize(n);
+    int b = (n % 2 == 0) ? n / 2 : n / 2 + 1;
+    double step = TWOPI_D / L;  // fft_helper.rs:26-51
+    for (int i = 0; i < b; ++i) out[i] = (double)i * step;
+    for (int i = b; i < n; ++i) out[i] = (double)(i - n) * step;
+}
+
+static void linspace(double a, double b, int n, std::vector<double>& out) {
+    out.resize(n);
+    double step = n > 1 ? (b - a) / ((double)n - 1.0) : 0.0;
+    for (int i = 0; i < n; ++i) out[i] = a + step * (double)i;
+}
+
+struct GW {
+    double x, y, z, phi, theta;
+};
+static GW grid_width(const ss_params& p) {
+    return {p.box_size.x / p.grid_size.x, p.box_size.y / p.grid_size.y, p.box_size.z / p.grid_size.z,
+            TWOPI_D / p.grid_size.phi, PI_D / p.grid_size.theta};
+}
+
+static bool default_direction(const ss_params& p) {
+    return p.field_direction[0] == 0.0 && p.field_direction[1] == 0.0 && p.field_direction[2] == 0.0;
+}
+
+// Moment kernels on the angular bin centres: stress_kernel (stress.rs:29-56,
+// closure src/bin/simulation/mod.rs:100-104) in `nsig` components, then the
+// orientation kernel (director.rs:69-92) when `mag`.  out: [nm][nphi][ntheta].
+static void moment_kernels(const ss_params& p, int nsig, bool mag, std::vector<double>& out) {
+    const int np = p.grid_size.phi, nt = p.grid_size.theta, na = np * nt;
+    GW gw = grid_width(p);
+    std::vector<double> ap, at;
+    linspace(0. + gw.phi / 2., TWOPI_D - gw.phi / 2., np, ap);
+    linspace(0. + gw.theta / 2., PI_D - gw.theta / 2., nt, at);
+    double d[3] = {p.field_direction[0], p.field_direction[1], p.field_direction[2]};
+    const bool ydir = default_direction(p);
+    if (ydir) {
+        d[0] = 0.0;
+        d[1] = 1.0;
+        d[2] = 0.0;
+    }
+    const int nm = nsig + (mag ? 3 : 0);
+    out.assign((size_t)nm * na, 0.0);
+    for (int i = 0; i < np; ++i)
+        for (int j = 0; j < nt; ++j) {
+            const double phi = ap[i], th = at[j];
+            const double cp = cos(phi), sp = sin(phi), ct = cos(th), st = sin(th);
+            double sa[9] = {-(1. / 3.) + cp * cp * st * st, cp * st * st * sp, ct * cp * st,
+                            cp * st * st * sp, -(1. / 3.) + st * st * sp * sp, ct * st * sp,
+                            ct * cp * st, ct * st * sp, -(1. / 3.) + ct * ct};
+            double sm[9];
+            if (ydir) {
+                const double t[9] = {0., cp * st, 0., -cp * st, 0., -ct, 0., ct, 0.};
+                for (int q = 0; q < 9; ++q) sm[q] = t[q] * 0.5;
+            } else {
+                const double n[3] = {st * cp, st * sp, ct};
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) sm[3 * a + b] = (n[a] * d[b] - d[a] * n[b]) * 0.5;
+            }
+            double sr[9] = {2. * ct * (cp * cp) * (st * st), ct * (st * st) * cos(2. * phi),
+                            cos(2. * th) * cp * st, ct * (st * st) * cos(2. * phi),
+                            st * sin(2. * th) * (sp * sp), cos(2. * th) * st * sp,
+                            cos(2. * th) * cp * st, cos(2. * th) * st * sp, -2. * ct * (st * st)};
+            for (int q = 0; q < 9; ++q) sr[q] *= 0.5;
+            double K[9];
+            for (int q = 0; q < 9; ++q)
+                K[q] = p.stress_active * sa[q] + p.stress_magnetic * sm[q] + p.shape * sr[q];
+            const int a = i * nt + j;
+            if (nsig == 6) {
+                const int qs[6] = {0, 1, 2, 4, 5, 8};
+                for (int q = 0; q < 6; ++q) out[(size_t)q * na + a] = K[qs[q]];
+            } else {
+                for (int q = 0; q < 9; ++q) out[(size_t)q * na + a] = K[q];
+            }
+            if (mag) {
+                // Orientation::new applies ang_pbc to the bin centre first
+                double ph = fmod(phi, TWOPI_D), tt = fmod(th, TWOPI_D);
+                if (ph < 0) ph += TWOPI_D;
+                if (tt < 0) tt += TWOPI_D;
+                if (tt > PI_D) {
+                    ph = fmod(ph + PI_D, TWOPI_D);
+                    tt = TWOPI_D - tt;
+                }
+                out[(size_t)(nsig + 0) * na + a] = sin(tt) * cos(ph);
+                out[(size_t)(nsig + 1) * na + a] = sin(tt) * sin(ph);
+                out[(size_t)(nsig + 2) * na + a] = cos(tt);
+            }
+        }
+}
+
+template <typename T>
+static int upload(void** dst, const std::vector<double>& src) {
+    std::vector<T> h(src.size());
+    for (size_t i = 0; i < src.size(); ++i) h[i] = (T)src[i];
+    SS_HIP_CHECK(hipMalloc(dst, std::max<size_t>(1, h.size()) * sizeof(T)));
+    SS_HIP_CHECK(hipMemcpy(*dst, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return SS_OK;
+}
+
+static inline unsigned blocks_for(uint64_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace ss
+
+using namespace ss;
+
+struct ss_sim {
+    ss_params p;
+    int prec = SS_FP64;
+    bool sorted = false;
+    bool mag = false;  // magnetic solve runs
+    bool use_e = false, use_rho = false, use_b = false, use_gb = false;
+    int nsig = 6;
+    int nm = 6;     // moment components
+    int nout = 6;   // hot inverse components
+    int rs = REC_SMALL;
+    uint64_t n = 0;
+    int nx, ny, nz, np, nt, ns, na;
+    size_t esz = 8;
+    uint64_t timestep = 0;
+    bool initialised = false;
+    bool have_noise = false;
+    bool have_fields = false;
+    bool have_hist = false;
+    hipStream_t stream = nullptr;
+
+    // particle state, A and B (B only on the sorted path)
+    void *x = nullptr, *y = nullptr, *z = nullptr, *phi = nullptr, *theta = nullptr;
+    void *x2 = nullptr, *y2 = nullptr, *z2 = nullptr, *phi2 = nullptr, *theta2 = nullptr;
+    float *lx = nullptr, *ly = nullptr, *lz = nullptr, *lx2 = nullptr, *ly2 = nullptr, *lz2 = nullptr;
+    uint32_t *id = nullptr, *id2 = nullptr;
+    uint32_t* key = nullptr;
+    uint16_t* ab = nullptr;
+    uint32_t *cnt = nullptr, *off = nullptr, *cursor = nullptr, *part = nullptr;
+    int nparts = 0;
+    uint32_t* hist = nullptr;
+    void* kern = nullptr;
+    void* spec_in = nullptr;
+    void* spec_out = nullptr;
+    void* spec_full = nullptr;
+    void* rho = nullptr;
+    void* rec = nullptr;
+    void *kx = nullptr, *ky = nullptr, *kz = nullptr, *ik2 = nullptr, *kinv = nullptr;
+    void* noise = nullptr;
+    double* staging = nullptr;  // n x 5 doubles
+    double* staging2 = nullptr; // n x 3 doubles
+    Fft3d fft;
+    std::vector<void*> owned;
+};
+
+namespace ss {
+
+template <typename T>
+static Geo<T> geo(const ss_sim* s) {
+    Geo<T> g;
+    g.nx = s->nx;
+    g.ny = s->ny;
+    g.nz = s->nz;
+    g.np = s->np;
+    g.nt = s->nt;
+    g.ns = s->ns;
+    g.na = s->na;
+    GW gw = grid_width(s->p);
+    g.gx = (T)gw.x;
+    g.gy = (T)gw.y;
+    g.gz = (T)gw.z;
+    g.gphi = (T)gw.phi;
+    g.gtheta = (T)gw.theta;
+    g.box[0] = (T)s->p.box_size.x;
+    g.box[1] = (T)s->p.box_size.y;
+    g.box[2] = (T)s->p.box_size.z;
+    return g;
+}
+
+template <typename T>
+static State<T> state_a(const ss_sim* s) {
+    return {(T*)s->x, (T*)s->y, (T*)s->z, (T*)s->phi, (T*)s->theta, s->lx, s->ly, s->lz, s->id};
+}
+template <typename T>
+static State<T> state_b(const ss_sim* s) {
+    return {(T*)s->x2, (T*)s->y2, (T*)s->z2, (T*)s->phi2, (T*)s->theta2, s->lx2, s->ly2, s->lz2, s->id2};
+}
+
+static void swap_ab(ss_sim* s) {
+    std::swap(s->x, s->x2);
+    std::swap(s->y, s->y2);
+    std::swap(s->z, s->z2);
+    std::swap(s->phi, s->phi2);
+    std::swap(s->theta, s->theta2);
+    std::swap(s->lx, s->lx2);
+    std::swap(s->ly, s->ly2);
+    std::swap(s->lz, s->lz2);
+    std::swap(s->id, s->id2);
+}
+
+template <typename T>
+static UpdateParams<T> update_params(const ss_sim* s) {
+    const ss_params& p = s->p;
+    UpdateParams<T> P;
+    P.dt = (T)p.timestep;
+    P.drag = (T)p.magnetic_drag;
+    P.shape = (T)p.shape;
+    P.vol_excl = (T)p.volume_exclusion;
+    P.kappa = (T)p.magnetic_reorientation;
+    P.default_dir = default_direction(p) ? 1 : 0;
+    for (int k = 0; k < 3; ++k) P.dir[k] = (T)p.field_direction[k];
+    P.d_trans = (T)p.diffusion_translational;
+    P.dr = (T)sqrt(2. * p.diffusion_rotational * p.timestep);
+    P.box[0] = (T)p.box_size.x;
+    P.box[1] = (T)p.box_size.y;
+    P.box[2] = (T)p.box_size.z;
+    P.use_e = s->use_e;
+    P.use_rho = s->use_rho;
+    P.use_b = s->use_b;
+    P.use_gb = s->use_gb;
+    P.quasi2d = (p.flags & SS_FLAG_QUASI2D) ? 1 : 0;
+    return P;
+}
+
+static SolveFlags solve_flags(const ss_sim* s, bool full) {
+    SolveFlags f;
+    f.nsig = s->nsig;
+    f.mag = s->mag ? 1 : 0;
+    f.full = full ? 1 : 0;
+    f.want_e = s->use_e;
+    f.want_b = s->use_b;
+    f.want_gb = s->use_gb;
+    return f;
+}
+
+// Scale factors: f = count / (gx gy gz gphi gtheta N) * (Lx Ly Lz)
+// (distribution/mod.rs:183, src/bin/simulation/mod.rs:255-257), then
+// measure = gphi gtheta (stress.rs:83) or dthph = gtheta gphi (mod.rs:315).
+static void scales(const ss_sim* s, double& moment, double& density) {
+    GW gw = grid_width(s->p);
+    double den = gw.x * gw.y * gw.z * gw.phi * gw.theta * (double)s->n;
+    double vol = s->p.box_size.x * s->p.box_size.y * s->p.box_size.z;
+    moment = vol / den * (gw.phi * gw.theta);
+    density = vol / den * (gw.theta * gw.phi);
+}
+
+template <typename T>
+static hipError_t launch_bin(ss_sim* s) {
+    Geo<T> g = geo<T>(s);
+    double ms, ds;
+    scales(s, ms, ds);
+    T* rho = s->use_rho ? (T*)s->rho : nullptr;
+    if (!s->sorted) {
+        (void)hipMemsetAsync(s->hist, 0, (size_t)s->ns * s->na * sizeof(uint32_t), s->stream);
+        hipLaunchKernelGGL(k_bin_hist<T>, dim3(blocks_for(s->n)), dim3(256), 0, s->stream, (const T*)s->x,
+                           (const T*)s->y, (const T*)s->z, (const T*)s->phi, (const T*)s->theta, s->n, g,
+                           s->hist);
+        const unsigned nb = (unsigned)((s->ns + 63) / 64);
+        const size_t lds = (size_t)s->nm * s->na * sizeof(T);
+        if (s->nm == 6)
+            hipLaunchKernelGGL((k_contract_hist<T, 6>), dim3(nb), dim3(256), lds, s->stream, s->hist, g,
+                               (const T*)s->kern, (T)ms, (T)ds, (cplx<T>*)s->spec_in, rho);
+        else if (s->nm == 9)
+            hipLaunchKernelGGL((k_contract_hist<T, 9>), dim3(nb), dim3(256), lds, s->stream, s->hist, g,
+                               (const T*)s->kern, (T)ms, (T)ds, (cplx<T>*)s->spec_in, rho);
+        else
+            hipLaunchKernelGGL((k_contract_hist<T, 12>), dim3(nb), dim3(256), lds, s->stream, s->hist, g,
+                               (const T*)s->kern, (T)ms, (T)ds, (cplx<T>*)s->spec_in, rho);
+    } else {
+        constexpr int CB = 16;
+        const unsigned nb = (unsigned)((s->ns + CB - 1) / CB);
+        const size_t lds = (size_t)s->nm * s->na * sizeof(T) + (size_t)CB * s->na * sizeof(uint32_t);
+        uint32_t* hout = (s->p.flags & SS_FLAG_KEEP_DISTRIBUTION) ? s->hist : nullptr;
+        if (s->nm == 6)
+            hipLaunchKernelGGL((k_bin_sorted<T, 6, CB>), dim3(nb), dim3(256), lds, s->stream, s->ab, s->off,
+                               g, (const T*)s->kern, (T)ms, (T)ds, (cplx<T>*)s->spec_in, rho, hout);
+        else if (s->nm == 9)
+            hipLaunchKernelGGL((k_bin_sorted<T, 9, CB>), dim3(nb), dim3(256), lds, s->stream, s->ab, s->off,
+                               g, (const T*)s->kern, (T)ms, (T)ds, (cplx<T>*)s->spec_in, rho, hout);
+        else
+            hipLaunchKernelGGL((k_bin_sorted<T, 12, CB>), dim3(nb), dim3(256), lds, s->stream, s->ab,
+                               s->off, g, (const T*)s->kern, (T)ms, (T)ds, (cplx<T>*)s->spec_in, rho, hout);
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_solve(ss_sim* s, bool full, void* out) {
+    Geo<T> g = geo<T>(s);
+    hipLaunchKernelGGL(k_solve<T>, dim3(blocks_for(s->ns)), dim3(256), 0, s->stream,
+                       (const cplx<T>*)s->spec_in, (cplx<T>*)out, g, (const T*)s->kx, (const T*)s->ky,
+                       (const T*)s->kz, (const T*)s->ik2, (const T*)s->kinv, (T)s->p.hydro_screening,
+                       (T)(double)s->ns, solve_flags(s, full));
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_pack(ss_sim* s) {
+    hipLaunchKernelGGL(k_pack<T>, dim3(blocks_for(s->ns)), dim3(256), 0, s->stream,
+                       (const cplx<T>*)s->spec_out, s->use_rho ? (const T*)s->rho : nullptr, s->ns, s->rs,
+                       solve_flags(s, false), (T)s->p.magnetic_dipole_dipole, (T*)s->rec);
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_update(ss_sim* s) {
+    Geo<T> g = geo<T>(s);
+    UpdateParams<T> P = update_params<T>(s);
+    const bool ext = s->p.noise_mode == SS_NOISE_EXTERNAL;
+    State<T> st = state_a<T>(s);
+    dim3 grid(blocks_for(s->n));
+#define SS_UPD(SORTED, EXT)                                                                          \
+    hipLaunchKernelGGL((k_update<T, SORTED, EXT>), grid, dim3(256), 0, s->stream, st, s->n, g, P,     \
+                       (const T*)s->rec, s->rs, s->p.seed, s->timestep, (const T*)s->noise, s->key, s->cnt)
+    if (s->sorted) {
+        if (ext)
+            SS_UPD(true, true);
+        else
+            SS_UPD(true, false);
+    } else {
+        if (ext)
+            SS_UPD(false, true);
+        else
+            SS_UPD(false, false);
+    }
+#undef SS_UPD
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_sort(ss_sim* s) {
+    hipLaunchKernelGGL(k_scan_reduce, dim3(s->nparts), dim3(256), 0, s->stream, s->cnt, s->ns, s->part);
+    hipLaunchKernelGGL(k_scan_apply, dim3(s->nparts), dim3(256), 0, s->stream, s->cnt, s->ns, s->part,
+                       s->nparts, s->off, s->cursor);
+    hipLaunchKernelGGL(k_permute<T>, dim3(blocks_for(s->n)), dim3(256), 0, s->stream, state_a<T>(s),
+                       state_b<T>(s), s->n, s->na, s->key, s->off, s->cursor, s->ab);
+    hipError_t e = hipGetLastError();
+    swap_ab(s);
+    return e;
+}
+
+template <typename T>
+static hipError_t launch_keys(ss_sim* s) {
+    hipLaunchKernelGGL(k_keys<T>, dim3(blocks_for(s->n)), dim3(256), 0, s->stream, state_a<T>(s), s->n,
+                       geo<T>(s), s->key, s->cnt);
+    return hipGetLastError();
+}
+
+enum Stage { ST_BIN = 0, ST_FWD, ST_SOLVE, ST_INV, ST_PACK, ST_UPDATE, ST_SORT, ST_COUNT };
+static const char* STAGE_NAMES[ST_COUNT] = {"bin", "fft_forward", "solve", "fft_inverse",
+                                            "pack", "update", "sort"};
+
+// One do_timestep.  ev (may be null): events recorded after each stage.
+template <typename T>
+static int step_once(ss_sim* s, hipEvent_t* ev) {
+    hipError_t e;
+    auto mark = [&](int st) {
+        if (ev) (void)hipEventRecord(ev[st + 1], s->stream);
+    };
+    if (ev) (void)hipEventRecord(ev[0], s->stream);
+    if ((e = launch_bin<T>(s)) != hipSuccess) return fail(SS_ERR_DEVICE, std::string("bin: ") + hipGetErrorString(e));
+    mark(ST_BIN);
+    int rc = s->fft.run(s->spec_in, s->nm, -1, s->stream);
+    if (rc != SS_OK) return rc;
+    mark(ST_FWD);
+    if ((e = launch_solve<T>(s, false, s->spec_out)) != hipSuccess)
+        return fail(SS_ERR_DEVICE, std::string("solve: ") + hipGetErrorString(e));
+    mark(ST_SOLVE);
+    rc = s->fft.run(s->spec_out, s->nout, +1, s->stream);
+    if (rc != SS_OK) return rc;
+    mark(ST_INV);
+    if ((e = launch_pack<T>(s)) != hipSuccess) return fail(SS_ERR_DEVICE, std::string("pack: ") + hipGetErrorString(e));
+    mark(ST_PACK);
+    if ((e = launch_update<T>(s)) != hipSuccess)
+        return fail(SS_ERR_DEVICE, std::string("update: ") + hipGetErrorString(e));
+    mark(ST_UPDATE);
+    if (s->sorted) {
+        if ((e = launch_sort<T>(s)) != hipSuccess)
+            return fail(SS_ERR_DEVICE, std::string("sort: ") + hipGetErrorString(e));
+    }
+    mark(ST_SORT);
+    s->timestep += 1;
+    s->have_fields = true;
+    s->have_hist = !s->sorted || (s->p.flags & SS_FLAG_KEEP_DISTRIBUTION);
+    return SS_OK;
+}
+
+// Sample the distribution of the current particles (Simulation::init,
+// src/bin/simulation/mod.rs:190-195): sorts on the sorted path and fills the
+// count grid where one is kept.
+template <typename T>
+static int initial_binning(ss_sim* s) {
+    hipError_t e;
+    if (s->sorted) {
+        if ((e = hipMemsetAsync(s->cnt, 0, sizeof(uint32_t) * s->ns, s->stream)) != hipSuccess)
+            return fail(SS_ERR_DEVICE, "memset");
+        if ((e = launch_keys<T>(s)) != hipSuccess) return fail(SS_ERR_DEVICE, hipGetErrorString(e));
+        if ((e = launch_sort<T>(s)) != hipSuccess) return fail(SS_ERR_DEVICE, hipGetErrorString(e));
+    }
+    if ((e = launch_bin<T>(s)) != hipSuccess) return fail(SS_ERR_DEVICE, hipGetErrorString(e));
+    s->have_hist = !s->sorted || (s->p.flags & SS_FLAG_KEEP_DISTRIBUTION);
+    return SS_OK;
+}
+
+template <typename T>
+static int import_particles(ss_sim* s, const double* parts, const double* comp, bool sanitize) {
+    SS_HIP_CHECK(hipMemcpyAsync(s->staging, parts, sizeof(double) * 5 * s->n, hipMemcpyHostToDevice, s->stream));
+    if (comp)
+        SS_HIP_CHECK(hipMemcpyAsync(s->staging2, comp, sizeof(double) * 3 * s->n, hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_import<T>, dim3(blocks_for(s->n)), dim3(256), 0, s->stream, s->staging,
+                       comp ? s->staging2 : nullptr, s->n, state_a<T>(s), s->p.box_size.x, s->p.box_size.y,
+                       s->p.box_size.z, sanitize ? 1 : 0);
+    SS_HIP_CHECK(hipGetLastError());
+    return initial_binning<T>(s);
+}
+
+template <typename T>
+static int export_particles(ss_sim* s, double* out, double* comp, uint64_t n) {
+    hipLaunchKernelGGL(k_export<T>, dim3(blocks_for(s->n)), dim3(256), 0, s->stream, state_a<T>(s), s->n,
+                       s->staging, comp ? s->staging2 : nullptr);
+    SS_HIP_CHECK(hipGetLastError());
+    if (out) SS_HIP_CHECK(hipMemcpyAsync(out, s->staging, sizeof(double) * 5 * n, hipMemcpyDeviceToHost, s->stream));
+    if (comp) SS_HIP_CHECK(hipMemcpyAsync(comp, s->staging2, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s->stream));
+    SS_HIP_CHECK(hipStreamSynchronize(s->stream));
+    return SS_OK;
+}
+
+// Full (non-hot) solve from the retained forward-transformed moments into
+// spec_full: u(3), G(9), [b(3), gradB(9)], inverse transformed.
+template <typename T>
+static int full_fields(ss_sim* s) {
+    hipError_t e = launch_solve<T>(s, true, s->spec_full);
+    if (e != hipSuccess) return fail(SS_ERR_DEVICE, hipGetErrorString(e));
+    const int nfull = 12 + (s->mag ? 12 : 0);
+    return s->fft.run(s->spec_full, nfull, +1, s->stream);
+}
+
+template <typename T>
+static int copy_real(ss_sim* s, const void* spec, int comp0, int ncomp, double* host) {
+    const size_t cnt = (size_t)ncomp * s->ns;
+    double* tmp = s->staging;  // n*5 doubles; sized at create to hold the field copies too
+    hipLaunchKernelGGL(k_real_parts<T>, dim3(blocks_for(cnt)), dim3(256), 0, s->stream,
+                       (const cplx<T>*)spec + (size_t)comp0 * s->ns, cnt, tmp);
+    SS_HIP_CHECK(hipGetLastError());
+    SS_HIP_CHECK(hipMemcpyAsync(host, tmp, sizeof(double) * cnt, hipMemcpyDeviceToHost, s->stream));
+    SS_HIP_CHECK(hipStreamSynchronize(s->stream));
+    return SS_OK;
+}
+
+static int alloc(ss_sim* s, void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 16));
+    if (e != hipSuccess) return fail(SS_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+    s->owned.push_back(*p);
+    return SS_OK;
+}
+
+static void free_all(ss_sim* s) {
+    for (void* p : s->owned) (void)hipFree(p);
+    s->owned.clear();
+    s->fft.release();
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    s->stream = nullptr;
+}
+
+template <typename T>
+static int create_impl(ss_sim* s) {
+    const ss_params& p = s->p;
+    int rc;
+#define A(ptr, bytes)                                               \
+    if ((rc = alloc(s, (void**)&(ptr), (bytes))) != SS_OK) return rc;
+    const size_t n = s->n;
+    A(s->x, n * sizeof(T));
+    A(s->y, n * sizeof(T));
+    A(s->z, n * sizeof(T));
+    A(s->phi, n * sizeof(T));
+    A(s->theta, n * sizeof(T));
+    if (s->prec == SS_FP32) {
+        A(s->lx, n * 4);
+        A(s->ly, n * 4);
+        A(s->lz, n * 4);
+    }
+    if (s->sorted) {
+        A(s->x2, n * sizeof(T));
+        A(s->y2, n * sizeof(T));
+        A(s->z2, n * sizeof(T));
+        A(s->phi2, n * sizeof(T));
+        A(s->theta2, n * sizeof(T));
+        if (s->prec == SS_FP32) {
+            A(s->lx2, n * 4);
+            A(s->ly2, n * 4);
+            A(s->lz2, n * 4);
+        }
+        A(s->id, n * 4);
+        A(s->id2, n * 4);
+        A(s->key, n * 4);
+        A(s->ab, n * 2);
+        A(s->cnt, (size_t)s->ns * 4);
+        A(s->off, ((size_t)s->ns + 1) * 4);
+        A(s->cursor, (size_t)s->ns * 4);
+        s->nparts = (s->ns + SCAN_CHUNK - 1) / SCAN_CHUNK;
+        A(s->part, (size_t)s->nparts * 4);
+        SS_HIP_CHECK(hipMemset(s->cnt, 0, (size_t)s->ns * 4));
+    }
+    if (!s->sorted || (p.flags & SS_FLAG_KEEP_DISTRIBUTION)) A(s->hist, (size_t)s->ns * s->na * 4);
+    const size_t cs = 2 * sizeof(T) * (size_t)s->ns;
+    A(s->spec_in, cs * s->nm);
+    A(s->spec_out, cs * s->nout);
+    A(s->spec_full, cs * (12 + (s->mag ? 12 : 0)));
+    if (s->use_rho) A(s->rho, (size_t)s->ns * sizeof(T));
+    A(s->rec, (size_t)s->ns * s->rs * sizeof(T));
+    SS_HIP_CHECK(hipMemset(s->rec, 0, (size_t)s->ns * s->rs * sizeof(T)));
+    SS_HIP_CHECK(hipMemset(s->spec_in, 0, cs * s->nm));
+    if (p.noise_mode == SS_NOISE_EXTERNAL) A(s->noise, 5 * n * sizeof(T));
+    const size_t stage_doubles = std::max<size_t>(5 * n, (size_t)24 * s->ns);
+    A(s->staging, stage_doubles * sizeof(double));
+    A(s->staging2, 3 * n * sizeof(double));
+#undef A
+    // tables
+    std::vector<double> kxs, kys, kzs;
+    k_sampling(s->nx, p.box_size.x, kxs);
+    k_sampling(s->ny, p.box_size.y, kys);
+    k_sampling(s->nz, p.box_size.z, kzs);
+    std::vector<double> ik2(s->ns), kinv(s->ns);
+    for (int i = 0; i < s->nx; ++i)
+        for (int j = 0; j < s->ny; ++j)
+            for (int k = 0; k < s->nz; ++k) {
+                // get_inverse_norm_squared / get_inverse_norm (fft_helper.rs:79-100)
+                double c = (kxs[i] * kxs[i] + kys[j] * kys[j]) + kzs[k] * kzs[k];
+                size_t idx = ((size_t)i * s->ny + j) * s->nz + k;
+                ik2[idx] = c / (c * c);
+                kinv[idx] = 1.0 / sqrt(c);
+            }
+    ik2[0] = 0.0;
+    kinv[0] = 0.0;
+    if ((rc = upload<T>(&s->kx, kxs)) != SS_OK) return rc;
+    s->owned.push_back(s->kx);
+    if ((rc = upload<T>(&s->ky, kys)) != SS_OK) return rc;
+    s->owned.push_back(s->ky);
+    if ((rc = upload<T>(&s->kz, kzs)) != SS_OK) return rc;
+    s->owned.push_back(s->kz);
+    if ((rc = upload<T>(&s->ik2, ik2)) != SS_OK) return rc;
+    s->owned.push_back(s->ik2);
+    if ((rc = upload<T>(&s->kinv, kinv)) != SS_OK) return rc;
+    s->owned.push_back(s->kinv);
+    std::vector<double> kern;
+    moment_kernels(p, s->nsig, s->mag, kern);
+    if ((rc = upload<T>(&s->kern, kern)) != SS_OK) return rc;
+    s->owned.push_back(s->kern);
+    return s->fft.init(s->nx, s->ny, s->nz, s->prec);
+}
+
+static int validate(const ss_params* p) {
+    if (!p) return fail(SS_ERR_INVALID, "params is NULL");
+    const ss_grid& g = p->grid_size;
+    if (g.x < 1 || g.y < 1 || g.z < 1 || g.phi < 1 || g.theta < 1)
+        return fail(SS_ERR_INVALID, "grid sizes must be >= 1");
+    if (!(p->box_size.x > 0) || !(p->box_size.y > 0) || !(p->box_size.z > 0))
+        return fail(SS_ERR_INVALID, "box sizes must be > 0");
+    if (p->number_of_particles < 1) return fail(SS_ERR_INVALID, "number_of_particles must be >= 1");
+    if (p->number_of_particles >= (1ull << 32)) return fail(SS_ERR_UNSUPPORTED, "at most 2^32-1 particles");
+    if (p->precision != SS_FP64 && p->precision != SS_FP32) return fail(SS_ERR_INVALID, "bad precision");
+    if (p->noise_mode != SS_NOISE_PHILOX && p->noise_mode != SS_NOISE_EXTERNAL)
+        return fail(SS_ERR_INVALID, "bad noise_mode");
+    if (p->binning < SS_BIN_AUTO || p->binning > SS_BIN_SORTED) return fail(SS_ERR_INVALID, "bad binning");
+    if (p->fft_variant != SS_FFT_AUTO && p->fft_variant != SS_FFT_RADIX)
+        return fail(SS_ERR_UNSUPPORTED, "only the radix FFT variant is built");
+    const double ns = (double)g.x * g.y * g.z, na = (double)g.phi * g.theta;
+    if (ns * na >= 4294967295.0) return fail(SS_ERR_UNSUPPORTED, "grid has 2^32 or more bins");
+    if (na > 65535) return fail(SS_ERR_UNSUPPORTED, "at most 65535 angle bins");
+    if ((p->flags & SS_FLAG_QUASI2D) && g.z != 1)
+        return fail(SS_ERR_INVALID, "quasi2d needs grid_size.z == 1");
+    return SS_OK;
+}
+
+}  // namespace ss
+
+// ===========================================================================
+// C ABI (every function below is declared extern "C" in include/ss_api.h)
+// ===========================================================================
+
+void ss_params_default(ss_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->box_size = {1.0, 1.0, 1.0};
+    p->grid_size = {1, 1, 1, 1, 1};
+    p->number_of_particles = 1;
+    p->precision = SS_FP64;
+    p->noise_mode = SS_NOISE_PHILOX;
+    p->binning = SS_BIN_AUTO;
+    p->fft_variant = SS_FFT_AUTO;
+}
+
+const char* ss_last_error(void) { return ss::g_err.c_str(); }
+
+int ss_version(void) { return SS_API_VERSION; }
+
+int ss_device_count(int* count) {
+    if (!count) return fail(SS_ERR_INVALID, "count is NULL");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return SS_OK;
+}
+
+int ss_create(const ss_params* params, ss_sim** out) {
+    if (!out) return fail(SS_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    int rc = validate(params);
+    if (rc != SS_OK) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SS_ERR_DEVICE, "no GPU visible");
+    if (params->device < 0 || params->device >= ndev) return fail(SS_ERR_INVALID, "bad device ordinal");
+    SS_HIP_CHECK(hipSetDevice(params->device));
+    ss_sim* s = new ss_sim();
+    s->p = *params;
+    const ss_params& p = s->p;
+    s->prec = p.precision;
+    s->esz = s->prec == SS_FP64 ? 8 : 4;
+    s->sorted = p.binning == SS_BIN_SORTED || (p.binning == SS_BIN_AUTO && p.precision == SS_FP32);
+    s->n = p.number_of_particles;
+    s->nx = p.grid_size.x;
+    s->ny = p.grid_size.y;
+    s->nz = p.grid_size.z;
+    s->np = p.grid_size.phi;
+    s->nt = p.grid_size.theta;
+    s->ns = s->nx * s->ny * s->nz;
+    s->na = s->np * s->nt;
+    s->use_e = p.shape != 0.0;
+    s->use_rho = p.volume_exclusion != 0.0;
+    s->use_b = p.magnetic_dipole_dipole != 0.0;
+    s->use_gb = p.magnetic_drag != 0.0;
+    s->mag = s->use_b || s->use_gb || (p.flags & SS_FLAG_MAGNETIC_ALWAYS);
+    s->nsig = p.stress_magnetic != 0.0 ? 9 : 6;
+    s->nm = s->nsig + (s->mag ? 3 : 0);
+    s->nout = 6 + (s->use_e ? 6 : 0) + (s->use_b ? 3 : 0) + (s->use_gb ? 9 : 0);
+    s->rs = (s->use_e || s->use_rho || s->use_b || s->use_gb) ? REC_FULL : REC_SMALL;
+    {
+        // LDS of the contraction kernels: the moment kernels (nm x na) plus, on
+        // the sorted path, 16 cells of u32 angle counts; at most 64 KiB.
+        const size_t kern_lds = (size_t)s->nm * s->na * s->esz;
+        const size_t sorted_lds = kern_lds + (size_t)16 * s->na * 4;
+        if (kern_lds > 65536) {
+            delete s;
+            return fail(SS_ERR_UNSUPPORTED, "too many angle bins for the LDS-resident moment kernels");
+        }
+        if (s->sorted && sorted_lds > 65536) {
+            if (p.binning == SS_BIN_SORTED) {
+                delete s;
+                return fail(SS_ERR_UNSUPPORTED, "too many angle bins for the sorted binning kernel");
+            }
+            s->sorted = false;
+        }
+    }
+    if (hipStreamCreate(&s->stream) != hipSuccess) {
+        delete s;
+        return fail(SS_ERR_DEVICE, "hipStreamCreate failed");
+    }
+    rc = s->prec == SS_FP64 ? create_impl<double>(s) : create_impl<float>(s);
+    if (rc != SS_OK) {
+        std::string msg = ss_last_error();
+        free_all(s);
+        delete s;
+        set_error(msg);
+        return rc;
+    }
+    *out = s;
+    return SS_OK;
+}
+
+int ss_destroy(ss_sim* s) {
+    if (!s) return SS_OK;
+    (void)hipSetDevice(s->p.device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    free_all(s);
+    delete s;
+    return SS_OK;
+}
+
+int ss_init(ss_sim* s, const double* particles, uint64_t n) {
+    if (!s || !particles) return fail(SS_ERR_INVALID, "NULL argument");
+    if (n != s->n)
+        return fail(SS_ERR_INVALID, "Given initial condition has not the same number of particles (" +
+                                        std::to_string(n) + ") as given in the parameter file (" +
+                                        std::to_string(s->n) + ").");
+    SS_HIP_CHECK(hipSetDevice(s->p.device));
+    int rc = s->prec == SS_FP64 ? import_particles<double>(s, particles, nullptr, true)
+                                : import_particles<float>(s, particles, nullptr, true);
+    if (rc != SS_OK) return rc;
+    SS_HIP_CHECK(hipStreamSynchronize(s->stream));
+    s->timestep = 0;
+    s->initialised = true;
+    s->have_fields = false;
+    return SS_OK;
+}
+
+int ss_set_noise(ss_sim* s, const double* rv, uint64_t n) {
+    if (!s || !rv) return fail(SS_ERR_INVALID, "NULL argument");
+    if (s->p.noise_mode != SS_NOISE_EXTERNAL) return fail(SS_ERR_STATE, "noise_mode is not SS_NOISE_EXTERNAL");
+    if (n != s->n) return fail(SS_ERR_INVALID, "noise length differs from number_of_particles");
+    SS_HIP_CHECK(hipSetDevice(s->p.device));
+    SS_HIP_CHECK(hipMemcpyAsync(s->staging, rv, sizeof(double) * 5 * n, hipMemcpyHostToDevice, s->stream));
+    if (s->prec == SS_FP64)
+        hipLaunchKernelGGL(k_import_noise<double>, dim3(blocks_for(n)), dim3(256), 0, s->stream, s->staging, n,
+                           (double*)s->noise);
+    else
+        hipLaunchKernelGGL(k_import_noise<float>, dim3(blocks_for(n)), dim3(256), 0, s->stream, s->staging, n,
+                           (float*)s->noise);
+    SS_HIP_CHECK(hipGetLastError());
+    SS_HIP_CHECK(hipStreamSynchronize(s->stream));
+    s->have_noise = true;
+    return SS_OK;
+}
+
+int ss_step(ss_sim* s, uint64_t nsteps, uint64_t* timestep_out) {
+    if (!s) return fail(SS_ERR_INVALID, "NULL handle");
+    if (!s->initialised) return fail(SS_ERR_STATE, "ss_step before ss_init");
+    if (s->p.noise_mode == SS_NOISE_EXTERNAL && !s->have_noise)
+        return fail(SS_ERR_STATE, "SS_NOISE_EXTERNAL needs ss_set_noise before ss_step");
+    SS_HIP_CHECK(hipSetDevice(s->p.device));
+    for (uint64_t k = 0; k < nsteps; ++k) {
+        int rc = s->prec == SS_FP64 ? step_once<double>(s, nullptr) : step_once<float>(s, nullptr);
+        if (rc != SS_OK) return rc;
+    }
+    SS_HIP_CHECK(hipStreamSynchronize(s->stream));
+    if (timestep_out) *timestep_out = s->timestep;
+    return SS_OK;
+}
+
+int ss_get_timestep(const ss_sim* s, uint64_t* t) {
+    if (!s || !t) return fail(SS_ERR_INVALID, "NULL argument");
+    *t = s->timestep;
+    return SS_OK;
+}
+
+int ss_get_particles(ss_sim* s, double* out, uint64_t n) {
+    if (!s || !out) return fail(SS_ERR_INVALID, "NULL argument");
+    if (n > s->n) return fail(SS_ERR_INVALID, "n exceeds number_of_particles");
+    if (!s->initialised) return fail(SS_ERR_STATE, "not initialised");
+    SS_HIP_CHECK(hipSetDevice(s->p.device));
+    return s->prec == SS_FP64 ? export_particles<double>(s, out, nullptr, n)
+                              : export_particles<float>(s, out, nullptr, n);
+}
+
+int ss_get_distribution(ss_sim* s, uint32_t* counts, double* dist) {
+    if (!s) return fail(SS_ERR_INVALID, "NULL handle");
+    if (!s->initialised) return fail(SS_ERR_STATE, "not initialised");
+    if (!s->have_hist || !s->hist)
+        return fail(SS_ERR_STATE, "no count grid kept: the sorted path needs SS_FLAG_KEEP_DISTRIBUTION");
+    SS_HIP_CHECK(hipSetDevice(s->p.device));
+    const size_t nb = (size_t)s->ns * s->na;
+    std::vector<uint32_t> tmp;
+    uint32_t* c = counts;
+    if (!c) {
+        tmp.resize(nb);
+        c = tmp.data();
+    }
+    SS_HIP_CHECK(hipMemcpyAsync(c, s->hist, nb * sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream));
+    SS_HIP_CHECK(hipStreamSynchronize(s->stream));
+    if (dist) {
+        // Distribution::sample_from then the volume rescale, in FP64 like the
+        // reference (distribution/mod.rs:183, src/bin/simulation/mod.rs:255-257).
+        GW gw = grid_width(s->p);
+        double den = gw.x * gw.y * gw.z * gw.phi * gw.theta * (double)s->n;
+        double vol = s->p.box_size.x * s->p.box_size.y * s->p.box_size.z;
+        for (size_t i = 0; i < nb; ++i) dist[i] = ((double)c[i] / den) * vol;
+    }
+    return SS_OK;
+}
+
+int ss_get_flow_field(ss_sim* s, double* u, double* grad_u) {
+    if (!s || !u) return fail(SS_ERR_INVALID, "NULL argument");
+    SS_HIP_CHECK(hipSetDevice(s->p.device));
+    const size_t ns = s->ns;
+    if (!s->have_fields) {
+        std::memset(u, 0, sizeof(double) * 3 * ns);
+        if (grad_u) std::memset(grad_u, 0, sizeof(double) * 9 * ns);
+        return SS_OK;
+    }
+    int rc = s->prec == SS_FP64 ? full_fields<double>(s) : full_fields<float>(s);
+    if (rc != SS_OK) return rc;
+    rc = s->prec == SS_FP64 ? copy_real<double>(s, s->spec_full, 0, 3, u) : copy_real<float>(s, s->spec_full, 0, 3, u);
+    if (rc != SS_OK || !grad_u) return rc;
+    return s->prec == SS_FP64 ? copy_real<double>(s, s->spec_full, 3, 9, grad_u)
+                              : copy_real<float>(s, s->spec_full, 3, 9, grad_u);
+}
+
+int ss_get_magnetic_field(ss_sim* s, double* b, double* grad_b) {
+    if (!s || !b) return fail(SS_ERR_INVALID, "NULL argument");
+    if (!s->mag)
+        return fail(SS_ERR_STATE,
+                    "magnetic solve disabled: set SS_FLAG_MAGNETIC_ALWAYS or a nonzero magnetic coefficient");
+    SS_HIP_CHECK(hipSetDevice(s->p.device));
+    const size_t ns = s->ns;
+    if (!s->have_fields) {
+        std::memset(b, 0, sizeof(double) * 3 * ns);
+        if (grad_b) std::memset(grad_b, 0, sizeof(double) * 9 * ns);
+        return SS_OK;
+    }
+    int rc = s->prec == SS_FP64 ? full_fields<double>(s) : full_fields<float>(s);
+    if (rc != SS_OK) return rc;
+    rc = s->prec == SS_FP64 ? copy_real<double>(s, s->spec_full, 12, 3, b)
+                            : copy_real<float>(s, s->spec_full, 12, 3, b);
+    if (rc != SS_OK || !grad_b) return rc;
+    return s->prec == SS_FP64 ? copy_real<double>(s, s->spec_full, 15, 9, grad_b)
+                              : copy_real<float>(s, s->spec_full, 15, 9, grad_b);
+}
+
+int ss_get_snapshot(ss_sim* s, double* particles, double* compensation, uint64_t* timestep) {
+    if (!s || !particles) return fail(SS_ERR_INVALID, "NULL argument");
+    if (!s->initialised) return fail(SS_ERR_STATE, "not initialised");
+    SS_HIP_CHECK(hipSetDevice(s->p.device));
+    int rc = SS_OK;
+    if (compensation)
+        rc = s->prec == SS_FP64 ? export_particles<double>(s, particles, compensation, s->n)
+                                : export_particles<float>(s, particles, compensation, s->n);
+    else
+        rc = s->prec == SS_FP64 ? export_particles<double>(s, particles, nullptr, s->n)
+                                : export_particles<float>(s, particles, nullptr, s->n);
+    if (timestep) *timestep = s->timestep;
+    return rc;
+}
+
+int ss_resume(ss_sim* s, const double* particles, const double* compensation, uint64_t n, uint64_t timestep) {
+    if (!s || !particles) return fail(SS_ERR_INVALID, "NULL argument");
+    if (n != s->n) return fail(SS_ERR_INVALID, "snapshot size differs from number_of_particles");
+    SS_HIP_CHECK(hipSetDevice(s->p.device));
+    // Simulation::resume re-runs init (src/bin/simulation/mod.rs:199-207); the
+    // snapshot's values are already in range, so they are taken as stored.
+    int rc = s->prec == SS_FP64 ? import_particles<double>(s, particles, compensation, compensation == nullptr)
+                                : import_particles<float>(s, particles, compensation, compensation == nullptr);
+    if (rc != SS_OK) return rc;
+    SS_HIP_CHECK(hipStreamSynchronize(s->stream));
+    s->timestep = timestep;
+    s->initialised = true;
+    s->have_fields = false;
+    return SS_OK;
+}
+
+int ss_synchronize(ss_sim* s) {
+    if (!s) return fail(SS_ERR_INVALID, "NULL handle");
+    SS_HIP_CHECK(hipSetDevice(s->p.device));
+    SS_HIP_CHECK(hipStreamSynchronize(s->stream));
+    return SS_OK;
+}
+
+int ss_profile_steps(ss_sim* s, uint64_t nsteps, double* ms_per_step, char* names, int name_stride,
+                     int max_stages) {
+    if (!s || !ms_per_step) return fail(SS_ERR_INVALID, "NULL argument");
+    if (!s->initialised) return fail(SS_ERR_STATE, "ss_profile_steps before ss_init");
+    if (s->p.noise_mode == SS_NOISE_EXTERNAL && !s->have_noise)
+        return fail(SS_ERR_STATE, "SS_NOISE_EXTERNAL needs ss_set_noise");
+    if (nsteps == 0) return fail(SS_ERR_INVALID, "nsteps must be >= 1");
+    SS_HIP_CHECK(hipSetDevice(s->p.device));
+    hipEvent_t ev[ST_COUNT + 1];
+    for (int i = 0; i <= ST_COUNT; ++i) SS_HIP_CHECK(hipEventCreate(&ev[i]));
+    double acc[ST_COUNT] = {0};
+    int rc = SS_OK;
+    for (uint64_t k = 0; k < nsteps && rc == SS_OK; ++k) {
+        rc = s->prec == SS_FP64 ? step_once<double>(s, ev) : step_once<float>(s, ev);
+        if (rc != SS_OK) break;
+        if (hipEventSynchronize(ev[ST_COUNT]) != hipSuccess) {
+            rc = fail(SS_ERR_DEVICE, "event sync failed");
+            break;
+        }
+        for (int st = 0; st < ST_COUNT; ++st) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, ev[st], ev[st + 1]);
+            acc[st] += ms;
+        }
+    }
+    for (int i = 0; i <= ST_COUNT; ++i) (void)hipEventDestroy(ev[i]);
+    if (rc != SS_OK) return rc;
+    const int nst = std::min(max_stages, (int)ST_COUNT);
+    for (int st = 0; st < nst; ++st) {
+        ms_per_step[st] = acc[st] / (double)nsteps;
+        if (names && name_stride > 0) {
+            std::strncpy(names + (size_t)st * name_stride, STAGE_NAMES[st], name_stride - 1);
+            names[(size_t)st * name_stride + name_stride - 1] = 0;
+        }
+    }
+    return nst;
+}
+
+// ---------------------------------------------------------------------------
+// Solver entry points (unit-level parity).
+// ---------------------------------------------------------------------------
+template <typename T>
+static int solve_dense(const ss_params* params, const double* dist, bool magnetic, double* o1, double* o2) {
+    ss_params p = *params;
+    p.number_of_particles = 1;
+    p.noise_mode = SS_NOISE_PHILOX;
+    p.binning = SS_BIN_HISTOGRAM;
+    if (magnetic) p.flags |= SS_FLAG_MAGNETIC_ALWAYS;
+    p.flags &= ~SS_FLAG_QUASI2D;
+    ss_sim* s = nullptr;
+    int rc = ss_create(&p, &s);
+    if (rc != SS_OK) return rc;
+    const size_t nb = (size_t)s->ns * s->na;
+    double* d = nullptr;
+    if (hipMalloc(&d, nb * sizeof(double)) != hipSuccess) {
+        ss_destroy(s);
+        return fail(SS_ERR_ALLOC, "hipMalloc");
+    }
+    rc = SS_OK;
+    if (hipMemcpy(d, dist, nb * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(SS_ERR_DEVICE, "copy in");
+    if (rc == SS_OK) {
+        Geo<T> g = geo<T>(s);
+        GW gw = grid_width(s->p);
+        const T measure = (T)(gw.phi * gw.theta);
+        const unsigned nbk = (unsigned)((s->ns + 3) / 4);
+        if (s->nm == 6)
+            hipLaunchKernelGGL((k_contract_dense<T, 6>), dim3(nbk), dim3(256), 0, s->stream, d, g,
+                               (const T*)s->kern, measure, (cplx<T>*)s->spec_in);
+        else if (s->nm == 9)
+            hipLaunchKernelGGL((k_contract_dense<T, 9>), dim3(nbk), dim3(256), 0, s->stream, d, g,
+                               (const T*)s->kern, measure, (cplx<T>*)s->spec_in);
+        else
+            hipLaunchKernelGGL((k_contract_dense<T, 12>), dim3(nbk), dim3(256), 0, s->stream, d, g,
+                               (const T*)s->kern, measure, (cplx<T>*)s->spec_in);
+        if (hipGetLastError() != hipSuccess) rc = fail(SS_ERR_DEVICE, "contract launch");
+    }
+    if (rc == SS_OK) rc = s->fft.run(s->spec_in, s->nm, -1, s->stream);
+    if (rc == SS_OK) rc = full_fields<T>(s);
+    if (rc == SS_OK) {
+        const int c0 = magnetic ? 12 : 0;
+        rc = copy_real<T>(s, s->spec_full, c0, 3, o1);
+        if (rc == SS_OK && o2) rc = copy_real<T>(s, s->spec_full, c0 + 3, 9, o2);
+    }
+    (void)hipFree(d);
+    std::string msg = ss_last_error();
+    ss_destroy(s);
+    if (rc != SS_OK) set_error(msg);
+    return rc;
+}
+
+int ss_flow_solve(const ss_params* params, const double* dist, double* u, double* grad_u) {
+    if (!params || !dist || !u) return fail(SS_ERR_INVALID, "NULL argument");
+    return params->precision == SS_FP64 ? solve_dense<double>(params, dist, false, u, grad_u)
+                                        : solve_dense<float>(params, dist, false, u, grad_u);
+}
+
+int ss_magnetic_solve(const ss_params* params, const double* dist, double* b, double* grad_b) {
+    if (!params || !dist || !b) return fail(SS_ERR_INVALID, "NULL argument");
+    return params->precision == SS_FP64 ? solve_dense<double>(params, dist, true, b, grad_b)
+                                        : solve_dense<float>(params, dist, true, b, grad_b);
+}
+
+template <typename T>
+static int debug_update(const ss_params* p, uint64_t n, const double* parts, const double* fields,
+                        const double* rv, double* out) {
+    ss_sim tmp;
+    tmp.p = *p;
+    tmp.use_e = true;
+    tmp.use_rho = true;
+    tmp.use_b = true;
+    tmp.use_gb = true;
+    UpdateParams<T> P = update_params<T>(&tmp);
+    double *dp = nullptr, *df = nullptr, *dr = nullptr, *dout = nullptr;
+    int rc = SS_OK;
+    if (hipMalloc(&dp, 5 * n * 8) != hipSuccess || hipMalloc(&df, 34 * n * 8) != hipSuccess ||
+        hipMalloc(&dr, 5 * n * 8) != hipSuccess || hipMalloc(&dout, 5 * n * 8) != hipSuccess)
+        rc = fail(SS_ERR_ALLOC, "hipMalloc");
+    if (rc == SS_OK && (hipMemcpy(dp, parts, 5 * n * 8, hipMemcpyHostToDevice) != hipSuccess ||
+                        hipMemcpy(df, fields, 34 * n * 8, hipMemcpyHostToDevice) != hipSuccess ||
+                        hipMemcpy(dr, rv, 5 * n * 8, hipMemcpyHostToDevice) != hipSuccess))
+        rc = fail(SS_ERR_DEVICE, "copy in");
+    if (rc == SS_OK) {
+        hipLaunchKernelGGL(k_debug_update<T>, dim3(blocks_for(n)), dim3(256), 0, 0, dp, df, dr, n, P,
+                           (T)p->magnetic_dipole_dipole, dout);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+            rc = fail(SS_ERR_DEVICE, "debug update failed");
+    }
+    if (rc == SS_OK && hipMemcpy(out, dout, 5 * n * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(SS_ERR_DEVICE, "copy out");
+    (void)hipFree(dp);
+    (void)hipFree(df);
+    (void)hipFree(dr);
+    (void)hipFree(dout);
+    return rc;
+}
+
+int ss_debug_update(const ss_params* params, uint64_t n, const double* particles, const double* fields,
+                    const double* rv, double* out) {
+    if (!params || !particles || !fields || !rv || !out) return fail(SS_ERR_INVALID, "NULL argument");
+    if (n == 0) return SS_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SS_ERR_DEVICE, "no GPU visible");
+    SS_HIP_CHECK(hipSetDevice(params->device));
+    return params->precision == SS_FP64 ? debug_update<double>(params, n, particles, fields, rv, out)
+                                        : debug_update<float>(params, n, particles, fields, rv, out);
+}
+
+template <typename T>
+static int debug_c2g(const ss_params* p, uint64_t n, const double* parts, int32_t* out) {
+    ss_sim tmp;
+    tmp.p = *p;
+    tmp.nx = p->grid_size.x;
+    tmp.ny = p->grid_size.y;
+    tmp.nz = p->grid_size.z;
+    tmp.np = p->grid_size.phi;
+    tmp.nt = p->grid_size.theta;
+    tmp.ns = tmp.nx * tmp.ny * tmp.nz;
+    tmp.na = tmp.np * tmp.nt;
+    Geo<T> g = geo<T>(&tmp);
+    double* dp = nullptr;
+    int32_t* dout = nullptr;
+    int rc = SS_OK;
+    if (hipMalloc(&dp, 5 * n * 8) != hipSuccess || hipMalloc(&dout, 5 * n * 4) != hipSuccess)
+        rc = fail(SS_ERR_ALLOC, "hipMalloc");
+    if (rc == SS_OK && hipMemcpy(dp, parts, 5 * n * 8, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(SS_ERR_DEVICE, "copy in");
+    if (rc == SS_OK) {
+        hipLaunchKernelGGL(k_coord_to_grid<T>, dim3(blocks_for(n)), dim3(256), 0, 0, dp, n, g, dout);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+            rc = fail(SS_ERR_DEVICE, "coord_to_grid failed");
+    }
+    if (rc == SS_OK && hipMemcpy(out, dout, 5 * n * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(SS_ERR_DEVICE, "copy out");
+    (void)hipFree(dp);
+    (void)hipFree(dout);
+    return rc;
+}
+
+int ss_debug_coord_to_grid(const ss_params* params, uint64_t n, const double* particles, int32_t* out) {
+    if (!params || !particles || !out) return fail(SS_ERR_INVALID, "NULL argument");
+    if (n == 0) return SS_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SS_ERR_DEVICE, "no GPU visible");
+    SS_HIP_CHECK(hipSetDevice(params->device));
+    return params->precision == SS_FP64 ? debug_c2g<double>(params, n, particles, out)
+                                        : debug_c2g<float>(params, n, particles, out);
+}
+
+// ---------------------------------------------------------------------------
+// Initial condition (host): uniform positions; cos(theta) by inverse CDF of
+// 1 + a sin(2 pi x / Lx) cos(theta); phi uniform.
+// ---------------------------------------------------------------------------
+int ss_initial_condition(const ss_params* params, double amplitude, uint64_t seed, double* particles,
+                         uint64_t n) {
+    if (!params || !particles) return fail(SS_ERR_INVALID, "NULL argument");
+    const double L[3] = {params->box_size.x, params->box_size.y, params->box_size.z};
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    for (uint64_t i = 0; i < n; ++i) {
+        uint32_t w[8];
+        philox4x32_10((uint32_t)i, 0xC0FFEEu, (uint32_t)(i >> 32), 0u, k0, k1, w);
+        philox4x32_10((uint32_t)i, 0xC0FFEEu, (uint32_t)(i >> 32), 1u, k0, k1, w + 4);
+        double u[5];
+        for (int k = 0; k < 5; ++k) u[k] = ((double)w[k] + 0.5) * (1.0 / 4294967296.0);
+        double* p = particles + 5 * i;
+        p[0] = L[0] * u[0];
+        p[1] = L[1] * u[1];
+        p[2] = L[2] * u[2];
+        p[3] = TWOPI_D * u[3];
+        const double eps = amplitude * sin(TWOPI_D * p[0] / L[0]);
+        // density (1 + eps c)/2 on c in [-1,1]; CDF F(c) = ((c+1) + eps (c^2-1)/2)/2
+        const double q = 1.0 - eps / 2.0 - 2.0 * u[4];  // eps c^2/2 + c + q = 0
+        double c = -2.0 * q / (1.0 + sqrt(1.0 - 2.0 * eps * q));
+        c = std::min(1.0, std::max(-1.0, c));
+        p[4] = acos(c);
+    }
+    return SS_OK;
+}

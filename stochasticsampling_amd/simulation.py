@@ -1,0 +1,253 @@
+"""Python mirror of the reference's ``Simulation`` API
+(src/bin/simulation/mod.rs:60-393), ``SpectralSolver`` / ``MagneticSolver``
+solves (spectral_solver.rs:233-258, magnetic_solver.rs:139-169) and ``FFTPlan``
+(crates/fftw3/src/fft.rs:33-244), over the C ABI of libss.
+
+Method names, argument meaning and error behaviour follow the reference:
+``Simulation(params)`` is ``Simulation::new``; ``init`` rejects a particle
+count that differs from the parameters (the reference asserts); getters copy
+out like the reference's clones.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import lib as _l
+from .params import Params
+
+_P = C.POINTER
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None and a.shape != shape:
+        a = a.reshape(shape)
+    return a
+
+
+class Snapshot:
+    """Snapshot (src/bin/simulation/mod.rs:80-86): particles, the low-order
+    position parts of the FP32 path, and the timestep (the counter-based noise
+    generator's whole state besides the seed)."""
+
+    def __init__(self, particles, compensation, timestep):
+        self.particles = particles
+        self.compensation = compensation
+        self.timestep = timestep
+
+
+class Simulation:
+    def __init__(self, params: Params):
+        self.L = _l.load()
+        self.params = params
+        h = C.c_void_p()
+        _l.check(self.L.ss_create(C.byref(params), C.byref(h)))
+        self._h = h
+        self.n = params.number_of_particles
+
+    # lifecycle -------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self.L.ss_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # Simulation::init / resume / get_snapshot --------------------------------
+    def init(self, particles):
+        p = _f64(particles)
+        if p.ndim != 2 or p.shape[1] != 5:
+            raise ValueError("particles must be N x 5 [x, y, z, phi, theta]")
+        _l.check(self.L.ss_init(self._h, _l.dptr(p), p.shape[0]))
+
+    def resume(self, snapshot: Snapshot):
+        p = _f64(snapshot.particles)
+        comp = None if snapshot.compensation is None else _f64(snapshot.compensation)
+        _l.check(
+            self.L.ss_resume(
+                self._h,
+                _l.dptr(p),
+                None if comp is None else _l.dptr(comp),
+                p.shape[0],
+                snapshot.timestep,
+            )
+        )
+
+    def get_snapshot(self):
+        p = np.zeros((self.n, 5))
+        comp = np.zeros((self.n, 3))
+        t = C.c_uint64()
+        _l.check(self.L.ss_get_snapshot(self._h, _l.dptr(p), _l.dptr(comp), C.byref(t)))
+        return Snapshot(p, comp, t.value)
+
+    # noise -------------------------------------------------------------------
+    def set_noise(self, rv):
+        r = _f64(rv)
+        _l.check(self.L.ss_set_noise(self._h, _l.dptr(r), r.shape[0]))
+
+    # Simulation::do_timestep / Iterator ---------------------------------------
+    def do_timestep(self):
+        t = C.c_uint64()
+        _l.check(self.L.ss_step(self._h, 1, C.byref(t)))
+        return t.value
+
+    def step(self, nsteps=1):
+        t = C.c_uint64()
+        _l.check(self.L.ss_step(self._h, nsteps, C.byref(t)))
+        return t.value
+
+    def __iter__(self):
+        return self
+
+    def __next__(self):
+        return self.do_timestep()
+
+    def synchronize(self):
+        _l.check(self.L.ss_synchronize(self._h))
+
+    # getters ----------------------------------------------------------------
+    def get_timestep(self):
+        t = C.c_uint64()
+        _l.check(self.L.ss_get_timestep(self._h, C.byref(t)))
+        return t.value
+
+    def get_particles(self):
+        return self.get_particles_head(self.n)
+
+    def get_particles_head(self, n):
+        out = np.zeros((n, 5))
+        _l.check(self.L.ss_get_particles(self._h, _l.dptr(out), n))
+        return out
+
+    def get_counts(self):
+        p = self.params
+        out = np.zeros(p.dist_shape, dtype=np.uint32)
+        _l.check(self.L.ss_get_distribution(self._h, out.ctypes.data_as(_P(C.c_uint32)), None))
+        return out
+
+    def get_distribution(self):
+        p = self.params
+        out = np.zeros(p.dist_shape)
+        _l.check(self.L.ss_get_distribution(self._h, None, _l.dptr(out)))
+        return out
+
+    def get_flow_field(self, with_gradient=False):
+        p = self.params
+        u = np.zeros((3,) + p.grid_shape)
+        g = np.zeros((3, 3) + p.grid_shape) if with_gradient else None
+        _l.check(self.L.ss_get_flow_field(self._h, _l.dptr(u), None if g is None else _l.dptr(g)))
+        return (u, g) if with_gradient else u
+
+    def get_magnetic_field(self, with_gradient=False):
+        p = self.params
+        b = np.zeros((3,) + p.grid_shape)
+        g = np.zeros((3, 3) + p.grid_shape) if with_gradient else None
+        _l.check(self.L.ss_get_magnetic_field(self._h, _l.dptr(b), None if g is None else _l.dptr(g)))
+        return (b, g) if with_gradient else b
+
+    def profile_steps(self, nsteps, max_stages=16):
+        """Runs nsteps steps timing every stage with HIP events on the
+        simulation's stream; returns {stage: mean ms per step}."""
+        ms = np.zeros(max_stages)
+        stride = 32
+        names = C.create_string_buffer(stride * max_stages)
+        k = _l.check(self.L.ss_profile_steps(self._h, nsteps, _l.dptr(ms), names, stride, max_stages))
+        raw = names.raw
+        out = {}
+        for i in range(k):
+            nm = raw[i * stride : (i + 1) * stride].split(b"\0", 1)[0].decode()
+            out[nm] = float(ms[i])
+        return out
+
+
+def initial_condition(params, amplitude=0.05, seed=None, n=None):
+    """Uniform positions, isotropic orientations with a polar perturbation
+    amplitude*sin(2 pi x / Lx) along z (N x 5, serde order)."""
+    L = _l.load()
+    n = params.number_of_particles if n is None else n
+    seed = params.seed if seed is None else seed
+    out = np.zeros((n, 5))
+    _l.check(L.ss_initial_condition(C.byref(params), amplitude, seed, _l.dptr(out), n))
+    return out
+
+
+def mean_flow_field(params, dist):
+    """SpectralSolver::new + mean_flow_field (spectral_solver.rs:33-68,233-254)
+    on a given distribution; returns (u [3,...], grad u [3,3,...]) real parts."""
+    L = _l.load()
+    d = _f64(dist)
+    u = np.zeros((3,) + params.grid_shape)
+    g = np.zeros((3, 3) + params.grid_shape)
+    _l.check(L.ss_flow_solve(C.byref(params), _l.dptr(d), _l.dptr(u), _l.dptr(g)))
+    return u, g
+
+
+def mean_magnetic_field(params, dist):
+    """MagneticSolver::mean_magnetic_field (magnetic_solver.rs:139-161)."""
+    L = _l.load()
+    d = _f64(dist)
+    b = np.zeros((3,) + params.grid_shape)
+    g = np.zeros((3, 3) + params.grid_shape)
+    _l.check(L.ss_magnetic_solve(C.byref(params), _l.dptr(d), _l.dptr(b), _l.dptr(g)))
+    return b, g
+
+
+class FFTPlan:
+    """FFTPlan (crates/fftw3/src/fft.rs:33-244): unnormalised in-place c2c 3-D,
+    ``direction`` -1 forward / +1 backward, batched over leading arrays."""
+
+    def __init__(self, shape, direction, batch=1, precision=0, device=0):
+        self.L = _l.load()
+        self.shape = tuple(shape)
+        self.batch = batch
+        self.precision = precision
+        h = C.c_void_p()
+        nx, ny, nz = self.shape
+        _l.check(
+            self.L.ss_fft_plan_create(nx, ny, nz, direction, batch, precision, 0, device, C.byref(h))
+        )
+        self._h = h
+
+    def reexecute3d(self, a):
+        """Transforms a (complex, shape (batch,)+shape or shape) in place."""
+        dt = np.complex128 if self.precision == 0 else np.complex64
+        if a.dtype != dt or not a.flags.c_contiguous:
+            raise TypeError(f"array must be C-contiguous {np.dtype(dt).name}")
+        _l.check(self.L.ss_fft_execute_host(self._h, a.ctypes.data_as(C.c_void_p)))
+        return a
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.L.ss_fft_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def debug_update(params, particles, fields, rv):
+    """The device particle update on explicit per-particle inputs
+    (ss_debug_update); fields: n x 34, rv: n x 5."""
+    L = _l.load()
+    p = _f64(particles)
+    f = _f64(fields)
+    r = _f64(rv)
+    out = np.zeros_like(p)
+    _l.check(L.ss_debug_update(C.byref(params), p.shape[0], _l.dptr(p), _l.dptr(f), _l.dptr(r), _l.dptr(out)))
+    return out
+
+
+def debug_coord_to_grid(params, particles):
+    """The device binning rule (ss_debug_coord_to_grid); returns n x 5 int32."""
+    L = _l.load()
+    p = _f64(particles)
+    out = np.zeros(p.shape, dtype=np.int32)
+    _l.check(L.ss_debug_coord_to_grid(C.byref(params), p.shape[0], _l.dptr(p), out.ctypes.data_as(_P(C.c_int32))))
+    return out

@@ -1,0 +1,358 @@
+"""GPU parity of libss against the FP64 CPU oracle (through the C ABI).
+
+Bars: integer outputs (bin indices, counts) bit-exact; FP64 fields and states
+within 1e-10 relative of the oracle; FP32 fields within 1e-5 relative of the
+FP64 oracle (the stated target for u and grad u); FP32 states as stated per
+test.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import stochasticsampling_amd as ss
+from helpers import isotropic_particles, rel_err, small_params, periodic_diff
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+FP32_FIELD_TOL = 1e-5
+
+
+# ---------------------------------------------------------------------------
+# FFT plan (FFTPlan, crates/fftw3/src/fft.rs)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("shape", [(16, 16, 16), (11, 11, 11), (8, 12, 10), (64, 64, 64), (32, 48, 6)])
+@pytest.mark.parametrize("precision", [ss.FP64, ss.FP32])
+@pytest.mark.parametrize("direction", [-1, 1])
+def test_fft_plan_matches_numpy(shape, precision, direction):
+    rng = np.random.default_rng(1)
+    batch = 3
+    x = rng.standard_normal((batch,) + shape) + 1j * rng.standard_normal((batch,) + shape)
+    dt = np.complex128 if precision == ss.FP64 else np.complex64
+    a = np.ascontiguousarray(x.astype(dt))
+    plan = ss.FFTPlan(shape, direction, batch=batch, precision=precision)
+    plan.reexecute3d(a)
+    axes = (1, 2, 3)
+    ref = np.fft.fftn(x, axes=axes) if direction == -1 else np.fft.ifftn(x, axes=axes) * np.prod(shape)
+    tol = 1e-13 if precision == ss.FP64 else 2e-6
+    assert rel_err(a, ref) < tol
+
+
+# ---------------------------------------------------------------------------
+# Binning rule (Distribution::coord_to_grid, distribution_test.rs:119-184)
+# ---------------------------------------------------------------------------
+def test_coord_to_grid_reference_table(oracle):
+    p = ss.make_params(box=(1.0, 1.0, 1.0), grid=(50, 50, 1, 10, 1))
+    inputs = [
+        [0.0, 0.0, 0.0, 0.0, 0.0],
+        [1.0, 0.0, 0.0, 0.0, 0.0],
+        [0.0, 1.0, 0.0, 0.0, 0.0],
+        [0.0, 0.0, 0.0, 0.5, 0.0],
+        [0.0, 0.0, 0.0, 7.0, 0.0],
+        [0.0, 0.0, 0.0, -1.0, 0.0],
+        [0.96, 0.0, 0.0, 0.0, 0.0],
+        [0.5, 0.5, 0.0, -1.0, 0.0],
+        [0.5, 0.5, 0.0, 0.0, 0.0],
+        [0.0, 0.0, 0.0, 2.0 * math.pi, 0.0],
+        [0.51000000000000005, 0.5, 0.0, 6.283185307179584, 0.0],
+    ]
+    expect = [
+        [0, 0, 0, 0, 0],
+        [0, 0, 0, 0, 0],
+        [0, 0, 0, 0, 0],
+        [0, 0, 0, 0, 0],
+        [0, 0, 0, 1, 0],
+        [0, 0, 0, 8, 0],
+        [48, 0, 0, 0, 0],
+        [25, 25, 0, 8, 0],
+        [25, 25, 0, 0, 0],
+        [0, 0, 0, 0, 0],
+        [25, 25, 0, 9, 0],
+    ]
+    sanitized = np.array([oracle.particle_new(i, (1.0, 1.0, 1.0)) for i in inputs])
+    got = ss.debug_coord_to_grid(p, sanitized)
+    np.testing.assert_array_equal(got, np.array(expect, dtype=np.int32))
+
+
+def test_coord_to_grid_fp32_matches_oracle_rule(oracle):
+    p = ss.make_params(box=(50.0, 50.0, 50.0), grid=(64, 64, 64, 16, 16), precision=ss.FP32)
+    parts = isotropic_particles(20000, (50.0, 50.0, 50.0), 3)
+    # plant values on and next to bin edges
+    g = 50.0 / 64
+    parts[:200, 0] = np.arange(200) % 64 * g
+    parts[200:400, 0] = np.nextafter((np.arange(200) % 64 + 1) * g, 0)
+    parts32 = parts.astype(np.float32).astype(np.float64)
+    got = ss.debug_coord_to_grid(p, parts32)
+    exp = np.array([oracle.coord_to_grid_f32(p, q) for q in parts32.astype(np.float32)])
+    np.testing.assert_array_equal(got, exp)
+
+
+# ---------------------------------------------------------------------------
+# Particle update (modifiers_test.rs:39-97, mod_test.rs) and random cases
+# ---------------------------------------------------------------------------
+def _fields(u=None, W=None, E=None, rho=0.0, B=None, GB=None):
+    f = np.zeros(34)
+    if u is not None:
+        f[0:3] = u
+    if W is not None:
+        f[3:12] = np.asarray(W).ravel()
+    if E is not None:
+        f[12:21] = np.asarray(E).ravel()
+    f[21] = rho
+    if B is not None:
+        f[22:25] = B
+    if GB is not None:
+        f[25:34] = np.asarray(GB).ravel()
+    return f
+
+
+def test_update_matches_oracle_random_fp64(oracle):
+    rng = np.random.default_rng(5)
+    n = 4096
+    p = ss.make_params(
+        box=(10.0, 10.0, 10.0),
+        timestep=1e-2,
+        shape=0.7,
+        magnetic_drag=0.3,
+        magnetic_reorientation=1.3,
+        magnetic_dipole_dipole=0.4,
+        volume_exclusion=0.2,
+    )
+    parts = isotropic_particles(n, (10.0, 10.0, 10.0), 11)
+    fields = rng.standard_normal((n, 34))
+    W = rng.standard_normal((n, 3, 3))
+    W = 0.5 * (W - W.transpose(0, 2, 1))
+    E = rng.standard_normal((n, 3, 3))
+    E = 0.5 * (E + E.transpose(0, 2, 1))
+    fields[:, 3:12] = W.reshape(n, 9)
+    fields[:, 12:21] = E.reshape(n, 9)
+    fields[:, 21] = np.abs(fields[:, 21])
+    rv = rng.standard_normal((n, 5))
+    rv[:, 3] = rng.random(n) * 2 * np.pi
+    rv[:, 4] = np.abs(rv[:, 4]) * 0.1
+    got = ss.debug_update(p, parts, fields, rv)
+    exp = np.array([oracle.update_one(p, parts[i], fields[i], rv[i]) for i in range(n)])
+    assert np.max(periodic_diff(got[:, :3], exp[:, :3], 10.0)) < 1e-12
+    assert np.max(periodic_diff(got[:, 3], exp[:, 3], 2 * np.pi)) < 1e-10
+    assert np.max(np.abs(got[:, 4] - exp[:, 4])) < 1e-10
+
+
+def test_update_modifier_cases_fp64(oracle):
+    pi = math.pi
+    bs = (10.0, 10.0, 10.0)
+    p0 = [0.0, 0.0, 0.0, 0.0, 0.0]
+    cases = []  # (params kwargs, particle, fields, rv, expected particle)
+    zero_rv = np.zeros(5)
+    cases.append(({}, p0, _fields(), zero_rv, [0.0, 0.0, 1.0, 0.0, 0.0]))  # self_propulsion
+    cases.append(
+        ({"magnetic_reorientation": 0.1}, [0, 0, 0, 0, pi / 2], _fields(u=[-1.0, 0.0, 0.0]), zero_rv,
+         [0.0, 0.0, 0.0, 0.09966865249116204, pi / 2]))
+    results = []
+    for kw, part, f, rv, exp in cases:
+        prm = ss.make_params(box=bs, timestep=1.0, diffusion_translational=0.0, **kw)
+        part = oracle.particle_new(part, bs)
+        got = ss.debug_update(prm, part[None, :], f[None, :], rv[None, :])[0]
+        want = oracle.particle_new(exp, bs)
+        results.append((got, want))
+    for got, want in results:
+        assert np.max(np.abs(got - want)) < 1e-15, (got, want)
+
+
+# ---------------------------------------------------------------------------
+# Spectral solves (ff_test fixture configuration; magnetic solve)
+# ---------------------------------------------------------------------------
+def _single_particle_dist(oracle, p, part):
+    cnt = oracle.histogram(p, np.asarray(part)[None, :])
+    return oracle.sample_from(p, cnt, 1)
+
+
+@pytest.mark.parametrize("precision", [ss.FP64, ss.FP32])
+def test_flow_solve_single_particle_fixture(oracle, precision):
+    p = ss.make_params(box=(11.0, 11.0, 11.0), grid=(11, 11, 11, 11, 11), n=1, stress_active=1.0,
+                       precision=precision)
+    part = oracle.particle_new([0, 0, 0, math.pi / 2, math.pi / 2], (11.0, 11.0, 11.0))
+    dist = _single_particle_dist(oracle, p, part)
+    u, g = ss.mean_flow_field(p, dist)
+    ue, ge = oracle.mean_flow_field(p, 0.0, dist)
+    tol = FP64_TOL if precision == ss.FP64 else FP32_FIELD_TOL
+    assert rel_err(u, ue) < tol
+    assert rel_err(g, ge) < tol
+
+
+@pytest.mark.parametrize("precision", [ss.FP64, ss.FP32])
+@pytest.mark.parametrize("magnetic_stress", [0.0, 0.8])
+def test_flow_solve_random_distribution(oracle, precision, magnetic_stress):
+    p = ss.make_params(box=(10.0, 12.0, 9.0), grid=(16, 12, 10, 6, 5), stress_active=-1.3,
+                       stress_magnetic=magnetic_stress, shape=0.4, hydro_screening=0.25,
+                       precision=precision)
+    rng = np.random.default_rng(2)
+    dist = rng.random(p.dist_shape) * 3.0
+    u, g = ss.mean_flow_field(p, dist)
+    ue, ge = oracle.mean_flow_field(p, 0.25, dist)
+    tol = FP64_TOL if precision == ss.FP64 else FP32_FIELD_TOL
+    assert rel_err(u, ue) < tol
+    assert rel_err(g, ge) < tol
+
+
+@pytest.mark.parametrize("precision", [ss.FP64, ss.FP32])
+def test_magnetic_solve_random_distribution(oracle, precision):
+    p = ss.make_params(box=(10.0, 10.0, 10.0), grid=(12, 12, 12, 8, 6), precision=precision)
+    rng = np.random.default_rng(3)
+    dist = rng.random(p.dist_shape)
+    b, g = ss.mean_magnetic_field(p, dist)
+    be, ge = oracle.mean_magnetic_field(p, dist)
+    tol = FP64_TOL if precision == ss.FP64 else FP32_FIELD_TOL
+    assert rel_err(b, be) < tol
+    assert rel_err(g, ge) < tol
+
+
+# ---------------------------------------------------------------------------
+# Whole steps against the oracle
+# ---------------------------------------------------------------------------
+def _run_pair(oracle, p, parts, steps):
+    sim = ss.Simulation(p)
+    sim.init(parts)
+    osim = oracle.simulation(p)
+    osim.init(parts)
+    if steps:
+        sim.step(steps)
+        osim.step(steps)
+    return sim, osim
+
+
+@pytest.mark.parametrize("binning", [ss.BIN_HISTOGRAM, ss.BIN_SORTED])
+def test_init_histogram_bit_exact_fp64(oracle, binning):
+    p = small_params(binning=binning, flags=ss.FLAG_KEEP_DISTRIBUTION)
+    parts = isotropic_particles(p.number_of_particles, (10.0, 10.0, 10.0), 4)
+    sim, osim = _run_pair(oracle, p, parts, 0)
+    np.testing.assert_array_equal(sim.get_counts(), osim.counts())
+    np.testing.assert_array_equal(sim.get_distribution(), osim.distribution())
+    np.testing.assert_array_equal(sim.get_particles(), osim.particles())
+
+
+@pytest.mark.parametrize("binning", [ss.BIN_HISTOGRAM, ss.BIN_SORTED])
+def test_steps_fp64_match_oracle(oracle, binning):
+    p = small_params(binning=binning, flags=ss.FLAG_KEEP_DISTRIBUTION, n=4096)
+    parts = ss.initial_condition(p, amplitude=0.05, seed=1)
+    sim, osim = _run_pair(oracle, p, parts, 5)
+    assert sim.get_timestep() == osim.timestep == 5
+    np.testing.assert_array_equal(sim.get_counts(), osim.counts())
+    u, g = sim.get_flow_field(with_gradient=True)
+    ue, ge = osim.flow_field()
+    assert rel_err(u, ue) < FP64_TOL
+    assert rel_err(g, ge) < FP64_TOL
+    a, b = sim.get_particles(), osim.particles()
+    assert np.max(periodic_diff(a[:, :3], b[:, :3], 10.0)) < 1e-9
+    assert np.max(periodic_diff(a[:, 3], b[:, 3], 2 * np.pi)) < 1e-9
+    assert np.max(np.abs(a[:, 4] - b[:, 4])) < 1e-9
+
+
+def test_sorted_and_histogram_paths_identical_fp64():
+    base = dict(n=4096, flags=ss.FLAG_KEEP_DISTRIBUTION)
+    p1 = small_params(binning=ss.BIN_HISTOGRAM, **base)
+    p2 = small_params(binning=ss.BIN_SORTED, **base)
+    parts = ss.initial_condition(p1, amplitude=0.05, seed=2)
+    out = []
+    for p in (p1, p2):
+        sim = ss.Simulation(p)
+        sim.init(parts)
+        sim.step(4)
+        out.append((sim.get_particles(), sim.get_counts()))
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    # identical per-particle arithmetic; the moments differ only in summation
+    # order, so the states agree to rounding
+    assert np.max(np.abs(out[0][0] - out[1][0])) < 1e-12
+
+
+def test_external_noise_equals_philox_fp64(oracle):
+    p = small_params(n=1024)
+    parts = ss.initial_condition(p, amplitude=0.05, seed=3)
+    sim = ss.Simulation(p)
+    sim.init(parts)
+    sim.step(1)
+    pe = small_params(n=1024, noise_mode=ss.NOISE_EXTERNAL)
+    dr = math.sqrt(2 * pe.diffusion_rotational * pe.timestep)
+    rv = np.array([oracle.noise(pe.seed, 0, i, dr) for i in range(1024)])
+    sim2 = ss.Simulation(pe)
+    sim2.init(parts)
+    sim2.set_noise(rv)
+    sim2.step(1)
+    assert np.max(np.abs(sim.get_particles() - sim2.get_particles())) < 1e-12
+
+
+def test_fp32_fields_within_target_of_fp64(oracle):
+    p32 = small_params(precision=ss.FP32, grid=(16, 16, 16, 8, 8), n=1 << 15, flags=ss.FLAG_KEEP_DISTRIBUTION)
+    p64 = small_params(precision=ss.FP64, grid=(16, 16, 16, 8, 8), n=1 << 15)
+    parts = ss.initial_condition(p64, amplitude=0.05, seed=4)
+    sim = ss.Simulation(p32)
+    sim.init(parts)
+    sim.step(1)
+    osim = oracle.simulation(p64)
+    osim.init(parts)
+    osim.step(1)
+    # same counts (positions rounded to FP32 stay in their bins for this IC)
+    cnt = sim.get_counts()
+    np.testing.assert_array_equal(cnt.sum(), p32.number_of_particles)
+    u, g = sim.get_flow_field(with_gradient=True)
+    ue, ge = osim.flow_field()
+    if np.array_equal(cnt, osim.counts()):
+        assert rel_err(u, ue) < FP32_FIELD_TOL
+        assert rel_err(g, ge) < FP32_FIELD_TOL
+    a, b = sim.get_particles(), osim.particles()
+    assert np.median(periodic_diff(a[:, :3], b[:, :3], 10.0)) < 1e-6
+    assert np.percentile(periodic_diff(a[:, :3], b[:, :3], 10.0), 99) < 1e-5
+
+
+def test_snapshot_resume_is_exact():
+    for prec in (ss.FP64, ss.FP32):
+        p = small_params(precision=prec, n=2048)
+        parts = ss.initial_condition(p, amplitude=0.05, seed=5)
+        a = ss.Simulation(p)
+        a.init(parts)
+        a.step(3)
+        snap = a.get_snapshot()
+        a.step(2)
+        b = ss.Simulation(p)
+        b.resume(snap)
+        assert b.get_timestep() == 3
+        b.step(2)
+        np.testing.assert_array_equal(a.get_particles(), b.get_particles())
+
+
+def test_init_rejects_wrong_particle_count():
+    p = small_params(n=100)
+    sim = ss.Simulation(p)
+    with pytest.raises(ss.SSError, match="same number of particles"):
+        sim.init(np.zeros((99, 5)))
+
+
+# ---------------------------------------------------------------------------
+# Size-independent properties at the full 2^24-particle, 64^3 x 16 x 16 size
+# ---------------------------------------------------------------------------
+def test_full_size_fp32_invariants():
+    p = ss.make_params(box=(50.0, 50.0, 50.0), grid=(64, 64, 64, 16, 16), n=1 << 24, seed=2,
+                       precision=ss.FP32, flags=ss.FLAG_KEEP_DISTRIBUTION)
+    parts = ss.initial_condition(p, amplitude=0.05, seed=2)
+    sim = ss.Simulation(p)
+    sim.init(parts)
+    sim.step(2)
+    cnt = sim.get_counts()
+    assert int(cnt.sum(dtype=np.uint64)) == p.number_of_particles
+    u = sim.get_flow_field()
+    # k = 0 mode of u is zero and the field is divergence free
+    assert abs(u.mean()) < 1e-6 * np.abs(u).max()
+    uh = np.fft.fftn(u, axes=(1, 2, 3))
+    k = [2 * np.pi * np.fft.fftfreq(64, d=50.0 / 64)] * 3
+    # use the reference's k-sampling (Nyquist stored negative)
+    kx = k[0].copy()
+    kx[32] = -kx[32] if kx[32] > 0 else kx[32]
+    KX, KY, KZ = np.meshgrid(kx, kx, kx, indexing="ij")
+    div = KX * uh[0] + KY * uh[1] + KZ * uh[2]
+    inner = (np.abs(KX) < 0.9 * np.pi / (50 / 64)) & (np.abs(KY) < 0.9 * np.pi / (50 / 64)) & (
+        np.abs(KZ) < 0.9 * np.pi / (50 / 64))
+    assert np.abs(div[inner]).max() < 1e-4 * np.abs(uh).max() * np.abs(kx).max()
+    parts2 = sim.get_particles()
+    assert np.all(parts2[:, :3] >= 0) and np.all(parts2[:, :3] <= 50.0)
+    assert np.all(parts2[:, 3] >= 0) and np.all(parts2[:, 3] <= 2 * np.pi)
+    assert np.all(parts2[:, 4] >= 0) and np.all(parts2[:, 4] <= np.pi)
