@@ -689,8 +689,9 @@ void o_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[
 }
 
 /* Twelve words per particle and step: counter (j, id, step lo, step hi),
-   j = 0,1,2; key (seed lo, seed hi).  Uniform k in (0,1) takes its high 32
-   bits from word k and its low 21 bits from the top of word 6+k. */
+   j = 0,1,2; key (seed lo, seed hi).  Uniform k takes a 52-bit integer m made
+   of word k (high 32 bits) and the top 20 bits of word 6+k, u = (m + 1/2) 2^-52,
+   so that 0 < u < 1 holds exactly in binary64. */
 void o_noise(uint64_t seed, uint64_t step, uint64_t id, double dr, double out[5]) {
     uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
     uint32_t w[12];
@@ -700,8 +701,8 @@ void o_noise(uint64_t seed, uint64_t step, uint64_t id, double dr, double out[5]
     }
     double u[6];
     for (int k = 0; k < 6; ++k) {
-        uint64_t m = ((uint64_t)w[k] << 21) | (uint64_t)(w[6 + k] >> 11);
-        u[k] = ((double)m + 0.5) * (1.0 / 9007199254740992.0);
+        uint64_t m = ((uint64_t)w[k] << 20) | (uint64_t)(w[6 + k] >> 12);
+        u[k] = ((double)m + 0.5) * (1.0 / 4503599627370496.0);
     }
     double r0 = sqrt(-2.0 * log(u[0]));
     double r1 = sqrt(-2.0 * log(u[2]));

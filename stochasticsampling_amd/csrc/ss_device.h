@@ -73,9 +73,11 @@ __host__ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
 }
 
 // Per particle and step: counters (j, id, step lo, step hi), j = 0..2, key
-// (seed lo, seed hi).  Uniform k takes its high 32 bits from word k and (FP64)
-// 21 more bits from the top of word 6+k.  The FP32 path uses the high 24 bits
-// of word k only, so both precisions see the same increments to ~2^-24.
+// (seed lo, seed hi).  FP64: uniform k is (m + 1/2) 2^-52 with m the 52-bit
+// integer of word k (high 32 bits) and the top 20 bits of word 6+k.  FP32: the
+// top 23 bits of word k, (m + 1/2) 2^-23.  Both are exactly representable and
+// strictly inside (0, 1), and the two precisions see the same increments to
+// ~2^-23.
 // Result: RandomVector [x, y, z, axis_angle, rotate_angle] of
 // src/bin/simulation/mod.rs:51-57,259-286 (Box-Muller normals, Rayleigh angle).
 __device__ inline void philox_noise(uint64_t seed, uint64_t step, uint32_t id, double dr,
@@ -89,8 +91,8 @@ __device__ inline void philox_noise(uint64_t seed, uint64_t step, uint32_t id, d
     double u[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-        uint64_t m = ((uint64_t)w[k] << 21) | (uint64_t)(w[6 + k] >> 11);
-        u[k] = ((double)m + 0.5) * (1.0 / 9007199254740992.0);
+        uint64_t m = ((uint64_t)w[k] << 20) | (uint64_t)(w[6 + k] >> 12);
+        u[k] = ((double)m + 0.5) * (1.0 / 4503599627370496.0);
     }
     const double TWOPI = Math<double>::TWOPI;
     double r0 = ::sqrt(-2.0 * ::log(u[0]));
@@ -111,7 +113,7 @@ __device__ inline void philox_noise(uint64_t seed, uint64_t step, uint32_t id, f
     philox4x32_10(1u, id, s0, s1, k0, k1, w + 4);
     float u[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) u[k] = ((float)(w[k] >> 8) + 0.5f) * (1.0f / 16777216.0f);
+    for (int k = 0; k < 6; ++k) u[k] = ((float)(w[k] >> 9) + 0.5f) * (1.0f / 8388608.0f);
     const float TWOPI = Math<float>::TWOPI;
     float r0 = sqrtf(-2.0f * logf(u[0]));
     float r1 = sqrtf(-2.0f * logf(u[2]));

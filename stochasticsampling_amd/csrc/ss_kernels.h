@@ -337,9 +337,10 @@ __device__ inline float add_wrap<float>(float x, float d, float L, float* lo) {
     float t = x + y;
     float bp = t - x;
     float l = (x - (t - bp)) + (y - bp);  // TwoSum: t + l == x + y exactly
-    if (t >= L) {
-        t = t - L;
-    } else if (t < 0.0f) {
+    // wrap on the value t + l, so that it lands in [0, L); t itself may equal L
+    if (t > L || (t == L && l >= 0.0f)) {
+        t = t - L;  // exact (Sterbenz)
+    } else if (t < 0.0f || (t == 0.0f && l < 0.0f)) {
         float s = t + L;
         l = l + (t - (s - L));
         t = s;
@@ -561,7 +562,8 @@ __global__ void __launch_bounds__(256)
 
 template <typename T>
 __global__ void __launch_bounds__(256)
-    k_export(State<T> s, uint64_t n, double* __restrict__ aos, double* __restrict__ comp) {
+    k_export(State<T> s, uint64_t n, double* __restrict__ aos, double* __restrict__ comp, double box0,
+             double box1, double box2) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint64_t j = s.id ? s.id[i] : i;
@@ -572,7 +574,13 @@ __global__ void __launch_bounds__(256)
         if (comp) {
             for (int k = 0; k < 3; ++k) comp[3 * j + k] = (double)lo[k][i];
         } else {
-            for (int k = 0; k < 3; ++k) aos[5 * j + k] = (double)src[k][i] + (double)lo[k][i];
+            const double L[3] = {box0, box1, box2};
+            for (int k = 0; k < 3; ++k) {
+                double v = (double)src[k][i] + (double)lo[k][i];
+                if (v < 0.0) v += L[k];
+                if (v >= L[k]) v -= L[k];
+                aos[5 * j + k] = v;
+            }
         }
     } else if (comp) {
         for (int k = 0; k < 3; ++k) comp[3 * j + k] = 0.0;

@@ -448,7 +448,8 @@ static int import_particles(ss_sim* s, const double* parts, const double* comp, 
 template <typename T>
 static int export_particles(ss_sim* s, double* out, double* comp, uint64_t n) {
     hipLaunchKernelGGL(k_export<T>, dim3(blocks_for(s->n)), dim3(256), 0, s->stream, state_a<T>(s), s->n,
-                       s->staging, comp ? s->staging2 : nullptr);
+                       s->staging, comp ? s->staging2 : nullptr, s->p.box_size.x, s->p.box_size.y,
+                       s->p.box_size.z);
     SS_HIP_CHECK(hipGetLastError());
     if (out) SS_HIP_CHECK(hipMemcpyAsync(out, s->staging, sizeof(double) * 5 * n, hipMemcpyDeviceToHost, s->stream));
     if (comp) SS_HIP_CHECK(hipMemcpyAsync(comp, s->staging2, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s->stream));
