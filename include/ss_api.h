@@ -41,7 +41,8 @@ extern "C" {
 /* Binning strategy for particle -> (x,y,z,phi,theta) histogram. */
 #define SS_BIN_AUTO 0      /* SORTED for FP32, HISTOGRAM for FP64                        */
 #define SS_BIN_HISTOGRAM 1 /* plain: full 5-D u32 histogram by global atomics           */
-#define SS_BIN_SORTED 2    /* particles kept sorted by cell; LDS-privatised counts      */
+#define SS_BIN_SORTED 2    /* particles grouped by spatial tile; binning fused into the
+                              update with LDS-privatised fixed-point moments          */
 
 /* FFT variant used by the spectral solve. */
 #define SS_FFT_AUTO 0  /* autotuned among the variants that pass the accuracy gate */
@@ -95,6 +96,11 @@ typedef struct ss_params {
     int32_t fft_variant; /* SS_FFT_*                   */
     uint32_t flags;      /* SS_FLAG_*                  */
     int32_t device;      /* HIP device ordinal          */
+    /* Tiled path (SS_BIN_SORTED): tile extent in cells per axis (0: chosen
+       automatically) and steps between re-sorts of the particles by tile
+       (0: 16). */
+    int32_t tile[3];
+    int32_t resort_interval;
 } ss_params;
 
 /* Fills p with the reference's defaults: zero optional parameters, ss y-field,

@@ -689,9 +689,11 @@ void o_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[
 }
 
 /* Twelve words per particle and step: counter (j, id, step lo, step hi),
-   j = 0,1,2; key (seed lo, seed hi).  Uniform k takes a 52-bit integer m made
-   of word k (high 32 bits) and the top 20 bits of word 6+k, u = (m + 1/2) 2^-52,
-   so that 0 < u < 1 holds exactly in binary64. */
+   j = 0,1,2; key (seed lo, seed hi).  Six 21-bit fields come from the words
+   w0..w3 of call 0 (the top 21 bits of each word; then the low 11 bits of
+   w0|w1 and of w2|w3, 11 + 10 bits); uniform k is (m_k + 1/2) 2^-52 with m_k
+   the field followed by the top 31 bits of word 4+k (calls 1 and 2).  The
+   uniforms are exactly representable and strictly inside (0, 1). */
 void o_noise(uint64_t seed, uint64_t step, uint64_t id, double dr, double out[5]) {
     uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
     uint32_t w[12];
@@ -699,9 +701,12 @@ void o_noise(uint64_t seed, uint64_t step, uint64_t id, double dr, double out[5]
         uint32_t ctr[4] = {j, (uint32_t)id, (uint32_t)step, (uint32_t)(step >> 32)};
         o_philox4x32_10(ctr, key, w + 4 * j);
     }
+    uint32_t f[6] = {w[0] >> 11, w[1] >> 11, w[2] >> 11, w[3] >> 11,
+                     ((w[0] & 0x7FFu) << 10) | ((w[1] & 0x7FFu) >> 1),
+                     ((w[2] & 0x7FFu) << 10) | ((w[3] & 0x7FFu) >> 1)};
     double u[6];
     for (int k = 0; k < 6; ++k) {
-        uint64_t m = ((uint64_t)w[k] << 20) | (uint64_t)(w[6 + k] >> 12);
+        uint64_t m = ((uint64_t)f[k] << 31) | (uint64_t)(w[4 + k] >> 1);
         u[k] = ((double)m + 0.5) * (1.0 / 4503599627370496.0);
     }
     double r0 = sqrt(-2.0 * log(u[0]));

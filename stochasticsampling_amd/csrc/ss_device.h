@@ -72,58 +72,89 @@ __host__ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
     out[3] = c3;
 }
 
-// Per particle and step: counters (j, id, step lo, step hi), j = 0..2, key
-// (seed lo, seed hi).  FP64: uniform k is (m + 1/2) 2^-52 with m the 52-bit
-// integer of word k (high 32 bits) and the top 20 bits of word 6+k.  FP32: the
-// top 23 bits of word k, (m + 1/2) 2^-23.  Both are exactly representable and
-// strictly inside (0, 1), and the two precisions see the same increments to
-// ~2^-23.
-// Result: RandomVector [x, y, z, axis_angle, rotate_angle] of
-// src/bin/simulation/mod.rs:51-57,259-286 (Box-Muller normals, Rayleigh angle).
-__device__ inline void philox_noise(uint64_t seed, uint64_t step, uint32_t id, double dr,
-                                    double out[5]) {
+// Noise of one particle and step.  sa, ca: sine and cosine of the rotation
+// axis angle, so that generators that draw the angle as a fraction of a turn
+// can evaluate them directly.
+template <typename T>
+struct Noise {
+    T x, y, z;  // standard normals
+    T sa, ca;   // sin and cos of axis_angle
+    T rot;      // rotate_angle
+};
+
+// Philox layout, per particle and step: counter (j, id, step lo, step hi),
+// key (seed lo, seed hi).  Call j = 0 gives four words w0..w3; six 21-bit
+// fields f0..f5 are the top 21 bits of w0..w3 and, for f4 and f5, the low 11
+// bits of w0|w1 and w2|w3 (11 + 10 bits).  FP32 uses u_k = (f_k + 1/2) 2^-21.
+// FP64 appends 31 bits from the top of the words of calls j = 1, 2 (in
+// order) to each f_k and uses u_k = (m_k + 1/2) 2^-52.  Both are exactly
+// representable and strictly inside (0, 1); the FP32 uniforms are the FP64
+// ones rounded to 21 bits, so both precisions see the same increments.
+// Transform: Box-Muller normals x, y from (u0, u1), z from (u2, u3), axis
+// angle 2 pi u4, rotate angle dr sqrt(-2 ln(1 - u5)) -- the RandomVector of
+// src/bin/simulation/mod.rs:51-57,259-286,383-385.
+__host__ __device__ inline void philox_fields(const uint32_t w[4], uint32_t f[6]) {
+    f[0] = w[0] >> 11;
+    f[1] = w[1] >> 11;
+    f[2] = w[2] >> 11;
+    f[3] = w[3] >> 11;
+    f[4] = ((w[0] & 0x7FFu) << 10) | ((w[1] & 0x7FFu) >> 1);
+    f[5] = ((w[2] & 0x7FFu) << 10) | ((w[3] & 0x7FFu) >> 1);
+}
+
+__device__ inline void philox_noise(uint64_t seed, uint64_t step, uint32_t id, double dr, Noise<double>& nz) {
     uint32_t w[12];
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     const uint32_t s0 = (uint32_t)step, s1 = (uint32_t)(step >> 32);
     philox4x32_10(0u, id, s0, s1, k0, k1, w);
     philox4x32_10(1u, id, s0, s1, k0, k1, w + 4);
     philox4x32_10(2u, id, s0, s1, k0, k1, w + 8);
+    uint32_t f[6];
+    philox_fields(w, f);
     double u[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-        uint64_t m = ((uint64_t)w[k] << 20) | (uint64_t)(w[6 + k] >> 12);
+        const uint64_t m = ((uint64_t)f[k] << 31) | (uint64_t)(w[4 + k] >> 1);
         u[k] = ((double)m + 0.5) * (1.0 / 4503599627370496.0);
     }
     const double TWOPI = Math<double>::TWOPI;
-    double r0 = ::sqrt(-2.0 * ::log(u[0]));
-    double r1 = ::sqrt(-2.0 * ::log(u[2]));
-    out[0] = r0 * ::cos(TWOPI * u[1]);
-    out[1] = r0 * ::sin(TWOPI * u[1]);
-    out[2] = r1 * ::cos(TWOPI * u[3]);
-    out[3] = TWOPI * u[4];
-    out[4] = dr * ::sqrt(-2.0 * ::log(1.0 - u[5]));
+    const double r0 = ::sqrt(-2.0 * ::log(u[0]));
+    const double r1 = ::sqrt(-2.0 * ::log(u[2]));
+    nz.x = r0 * ::cos(TWOPI * u[1]);
+    nz.y = r0 * ::sin(TWOPI * u[1]);
+    nz.z = r1 * ::cos(TWOPI * u[3]);
+    ::sincos(TWOPI * u[4], &nz.sa, &nz.ca);
+    nz.rot = dr * ::sqrt(-2.0 * ::log(1.0 - u[5]));
 }
 
-__device__ inline void philox_noise(uint64_t seed, uint64_t step, uint32_t id, float dr,
-                                    float out[5]) {
-    uint32_t w[8];
-    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
-    const uint32_t s0 = (uint32_t)step, s1 = (uint32_t)(step >> 32);
-    philox4x32_10(0u, id, s0, s1, k0, k1, w);
-    philox4x32_10(1u, id, s0, s1, k0, k1, w + 4);
+__device__ inline void philox_noise(uint64_t seed, uint64_t step, uint32_t id, float dr, Noise<float>& nz) {
+    uint32_t w[4];
+    philox4x32_10(0u, id, (uint32_t)step, (uint32_t)(step >> 32), (uint32_t)seed, (uint32_t)(seed >> 32), w);
+    uint32_t f[6];
+    philox_fields(w, f);
     float u[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) u[k] = ((float)(w[k] >> 9) + 0.5f) * (1.0f / 8388608.0f);
-    const float TWOPI = Math<float>::TWOPI;
-    float r0 = sqrtf(-2.0f * logf(u[0]));
-    float r1 = sqrtf(-2.0f * logf(u[2]));
-    float s, c;
-    sincosf(TWOPI * u[1], &s, &c);
-    out[0] = r0 * c;
-    out[1] = r0 * s;
-    out[2] = r1 * cosf(TWOPI * u[3]);
-    out[3] = TWOPI * u[4];
-    out[4] = dr * sqrtf(-2.0f * logf(1.0f - u[5]));
+    for (int k = 0; k < 6; ++k) u[k] = ((float)f[k] + 0.5f) * (1.0f / 2097152.0f);
+    // hardware transcendentals: v_log_f32 is log2, v_sin/v_cos take turns
+    const float m2ln2 = -1.38629436111989061883f;  // -2 ln 2
+    const float r0 = __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(u[0]));
+    const float r1 = __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(u[2]));
+    nz.x = r0 * __builtin_amdgcn_cosf(u[1]);
+    nz.y = r0 * __builtin_amdgcn_sinf(u[1]);
+    nz.z = r1 * __builtin_amdgcn_cosf(u[3]);
+    nz.sa = __builtin_amdgcn_sinf(u[4]);
+    nz.ca = __builtin_amdgcn_cosf(u[4]);
+    nz.rot = dr * __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(1.0f - u[5]));
+}
+
+// Caller-supplied RandomVector [x, y, z, axis_angle, rotate_angle].
+template <typename T>
+__device__ inline void noise_from_vector(const T rv[5], Noise<T>& nz) {
+    nz.x = rv[0];
+    nz.y = rv[1];
+    nz.z = rv[2];
+    Math<T>::sincos(rv[3], &nz.sa, &nz.ca);
+    nz.rot = rv[4];
 }
 
 // ---------------------------------------------------------------------------
@@ -133,6 +164,30 @@ template <typename T>
 __device__ inline T modulo(T f, T m) {
     T r = Math<T>::fmod(f, m);
     return r < T(0) ? r + Math<T>::fabs(m) : r;
+}
+
+// modulo for |f| < 2|m| without the division inside fmod; the same value as
+// modulo() (fmod is exact, and so is f - m for m <= f < 2m).
+template <typename T>
+__device__ inline T modulo_near(T f, T m) {
+    if (f >= m) {
+        const T r = f - m;
+        return r < m ? r : modulo(f, m);
+    }
+    if (f < T(0)) {
+        if (f >= -m) return f + m;
+        return modulo(f, m);
+    }
+    return f;
+}
+
+// ang_pbc of angles just produced by Orientation::from_vector: theta =
+// pi/2 - atan2(z, rxy) lies in [0, pi] and phi = atan2(y, x) in [-pi, pi], so
+// ang_pbc (particle.rs:37-44) reduces to wrapping phi once.
+template <typename T>
+__device__ inline void ang_pbc_from_vector(T phi, T theta, T& ophi, T& otheta) {
+    ophi = phi < T(0) ? phi + Math<T>::TWOPI : phi;
+    otheta = theta;
 }
 
 template <typename T>
@@ -195,8 +250,23 @@ struct UpdateParams {
 // modifiers.rs).  rec: field record of the particle's cell.  rv: RandomVector.
 // Outputs the position increment dp (to be added with pbc) and the new
 // orientation angles after ang_pbc.
+// sin and cos of the half rotation angle: a short Taylor polynomial in FP32
+// (|x| < 0.5, error below 1e-8), the library functions otherwise.
 template <typename T>
-__device__ inline void langevin(const UpdateParams<T>& P, T phi, T theta, const T* rec, const T rv[5],
+__device__ inline void half_angle(T x, T* s, T* c) {
+    if constexpr (sizeof(T) == 4) {
+        if (x < 0.5f && x > -0.5f) {
+            const float x2 = x * x;
+            *s = x * (1.0f + x2 * (-1.0f / 6.0f + x2 * (1.0f / 120.0f - x2 * (1.0f / 5040.0f))));
+            *c = 1.0f + x2 * (-0.5f + x2 * (1.0f / 24.0f + x2 * (-1.0f / 720.0f + x2 * (1.0f / 40320.0f))));
+            return;
+        }
+    }
+    Math<T>::sincos(x, s, c);
+}
+
+template <typename T>
+__device__ inline void langevin(const UpdateParams<T>& P, T phi, T theta, const T* rec, const Noise<T>& rv,
                                 T dp[3], T& ophi, T& otheta) {
     T sp, cp, st, ct;
     Math<T>::sincos(phi, &sp, &cp);
@@ -274,19 +344,18 @@ __device__ inline void langevin(const UpdateParams<T>& P, T phi, T theta, const 
     {
         T diff = P.use_rho ? Math<T>::sqrt(T(2) * P.dt * (P.d_trans + P.vol_excl * rec[REC_RHO]))
                            : Math<T>::sqrt(T(2) * P.dt * P.d_trans);
-        p0 = p0 + rv[0] * diff;
-        p1 = p1 + rv[1] * diff;
-        p2 = p2 + rv[2] * diff;
+        p0 = p0 + rv.x * diff;
+        p1 = p1 + rv.y * diff;
+        p2 = p2 + rv.z * diff;
     }
     // rotational_diffusion (modifiers.rs:189-231): q (0,n) q*, quaternion 0.3.1
     {
-        T sa, ca;
-        Math<T>::sincos(rv[3], &sa, &ca);
+        const T sa = rv.sa, ca = rv.ca;
         const T ax0 = cp * ct * sa - ca * sp;
         const T ax1 = ca * cp + ct * sa * sp;
         const T ax2 = -sa * st;
         T sh, q0;
-        Math<T>::sincos(rv[4] / T(2), &sh, &q0);
+        half_angle<T>(rv.rot / T(2), &sh, &q0);
         const T q1 = ax0 * sh, q2 = ax1 * sh, q3 = ax2 * sh;
         // t = q * (0, n)
         const T t0 = T(0) * q0 - (q1 * n0 + q2 * n1 + q3 * n2);
@@ -307,7 +376,7 @@ __device__ inline void langevin(const UpdateParams<T>& P, T phi, T theta, const 
     const T rxy = Math<T>::sqrt(v0 * v0 + v1 * v1);
     const T nphi = Math<T>::atan2(v1, v0);
     const T ntheta = Math<T>::PI / T(2) - Math<T>::atan2(v2, rxy);
-    ang_pbc(nphi, ntheta, ophi, otheta);
+    ang_pbc_from_vector(nphi, ntheta, ophi, otheta);
     if (P.quasi2d) otheta = Math<T>::PI / T(2);
     dp[0] = p0;
     dp[1] = p1;

@@ -122,47 +122,6 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------
-// Binning, sorted path: a workgroup owns CB consecutive cells, counts the
-// angle bins of their particles (contiguous in memory) in LDS and contracts
-// them.  Reads 2 bytes per particle.
-// ---------------------------------------------------------------------------
-template <typename T, int NM, int CB>
-__global__ void __launch_bounds__(256)
-    k_bin_sorted(const uint16_t* __restrict__ ab, const uint32_t* __restrict__ off, Geo<T> g,
-                 const T* __restrict__ kern_g, T scale, T rho_scale, cplx<T>* __restrict__ spec,
-                 T* __restrict__ rho, uint32_t* __restrict__ hist_out) {
-    extern __shared__ unsigned char smem_raw[];
-    T* kern = reinterpret_cast<T*>(smem_raw);
-    uint32_t* cnt = reinterpret_cast<uint32_t*>(kern + NM * g.na);
-    __shared__ uint32_t soff[CB + 1];
-    const int c0 = blockIdx.x * CB;
-    const int ncell = min(CB, g.ns - c0);
-    for (int t = threadIdx.x; t < NM * g.na; t += blockDim.x) kern[t] = kern_g[t];
-    for (int t = threadIdx.x; t < CB * g.na; t += blockDim.x) cnt[t] = 0u;
-    for (int t = threadIdx.x; t <= ncell; t += blockDim.x) soff[t] = off[c0 + t];
-    __syncthreads();
-    const uint32_t p0 = soff[0], p1 = soff[ncell];
-    for (uint32_t j = p0 + threadIdx.x; j < p1; j += blockDim.x) {
-        int lo = 0, hi = ncell;  // find c with soff[c] <= j < soff[c+1]
-        while (hi - lo > 1) {
-            int mid = (lo + hi) >> 1;
-            if (soff[mid] <= j)
-                lo = mid;
-            else
-                hi = mid;
-        }
-        atomicAdd(cnt + lo * g.na + ab[j], 1u);
-    }
-    __syncthreads();
-    const int wave = threadIdx.x >> 6;
-    for (int c = wave; c < ncell; c += 4)
-        contract_cell<T, NM>(cnt + c * g.na, g.na, kern, scale, rho_scale, c0 + c, g.ns, spec, rho);
-    if (hist_out)
-        for (int t = threadIdx.x; t < ncell * g.na; t += blockDim.x)
-            hist_out[(size_t)c0 * g.na + t] = cnt[t];
-}
-
-// ---------------------------------------------------------------------------
 // k-space solve: fft_mean_flow_field + update_gradient
 // (spectral_solver.rs:133-229) and fft_mean_magnetic_field + update_gradient
 // (magnetic_solver.rs:71-135), one thread per wave vector.
@@ -326,7 +285,7 @@ struct State {
 template <typename T>
 __device__ inline T add_wrap(T x, T d, T L, float* lo) {
     // FP64: Position::pbc after pos + delta (particle.rs:72-76)
-    return modulo(x + d, L);
+    return modulo_near(x + d, L);
 }
 
 template <>
@@ -339,87 +298,79 @@ __device__ inline float add_wrap<float>(float x, float d, float L, float* lo) {
     float l = (x - (t - bp)) + (y - bp);  // TwoSum: t + l == x + y exactly
     // wrap on the value t + l, so that it lands in [0, L); t itself may equal L
     if (t > L || (t == L && l >= 0.0f)) {
-        t = t - L;  // exact (Sterbenz)
+        t = t - L;  // exact (Sterbenz) for t < 2L
     } else if (t < 0.0f || (t == 0.0f && l < 0.0f)) {
         float s = t + L;
         l = l + (t - (s - L));
         t = s;
     }
+    if (!(t >= 0.0f && t <= L)) {  // increment of a box length or more
+        t = modulo(t, L);
+        l = 0.0f;
+    }
     *lo = l;
     return t;
 }
 
-// Update of every particle: gather the cell record, draw the noise, run the
-// LangevinBuilder chain, move, wrap.  With SORTED, also emit the new
-// (cell, angle) key and count particles per new cell.
-template <typename T, bool SORTED, bool EXT_NOISE>
-__global__ void __launch_bounds__(256)
-    k_update(State<T> s, uint64_t n, Geo<T> g, UpdateParams<T> P, const T* __restrict__ rec, int rs,
-             uint64_t seed, uint64_t step, const T* __restrict__ noise, uint32_t* __restrict__ key,
-             uint32_t* __restrict__ cnt) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = i < n;
-    uint32_t newcell = INVALID_KEY;
-    if (valid) {
-        const T x = s.x[i], y = s.y[i], z = s.z[i];
-        const uint32_t id = s.id ? s.id[i] : (uint32_t)i;
-        const int c = cell_of(g, x, y, z);
-        T rv[5];
-        if (EXT_NOISE) {
-#pragma unroll
-            for (int k = 0; k < 5; ++k) rv[k] = noise[(size_t)k * n + id];
-        } else {
-            philox_noise(seed, step, id, P.dr, rv);
+// Update of every particle, plain path (particles in their original order):
+// gather the cell record, draw the noise, run the LangevinBuilder chain,
+// move, wrap.
+template <typename T>
+__device__ inline void move_particle(State<T> s, uint64_t i, uint32_t id, const Geo<T>& g,
+                                     const UpdateParams<T>& P, const T* __restrict__ rec, int rs,
+                                     const Noise<T>& rv, T& nx, T& ny, T& nz, T& ophi, T& otheta) {
+    const T x = s.x[i], y = s.y[i], z = s.z[i];
+    const int c = cell_of(g, x, y, z);
+    T dp[3];
+    langevin<T>(P, s.phi[i], s.theta[i], rec + (size_t)c * rs, rv, dp, ophi, otheta);
+    if constexpr (sizeof(T) == 4) {
+        float lx = s.lx[i], ly = s.ly[i], lz = s.lz[i];
+        nx = add_wrap<T>(x, dp[0], P.box[0], &lx);
+        ny = add_wrap<T>(y, dp[1], P.box[1], &ly);
+        nz = add_wrap<T>(z, dp[2], P.box[2], &lz);
+        if (P.quasi2d) {
+            nz = T(0);
+            lz = 0.0f;
         }
-        T dp[3], ophi, otheta;
-        langevin<T>(P, s.phi[i], s.theta[i], rec + (size_t)c * rs, rv, dp, ophi, otheta);
-        T nx, ny, nz;
-        if constexpr (sizeof(T) == 4) {
-            float lx = s.lx[i], ly = s.ly[i], lz = s.lz[i];
-            nx = add_wrap<T>(x, dp[0], P.box[0], &lx);
-            ny = add_wrap<T>(y, dp[1], P.box[1], &ly);
-            nz = add_wrap<T>(z, dp[2], P.box[2], &lz);
-            if (P.quasi2d) {
-                nz = T(0);
-                lz = 0.0f;
-            }
-            s.lx[i] = lx;
-            s.ly[i] = ly;
-            s.lz[i] = lz;
-        } else {
-            nx = add_wrap<T>(x, dp[0], P.box[0], nullptr);
-            ny = add_wrap<T>(y, dp[1], P.box[1], nullptr);
-            nz = add_wrap<T>(z, dp[2], P.box[2], nullptr);
-            if (P.quasi2d) nz = T(0);
-        }
-        s.x[i] = nx;
-        s.y[i] = ny;
-        s.z[i] = nz;
-        s.phi[i] = ophi;
-        s.theta[i] = otheta;
-        if (SORTED) {
-            newcell = (uint32_t)cell_of(g, nx, ny, nz);
-            key[i] = newcell * (uint32_t)g.na + (uint32_t)angle_of(g, ophi, otheta);
-        }
+        s.lx[i] = lx;
+        s.ly[i] = ly;
+        s.lz[i] = lz;
+    } else {
+        nx = add_wrap<T>(x, dp[0], P.box[0], nullptr);
+        ny = add_wrap<T>(y, dp[1], P.box[1], nullptr);
+        nz = add_wrap<T>(z, dp[2], P.box[2], nullptr);
+        if (P.quasi2d) nz = T(0);
     }
-    if (SORTED) {
-        Run r = wave_run(newcell);
-        if (valid && r.is_head) atomicAdd(cnt + newcell, (uint32_t)r.len);
+    s.x[i] = nx;
+    s.y[i] = ny;
+    s.z[i] = nz;
+    s.phi[i] = ophi;
+    s.theta[i] = otheta;
+}
+
+template <typename T>
+__device__ inline void draw_noise(const UpdateParams<T>& P, uint64_t seed, uint64_t step, uint32_t id,
+                                  uint64_t n, const T* __restrict__ noise, bool ext, Noise<T>& rv) {
+    if (ext) {
+        T v[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) v[k] = noise[(size_t)k * n + id];
+        noise_from_vector<T>(v, rv);
+    } else {
+        philox_noise(seed, step, id, P.dr, rv);
     }
 }
 
-// Keys and per-cell counts of the current state (initial sort).
-template <typename T>
+template <typename T, bool EXT_NOISE>
 __global__ void __launch_bounds__(256)
-    k_keys(State<T> s, uint64_t n, Geo<T> g, uint32_t* __restrict__ key, uint32_t* __restrict__ cnt) {
+    k_update_plain(State<T> s, uint64_t n, Geo<T> g, UpdateParams<T> P, const T* __restrict__ rec, int rs,
+                   uint64_t seed, uint64_t step, const T* __restrict__ noise) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t cell = INVALID_KEY;
-    if (i < n) {
-        cell = (uint32_t)cell_of(g, s.x[i], s.y[i], s.z[i]);
-        key[i] = cell * (uint32_t)g.na + (uint32_t)angle_of(g, s.phi[i], s.theta[i]);
-    }
-    Run r = wave_run(cell);
-    if (i < n && r.is_head) atomicAdd(cnt + cell, (uint32_t)r.len);
+    if (i >= n) return;
+    Noise<T> rv;
+    draw_noise<T>(P, seed, step, (uint32_t)i, n, noise, EXT_NOISE, rv);
+    T nx, ny, nz, ophi, otheta;
+    move_particle<T>(s, i, (uint32_t)i, g, P, rec, rs, rv, nx, ny, nz, ophi, otheta);
 }
 
 // Exclusive scan of the per-cell counts, two kernels: chunk sums, then chunk
@@ -486,36 +437,6 @@ __global__ void __launch_bounds__(256)
         run += v[k];
     }
     if (blockIdx.x == nparts - 1 && threadIdx.x == 255) off[ns] = run;
-}
-
-// Scatter every particle to its slot in cell order: runs of equal cells
-// within a wavefront reserve contiguous slots with one atomic.
-template <typename T>
-__global__ void __launch_bounds__(256)
-    k_permute(State<T> a, State<T> b, uint64_t n, int na, const uint32_t* __restrict__ key,
-              const uint32_t* __restrict__ off, uint32_t* __restrict__ cursor, uint16_t* __restrict__ ab) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = i < n;
-    uint32_t k = valid ? key[i] : INVALID_KEY;
-    uint32_t cell = valid ? k / (uint32_t)na : INVALID_KEY;
-    Run r = wave_run(cell);
-    uint32_t base = 0;
-    if (valid && r.is_head) base = atomicAdd(cursor + cell, (uint32_t)r.len);
-    base = __shfl(base, r.head);
-    if (!valid) return;
-    const uint32_t dst = off[cell] + base + (uint32_t)(__lane_id() - r.head);
-    b.x[dst] = a.x[i];
-    b.y[dst] = a.y[i];
-    b.z[dst] = a.z[i];
-    b.phi[dst] = a.phi[i];
-    b.theta[dst] = a.theta[i];
-    if (a.lx) {
-        b.lx[dst] = a.lx[i];
-        b.ly[dst] = a.ly[i];
-        b.lz[dst] = a.lz[i];
-    }
-    b.id[dst] = a.id ? a.id[i] : (uint32_t)i;
-    ab[dst] = (uint16_t)(k - cell * (uint32_t)na);
 }
 
 // ---------------------------------------------------------------------------
@@ -662,8 +583,10 @@ __global__ void k_debug_update(const double* __restrict__ parts, const double* _
     rec[REC_RHO] = (T)f[21];
     for (int q = 0; q < 3; ++q) rec[REC_B + q] = (T)f[22 + q] * mu_dd;
     for (int q = 0; q < 9; ++q) rec[REC_GB + q] = (T)f[25 + q];
-    T rv[5];
-    for (int k = 0; k < 5; ++k) rv[k] = (T)rvs[5 * i + k];
+    T v[5];
+    for (int k = 0; k < 5; ++k) v[k] = (T)rvs[5 * i + k];
+    Noise<T> rv;
+    noise_from_vector<T>(v, rv);
     const double* p = parts + 5 * i;
     T dp[3], ophi, otheta;
     langevin<T>(P, (T)p[3], (T)p[4], rec, rv, dp, ophi, otheta);

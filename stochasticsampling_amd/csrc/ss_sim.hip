@@ -19,6 +19,7 @@
 
 #include "ss_internal.h"
 #include "ss_kernels.h"
+#include "ss_tiles.h"
 
 namespace ss {
 
@@ -165,11 +166,17 @@ struct ss_sim {
     void *x2 = nullptr, *y2 = nullptr, *z2 = nullptr, *phi2 = nullptr, *theta2 = nullptr;
     float *lx = nullptr, *ly = nullptr, *lz = nullptr, *lx2 = nullptr, *ly2 = nullptr, *lz2 = nullptr;
     uint32_t *id = nullptr, *id2 = nullptr;
-    uint32_t* key = nullptr;
-    uint16_t* ab = nullptr;
-    uint32_t *cnt = nullptr, *off = nullptr, *cursor = nullptr, *part = nullptr;
+    // tiled path
+    TileGeo tg;
+    int resort = 16;             // steps between tile sorts
+    uint64_t steps_since_sort = 0;
+    uint32_t *tcnt = nullptr, *toff = nullptr, *cursor = nullptr, *part = nullptr;
     int nparts = 0;
-    uint32_t* hist = nullptr;
+    long long *mom_main = nullptr, *mom_inflow = nullptr, *kfix = nullptr;
+    double fixed_scale = 1.0;    // value of one fixed-point unit
+    uint32_t* hist = nullptr;       // counts of the last step's binning
+    uint32_t* hist_next = nullptr;  // tiled path: counts of the next step
+    bool swap_hist = false;
     void* kern = nullptr;
     void* spec_in = nullptr;
     void* spec_out = nullptr;
@@ -298,19 +305,20 @@ static hipError_t launch_bin(ss_sim* s) {
             hipLaunchKernelGGL((k_contract_hist<T, 12>), dim3(nb), dim3(256), lds, s->stream, s->hist, g,
                                (const T*)s->kern, (T)ms, (T)ds, (cplx<T>*)s->spec_in, rho);
     } else {
-        constexpr int CB = 16;
-        const unsigned nb = (unsigned)((s->ns + CB - 1) / CB);
-        const size_t lds = (size_t)s->nm * s->na * sizeof(T) + (size_t)CB * s->na * sizeof(uint32_t);
-        uint32_t* hout = (s->p.flags & SS_FLAG_KEEP_DISTRIBUTION) ? s->hist : nullptr;
+        // the moments were accumulated by the previous update (or the initial
+        // binning); convert them
+        const double sc = s->fixed_scale * ms;
+        const double rs = ds;
+        const dim3 grid(blocks_for(s->ns));
         if (s->nm == 6)
-            hipLaunchKernelGGL((k_bin_sorted<T, 6, CB>), dim3(nb), dim3(256), lds, s->stream, s->ab, s->off,
-                               g, (const T*)s->kern, (T)ms, (T)ds, (cplx<T>*)s->spec_in, rho, hout);
+            hipLaunchKernelGGL((k_tile_convert<T, 6>), grid, dim3(256), 0, s->stream, s->mom_main, s->mom_inflow,
+                               s->ns, sc, rs, (cplx<T>*)s->spec_in, rho);
         else if (s->nm == 9)
-            hipLaunchKernelGGL((k_bin_sorted<T, 9, CB>), dim3(nb), dim3(256), lds, s->stream, s->ab, s->off,
-                               g, (const T*)s->kern, (T)ms, (T)ds, (cplx<T>*)s->spec_in, rho, hout);
+            hipLaunchKernelGGL((k_tile_convert<T, 9>), grid, dim3(256), 0, s->stream, s->mom_main, s->mom_inflow,
+                               s->ns, sc, rs, (cplx<T>*)s->spec_in, rho);
         else
-            hipLaunchKernelGGL((k_bin_sorted<T, 12, CB>), dim3(nb), dim3(256), lds, s->stream, s->ab,
-                               s->off, g, (const T*)s->kern, (T)ms, (T)ds, (cplx<T>*)s->spec_in, rho, hout);
+            hipLaunchKernelGGL((k_tile_convert<T, 12>), grid, dim3(256), 0, s->stream, s->mom_main,
+                               s->mom_inflow, s->ns, sc, rs, (cplx<T>*)s->spec_in, rho);
     }
     return hipGetLastError();
 }
@@ -333,48 +341,112 @@ static hipError_t launch_pack(ss_sim* s) {
     return hipGetLastError();
 }
 
+template <typename T, int NM, bool MOVE, bool EXT, bool KEEP>
+static void launch_tile_kernel(ss_sim* s, const TileArgs<T, NM>& a) {
+    const int per = (s->tg.ntiles + s->tg.nxcd - 1) / s->tg.nxcd;
+    const dim3 grid((unsigned)(per * s->tg.nxcd));
+    const size_t lds = (size_t)s->tg.ws * (NM + 1) * 8 + (size_t)NM * s->na * 8;
+    hipLaunchKernelGGL((k_tile_update<T, NM, MOVE, EXT, KEEP>), grid, dim3(256), lds, s->stream, a);
+}
+
+template <typename T, int NM, bool MOVE>
+static void launch_tile_nm(ss_sim* s, const TileArgs<T, NM>& a) {
+    const bool ext = s->p.noise_mode == SS_NOISE_EXTERNAL;
+    const bool keep = a.hist != nullptr;
+    if (ext) {
+        if (keep) launch_tile_kernel<T, NM, MOVE, true, true>(s, a);
+        else launch_tile_kernel<T, NM, MOVE, true, false>(s, a);
+    } else {
+        if (keep) launch_tile_kernel<T, NM, MOVE, false, true>(s, a);
+        else launch_tile_kernel<T, NM, MOVE, false, false>(s, a);
+    }
+}
+
+template <typename T, int NM>
+static TileArgs<T, NM> tile_args(ss_sim* s) {
+    TileArgs<T, NM> a;
+    a.s = state_a<T>(s);
+    a.toff = s->toff;
+    a.n = s->n;
+    a.g = geo<T>(s);
+    a.tg = s->tg;
+    a.P = update_params<T>(s);
+    a.rec = (const T*)s->rec;
+    a.rs = s->rs;
+    a.seed = s->p.seed;
+    a.step = s->timestep;
+    a.noise = (const T*)s->noise;
+    a.kfix = s->kfix;
+    a.main = s->mom_main;
+    a.inflow = s->mom_inflow;
+    a.hist = (s->p.flags & SS_FLAG_KEEP_DISTRIBUTION) ? s->hist_next : nullptr;
+    return a;
+}
+
+// Tiled path: move every particle and bin the new state (MOVE), or only bin
+// the current state (initial sampling).
+template <typename T, bool MOVE>
+static hipError_t launch_tile_update(ss_sim* s) {
+    if (s->hist_next) (void)hipMemsetAsync(s->hist_next, 0, (size_t)s->ns * s->na * 4, s->stream);
+    if (s->nm == 6)
+        launch_tile_nm<T, 6, MOVE>(s, tile_args<T, 6>(s));
+    else if (s->nm == 9)
+        launch_tile_nm<T, 9, MOVE>(s, tile_args<T, 9>(s));
+    else
+        launch_tile_nm<T, 12, MOVE>(s, tile_args<T, 12>(s));
+    return hipGetLastError();
+}
+
 template <typename T>
 static hipError_t launch_update(ss_sim* s) {
+    if (s->sorted) return launch_tile_update<T, true>(s);
     Geo<T> g = geo<T>(s);
     UpdateParams<T> P = update_params<T>(s);
-    const bool ext = s->p.noise_mode == SS_NOISE_EXTERNAL;
     State<T> st = state_a<T>(s);
     dim3 grid(blocks_for(s->n));
-#define SS_UPD(SORTED, EXT)                                                                          \
-    hipLaunchKernelGGL((k_update<T, SORTED, EXT>), grid, dim3(256), 0, s->stream, st, s->n, g, P,     \
-                       (const T*)s->rec, s->rs, s->p.seed, s->timestep, (const T*)s->noise, s->key, s->cnt)
-    if (s->sorted) {
-        if (ext)
-            SS_UPD(true, true);
-        else
-            SS_UPD(true, false);
-    } else {
-        if (ext)
-            SS_UPD(false, true);
-        else
-            SS_UPD(false, false);
-    }
-#undef SS_UPD
+    if (s->p.noise_mode == SS_NOISE_EXTERNAL)
+        hipLaunchKernelGGL((k_update_plain<T, true>), grid, dim3(256), 0, s->stream, st, s->n, g, P,
+                           (const T*)s->rec, s->rs, s->p.seed, s->timestep, (const T*)s->noise);
+    else
+        hipLaunchKernelGGL((k_update_plain<T, false>), grid, dim3(256), 0, s->stream, st, s->n, g, P,
+                           (const T*)s->rec, s->rs, s->p.seed, s->timestep, (const T*)s->noise);
     return hipGetLastError();
 }
 
+// Counting sort of the particles by tile.  from_state: count from the
+// particle state (initial sort) instead of the moment arrays.
 template <typename T>
-static hipError_t launch_sort(ss_sim* s) {
-    hipLaunchKernelGGL(k_scan_reduce, dim3(s->nparts), dim3(256), 0, s->stream, s->cnt, s->ns, s->part);
-    hipLaunchKernelGGL(k_scan_apply, dim3(s->nparts), dim3(256), 0, s->stream, s->cnt, s->ns, s->part,
-                       s->nparts, s->off, s->cursor);
-    hipLaunchKernelGGL(k_permute<T>, dim3(blocks_for(s->n)), dim3(256), 0, s->stream, state_a<T>(s),
-                       state_b<T>(s), s->n, s->na, s->key, s->off, s->cursor, s->ab);
+static hipError_t launch_sort(ss_sim* s, bool from_state) {
+    const int nt = s->tg.ntiles;
+    if (from_state) {
+        (void)hipMemsetAsync(s->tcnt, 0, sizeof(uint32_t) * nt, s->stream);
+        hipLaunchKernelGGL(k_tile_count_state<T>, dim3(blocks_for(s->n)), dim3(256), 0, s->stream,
+                           state_a<T>(s), s->n, geo<T>(s), s->tg, s->tcnt);
+    } else {
+        const dim3 grid(blocks_for(nt));
+        if (s->nm == 6)
+            hipLaunchKernelGGL(k_tile_counts<6>, grid, dim3(256), 0, s->stream, s->mom_main, s->mom_inflow,
+                               s->nx, s->ny, s->nz, s->tg, s->tcnt);
+        else if (s->nm == 9)
+            hipLaunchKernelGGL(k_tile_counts<9>, grid, dim3(256), 0, s->stream, s->mom_main, s->mom_inflow,
+                               s->nx, s->ny, s->nz, s->tg, s->tcnt);
+        else
+            hipLaunchKernelGGL(k_tile_counts<12>, grid, dim3(256), 0, s->stream, s->mom_main, s->mom_inflow,
+                               s->nx, s->ny, s->nz, s->tg, s->tcnt);
+    }
+    hipLaunchKernelGGL(k_scan_reduce, dim3(s->nparts), dim3(256), 0, s->stream, s->tcnt, nt, s->part);
+    hipLaunchKernelGGL(k_scan_apply, dim3(s->nparts), dim3(256), 0, s->stream, s->tcnt, nt, s->part, s->nparts,
+                       s->toff, s->cursor);
+    if (from_state)
+        hipLaunchKernelGGL(k_tile_permute<T>, dim3(blocks_for(s->n)), dim3(256), 0, s->stream, state_a<T>(s),
+                           state_b<T>(s), s->n, geo<T>(s), s->tg, s->toff, s->cursor);
+    else
+        hipLaunchKernelGGL(k_tile_permute_agg<T>, dim3(blocks_for(s->n, 4 * 64 * PW_ITER)), dim3(256), 0,
+                           s->stream, state_a<T>(s), state_b<T>(s), s->n, geo<T>(s), s->tg, s->toff, s->cursor);
     hipError_t e = hipGetLastError();
     swap_ab(s);
+    s->steps_since_sort = 0;
     return e;
-}
-
-template <typename T>
-static hipError_t launch_keys(ss_sim* s) {
-    hipLaunchKernelGGL(k_keys<T>, dim3(blocks_for(s->n)), dim3(256), 0, s->stream, state_a<T>(s), s->n,
-                       geo<T>(s), s->key, s->cnt);
-    return hipGetLastError();
 }
 
 enum Stage { ST_BIN = 0, ST_FWD, ST_SOLVE, ST_INV, ST_PACK, ST_UPDATE, ST_SORT, ST_COUNT };
@@ -389,6 +461,7 @@ static int step_once(ss_sim* s, hipEvent_t* ev) {
         if (ev) (void)hipEventRecord(ev[st + 1], s->stream);
     };
     if (ev) (void)hipEventRecord(ev[0], s->stream);
+    if (s->swap_hist) std::swap(s->hist, s->hist_next);
     if ((e = launch_bin<T>(s)) != hipSuccess) return fail(SS_ERR_DEVICE, std::string("bin: ") + hipGetErrorString(e));
     mark(ST_BIN);
     int rc = s->fft.run(s->spec_in, s->nm, -1, s->stream);
@@ -406,8 +479,12 @@ static int step_once(ss_sim* s, hipEvent_t* ev) {
         return fail(SS_ERR_DEVICE, std::string("update: ") + hipGetErrorString(e));
     mark(ST_UPDATE);
     if (s->sorted) {
-        if ((e = launch_sort<T>(s)) != hipSuccess)
-            return fail(SS_ERR_DEVICE, std::string("sort: ") + hipGetErrorString(e));
+        s->steps_since_sort += 1;
+        if (s->steps_since_sort >= (uint64_t)s->resort) {
+            if ((e = launch_sort<T>(s, false)) != hipSuccess)
+                return fail(SS_ERR_DEVICE, std::string("sort: ") + hipGetErrorString(e));
+        }
+        s->swap_hist = s->hist_next != nullptr;
     }
     mark(ST_SORT);
     s->timestep += 1;
@@ -417,18 +494,20 @@ static int step_once(ss_sim* s, hipEvent_t* ev) {
 }
 
 // Sample the distribution of the current particles (Simulation::init,
-// src/bin/simulation/mod.rs:190-195): sorts on the sorted path and fills the
-// count grid where one is kept.
+// src/bin/simulation/mod.rs:190-195).  Tiled path: sort by tile, then bin
+// without moving; the next step converts the accumulated moments.
 template <typename T>
 static int initial_binning(ss_sim* s) {
     hipError_t e;
     if (s->sorted) {
-        if ((e = hipMemsetAsync(s->cnt, 0, sizeof(uint32_t) * s->ns, s->stream)) != hipSuccess)
-            return fail(SS_ERR_DEVICE, "memset");
-        if ((e = launch_keys<T>(s)) != hipSuccess) return fail(SS_ERR_DEVICE, hipGetErrorString(e));
-        if ((e = launch_sort<T>(s)) != hipSuccess) return fail(SS_ERR_DEVICE, hipGetErrorString(e));
+        if ((e = launch_sort<T>(s, true)) != hipSuccess) return fail(SS_ERR_DEVICE, hipGetErrorString(e));
+        SS_HIP_CHECK(hipMemsetAsync(s->mom_inflow, 0, (size_t)s->ns * (s->nm + 1) * 8, s->stream));
+        if ((e = launch_tile_update<T, false>(s)) != hipSuccess) return fail(SS_ERR_DEVICE, hipGetErrorString(e));
+        if (s->hist_next) std::swap(s->hist, s->hist_next);
+        s->swap_hist = false;
+    } else {
+        if ((e = launch_bin<T>(s)) != hipSuccess) return fail(SS_ERR_DEVICE, hipGetErrorString(e));
     }
-    if ((e = launch_bin<T>(s)) != hipSuccess) return fail(SS_ERR_DEVICE, hipGetErrorString(e));
     s->have_hist = !s->sorted || (s->p.flags & SS_FLAG_KEEP_DISTRIBUTION);
     return SS_OK;
 }
@@ -524,14 +603,16 @@ static int create_impl(ss_sim* s) {
         }
         A(s->id, n * 4);
         A(s->id2, n * 4);
-        A(s->key, n * 4);
-        A(s->ab, n * 2);
-        A(s->cnt, (size_t)s->ns * 4);
-        A(s->off, ((size_t)s->ns + 1) * 4);
-        A(s->cursor, (size_t)s->ns * 4);
-        s->nparts = (s->ns + SCAN_CHUNK - 1) / SCAN_CHUNK;
+        const int nt = s->tg.ntiles;
+        A(s->tcnt, (size_t)nt * 4);
+        A(s->toff, ((size_t)nt + 1) * 4);
+        A(s->cursor, (size_t)nt * 4);
+        s->nparts = (nt + SCAN_CHUNK - 1) / SCAN_CHUNK;
         A(s->part, (size_t)s->nparts * 4);
-        SS_HIP_CHECK(hipMemset(s->cnt, 0, (size_t)s->ns * 4));
+        A(s->mom_main, (size_t)s->ns * (s->nm + 1) * 8);
+        A(s->mom_inflow, (size_t)s->ns * (s->nm + 1) * 8);
+        SS_HIP_CHECK(hipMemset(s->mom_inflow, 0, (size_t)s->ns * (s->nm + 1) * 8));
+        if (p.flags & SS_FLAG_KEEP_DISTRIBUTION) A(s->hist_next, (size_t)s->ns * s->na * 4);
     }
     if (!s->sorted || (p.flags & SS_FLAG_KEEP_DISTRIBUTION)) A(s->hist, (size_t)s->ns * s->na * 4);
     const size_t cs = 2 * sizeof(T) * (size_t)s->ns;
@@ -578,7 +659,67 @@ static int create_impl(ss_sim* s) {
     moment_kernels(p, s->nsig, s->mag, kern);
     if ((rc = upload<T>(&s->kern, kern)) != SS_OK) return rc;
     s->owned.push_back(s->kern);
+    if (s->sorted) {
+        // Fixed-point moment kernels: the largest possible cell sum,
+        // N * max|K|, stays below 2^62.
+        double kmax = 1.0;
+        for (double v : kern) kmax = std::max(kmax, std::fabs(v));
+        const int shift = (int)std::floor(62.0 - std::log2((double)s->n * kmax));
+        s->fixed_scale = std::ldexp(1.0, -shift);
+        std::vector<long long> kf(kern.size());
+        for (size_t i = 0; i < kern.size(); ++i) kf[i] = std::llround(std::ldexp(kern[i], shift));
+        if ((rc = alloc(s, (void**)&s->kfix, kf.size() * 8)) != SS_OK) return rc;
+        SS_HIP_CHECK(hipMemcpy(s->kfix, kf.data(), kf.size() * 8, hipMemcpyHostToDevice));
+    }
     return s->fft.init(s->nx, s->ny, s->nz, s->prec);
+}
+
+// Tile geometry of the tiled path.  Unless given, tiles start at 4^3 cells and
+// shrink until there are at least 2048 of them (one workgroup each).  An axis
+// whose length is at most tile + 1 is covered by one tile without halo.
+static int choose_tiles(ss_sim* s) {
+    const int n[3] = {s->nx, s->ny, s->nz};
+    int t[3];
+    const bool given = s->p.tile[0] > 0 && s->p.tile[1] > 0 && s->p.tile[2] > 0;
+    for (int a = 0; a < 3; ++a) t[a] = given ? s->p.tile[a] : 4;
+    auto count = [&]() {
+        long long c = 1;
+        for (int a = 0; a < 3; ++a) {
+            int ta = std::min(t[a], n[a]);
+            if (n[a] <= ta + 1) ta = n[a];
+            c *= (n[a] + ta - 1) / ta;
+        }
+        return c;
+    };
+    while (!given && count() < 2048) {
+        int best = -1;
+        for (int a = 0; a < 3; ++a)
+            if (t[a] > 1 && (best < 0 || t[a] > t[best])) best = a;
+        if (best < 0) break;
+        t[best] /= 2;
+    }
+    TileGeo& tg = s->tg;
+    tg.ntiles = 1;
+    tg.ws = 1;
+    for (int a = 0; a < 3; ++a) {
+        int ta = std::max(1, std::min(t[a], n[a]));
+        if (n[a] <= ta + 1) ta = n[a];
+        tg.t[a] = ta;
+        tg.h[a] = ta == n[a] ? 0 : 1;
+        tg.w[a] = ta + 2 * tg.h[a];
+        tg.nt[a] = (n[a] + ta - 1) / ta;
+        tg.ntiles *= tg.nt[a];
+        tg.ws *= tg.w[a];
+    }
+    tg.nxcd = 8;
+    s->resort = s->p.resort_interval > 0 ? s->p.resort_interval : 16;
+    const size_t lds = (size_t)tg.ws * (s->nm + 1) * 8 + (size_t)s->nm * s->na * 8;
+    if (lds > 65536) {
+        if (s->p.binning == SS_BIN_SORTED)
+            return fail(SS_ERR_UNSUPPORTED, "tile window and angle bins exceed the LDS of the tiled binning");
+        s->sorted = false;
+    }
+    return SS_OK;
 }
 
 static int validate(const ss_params* p) {
@@ -589,7 +730,7 @@ static int validate(const ss_params* p) {
     if (!(p->box_size.x > 0) || !(p->box_size.y > 0) || !(p->box_size.z > 0))
         return fail(SS_ERR_INVALID, "box sizes must be > 0");
     if (p->number_of_particles < 1) return fail(SS_ERR_INVALID, "number_of_particles must be >= 1");
-    if (p->number_of_particles >= (1ull << 32)) return fail(SS_ERR_UNSUPPORTED, "at most 2^32-1 particles");
+    if (p->number_of_particles >= (1ull << 31)) return fail(SS_ERR_UNSUPPORTED, "at most 2^31-1 particles");
     if (p->precision != SS_FP64 && p->precision != SS_FP32) return fail(SS_ERR_INVALID, "bad precision");
     if (p->noise_mode != SS_NOISE_PHILOX && p->noise_mode != SS_NOISE_EXTERNAL)
         return fail(SS_ERR_INVALID, "bad noise_mode");
@@ -666,22 +807,16 @@ int ss_create(const ss_params* params, ss_sim** out) {
     s->nm = s->nsig + (s->mag ? 3 : 0);
     s->nout = 6 + (s->use_e ? 6 : 0) + (s->use_b ? 3 : 0) + (s->use_gb ? 9 : 0);
     s->rs = (s->use_e || s->use_rho || s->use_b || s->use_gb) ? REC_FULL : REC_SMALL;
-    {
-        // LDS of the contraction kernels: the moment kernels (nm x na) plus, on
-        // the sorted path, 16 cells of u32 angle counts; at most 64 KiB.
-        const size_t kern_lds = (size_t)s->nm * s->na * s->esz;
-        const size_t sorted_lds = kern_lds + (size_t)16 * s->na * 4;
-        if (kern_lds > 65536) {
+    if (s->sorted) {
+        int rc2 = choose_tiles(s);
+        if (rc2 != SS_OK) {
             delete s;
-            return fail(SS_ERR_UNSUPPORTED, "too many angle bins for the LDS-resident moment kernels");
+            return rc2;
         }
-        if (s->sorted && sorted_lds > 65536) {
-            if (p.binning == SS_BIN_SORTED) {
-                delete s;
-                return fail(SS_ERR_UNSUPPORTED, "too many angle bins for the sorted binning kernel");
-            }
-            s->sorted = false;
-        }
+    }
+    if ((size_t)s->nm * s->na * s->esz > 65536) {
+        delete s;
+        return fail(SS_ERR_UNSUPPORTED, "too many angle bins for the LDS-resident moment kernels");
     }
     if (hipStreamCreate(&s->stream) != hipSuccess) {
         delete s;

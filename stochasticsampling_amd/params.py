@@ -72,6 +72,8 @@ class Params(C.Structure):
         ("fft_variant", C.c_int32),
         ("flags", C.c_uint32),
         ("device", C.c_int32),
+        ("tile", C.c_int32 * 3),
+        ("resort_interval", C.c_int32),
     ]
 
     @property
@@ -118,6 +120,8 @@ def make_params(
     fft_variant=FFT_AUTO,
     flags=0,
     device=0,
+    tile=(0, 0, 0),
+    resort_interval=0,
 ):
     """Build a ``Params``; defaults are the small CPU-runnable configuration
     (L=10, 16^3 x 8 x 8, 2^16 particles, dt=1e-3, D_t=0.1, D_r=0.5, alpha=-1)."""
@@ -144,4 +148,6 @@ def make_params(
     p.fft_variant = fft_variant
     p.flags = flags
     p.device = device
+    p.tile = (C.c_int32 * 3)(*tile)
+    p.resort_interval = resort_interval
     return p

@@ -265,6 +265,43 @@ def test_sorted_and_histogram_paths_identical_fp64():
     assert np.max(np.abs(out[0][0] - out[1][0])) < 1e-12
 
 
+@pytest.mark.parametrize("precision", [ss.FP64, ss.FP32])
+def test_tiled_path_independent_of_sort_interval_and_tiles(precision):
+    """The tiled binning sums exact fixed-point integers, so neither the
+    re-sort interval nor the tile shape changes a single bit of the result."""
+    parts = ss.initial_condition(small_params(grid=(16, 16, 16, 8, 8), n=1 << 14), amplitude=0.05, seed=6)
+    out = []
+    for tile, resort in [((4, 4, 4), 1), ((4, 4, 4), 7), ((2, 2, 2), 16), ((16, 4, 2), 3), ((1, 1, 1), 5)]:
+        p = small_params(grid=(16, 16, 16, 8, 8), n=1 << 14, precision=precision, binning=ss.BIN_SORTED,
+                         tile=tile, resort_interval=resort, flags=ss.FLAG_KEEP_DISTRIBUTION)
+        sim = ss.Simulation(p)
+        sim.init(parts)
+        sim.step(9)
+        out.append((sim.get_particles(), sim.get_counts(), sim.get_flow_field()))
+    for o in out[1:]:
+        np.testing.assert_array_equal(o[0], out[0][0])
+        np.testing.assert_array_equal(o[1], out[0][1])
+        np.testing.assert_array_equal(o[2], out[0][2])
+
+
+def test_tiled_quasi2d_and_odd_grid(oracle):
+    for grid, box, flags in [((12, 10, 1, 6, 4), (10.0, 8.0, 1.0), ss.FLAG_QUASI2D), ((11, 9, 7, 5, 3),
+                                                                                      (11.0, 9.0, 7.0), 0)]:
+        kw = dict(grid=grid, box=box, n=3000, flags=flags)
+        p = small_params(binning=ss.BIN_SORTED, **kw)
+        p.flags |= ss.FLAG_KEEP_DISTRIBUTION
+        parts = ss.initial_condition(small_params(**kw), amplitude=0.0, seed=8)
+        sim = ss.Simulation(p)
+        sim.init(parts)
+        osim = oracle.simulation(small_params(**kw))
+        osim.init(parts)
+        sim.step(3)
+        osim.step(3)
+        np.testing.assert_array_equal(sim.get_counts(), osim.counts())
+        a, b = sim.get_particles(), osim.particles()
+        assert np.max(periodic_diff(a[:, 0], b[:, 0], box[0])) < 1e-9
+
+
 def test_external_noise_equals_philox_fp64(oracle):
     p = small_params(n=1024)
     parts = ss.initial_condition(p, amplitude=0.05, seed=3)
