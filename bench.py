@@ -85,6 +85,24 @@ def cpu_baseline(parts, threads, steps):
     return parts.shape[0] / per_step, per_step
 
 
+def copy_probe(gib=1.0, reps=10):
+    """Device-to-device copy of a known size (torch), for the bandwidth
+    reference and the calibration of the HBM byte counters."""
+    import torch
+
+    n = int(gib * (1 << 30)) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"probe": "copy", "bytes_per_copy": 2 * 4 * n, "GBps": 2 * 4 * n / dt / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,7 +114,11 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true", help="only run a 1 GiB device copy")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU sharding "
               "of this path is a design note, not a measured configuration.", file=sys.stderr)
@@ -122,17 +144,23 @@ def main():
         fp64 = {"value": n / (ms64 * 1e-3), "ms_per_step": ms64, "steps": max(1, args.steps // 4),
                 "dtype": "f64", "binning": "histogram", "stages_ms": st64}
 
-    # roofline of the dominant kernel (the particle update)
+    # roofline of the dominant kernel (the particle update); traffic comes from
+    # the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this kernel
     upd_ms = st32.get("update", float("nan"))
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_tile_update_fp32_2e24_64c16a.json")
+    if os.path.exists(tpath) and n == 1 << 24:
+        with open(tpath) as f:
+            traffic = json.load(f).get("traffic_bytes_per_launch")
     achieved = UPDATE_BYTES["f32"] * n / (upd_ms * 1e-3) / 1e9
     roofline = {
-        "kernel": "k_update<float,sorted,philox>",
+        "kernel": "ss::k_tile_update<float, 6, true, false, false>",
         "bound": "hbm",
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": None,
+        "traffic": traffic,
         "bytes_per_particle": UPDATE_BYTES["f32"],
         "launch_ms": upd_ms,
     }
@@ -164,7 +192,7 @@ def main():
         "config": {
             "workload": "2^24 particles, grid 64^3 x 16 x 16, L=50, dt=1e-3, D_t=0.1, D_r=0.5, "
                         "alpha=-1, FP32 states with compensated positions, FP32 FFT, Philox noise "
-                        "seed 2, particles sorted by cell",
+                        "seed 2, particles grouped by 4^3-cell tiles",
             "particles": n,
             "grid": [64, 64, 64, 16, 16],
             "box": [50.0, 50.0, 50.0],

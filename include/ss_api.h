@@ -103,7 +103,7 @@ typedef struct ss_params {
     int32_t resort_interval;
 } ss_params;
 
-/* Fills p with the reference's defaults: zero optional parameters, ss y-field,
+/* Fills p with the reference's defaults: zero optional parameters, field along y,
    FP64, Philox noise, automatic binning and FFT, device 0. */
 void ss_params_default(ss_params* p);
 
