@@ -53,6 +53,8 @@ extern "C" {
 #define SS_FLAG_QUASI2D 1u            /* z = 0, theta = pi/2 after every step (feature "quasi2d") */
 #define SS_FLAG_MAGNETIC_ALWAYS 2u    /* run the magnetic solve even when its coefficients are 0  */
 #define SS_FLAG_KEEP_DISTRIBUTION 4u  /* keep the full count grid of every step's binning         */
+#define SS_FLAG_NO_GRAPH 8u           /* enqueue every step eagerly instead of replaying captured
+                                         hipGraphs of whole re-sort periods                       */
 
 /* Size of the simulation box (lib.rs BoxSize). */
 typedef struct ss_box {

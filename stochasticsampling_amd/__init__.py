@@ -8,6 +8,7 @@ from .params import (  # noqa: F401
     FFT_RADIX,
     FLAG_KEEP_DISTRIBUTION,
     FLAG_MAGNETIC_ALWAYS,
+    FLAG_NO_GRAPH,
     FLAG_QUASI2D,
     FP32,
     FP64,

@@ -364,13 +364,24 @@ __device__ inline void draw_noise(const UpdateParams<T>& P, uint64_t seed, uint6
 template <typename T, bool EXT_NOISE>
 __global__ void __launch_bounds__(256)
     k_update_plain(State<T> s, uint64_t n, Geo<T> g, UpdateParams<T> P, const T* __restrict__ rec, int rs,
-                   uint64_t seed, uint64_t step, const T* __restrict__ noise) {
+                   uint64_t seed, const uint64_t* __restrict__ step, uint32_t step_off,
+                   const T* __restrict__ noise) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Noise<T> rv;
-    draw_noise<T>(P, seed, step, (uint32_t)i, n, noise, EXT_NOISE, rv);
+    draw_noise<T>(P, seed, *step + step_off, (uint32_t)i, n, noise, EXT_NOISE, rv);
     T nx, ny, nz, ophi, otheta;
     move_particle<T>(s, i, (uint32_t)i, g, P, rec, rs, rv, nx, ny, nz, ophi, otheta);
+}
+
+// Fill and counter kernels (graph-capturable replacements of memset).
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_add_step(uint64_t* step, uint32_t k) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *step += k;
 }
 
 // Exclusive scan of the per-cell counts, two kernels: chunk sums, then chunk

@@ -177,6 +177,14 @@ struct ss_sim {
     uint32_t* hist = nullptr;       // counts of the last step's binning
     uint32_t* hist_next = nullptr;  // tiled path: counts of the next step
     bool swap_hist = false;
+    // device step counter and captured graphs of one re-sort period
+    uint64_t* d_step = nullptr;
+    uint32_t step_off = 0;        // offset of the step being enqueued within a capture
+    bool capturing = false;
+    bool use_graph = true;
+    void* x_buf0 = nullptr;       // the first x buffer: tells which state buffer is current
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_steps = 0;
     void* kern = nullptr;
     void* spec_in = nullptr;
     void* spec_out = nullptr;
@@ -282,6 +290,11 @@ static void scales(const ss_sim* s, double& moment, double& density) {
     density = vol / den * (gw.theta * gw.phi);
 }
 
+static void fill_u32(ss_sim* s, uint32_t* p, size_t n, uint32_t v) {
+    const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_fill_u32, dim3(std::max(1u, grid)), dim3(256), 0, s->stream, p, n, v);
+}
+
 template <typename T>
 static hipError_t launch_bin(ss_sim* s) {
     Geo<T> g = geo<T>(s);
@@ -289,7 +302,7 @@ static hipError_t launch_bin(ss_sim* s) {
     scales(s, ms, ds);
     T* rho = s->use_rho ? (T*)s->rho : nullptr;
     if (!s->sorted) {
-        (void)hipMemsetAsync(s->hist, 0, (size_t)s->ns * s->na * sizeof(uint32_t), s->stream);
+        fill_u32(s, s->hist, (size_t)s->ns * s->na, 0u);
         hipLaunchKernelGGL(k_bin_hist<T>, dim3(blocks_for(s->n)), dim3(256), 0, s->stream, (const T*)s->x,
                            (const T*)s->y, (const T*)s->z, (const T*)s->phi, (const T*)s->theta, s->n, g,
                            s->hist);
@@ -374,7 +387,8 @@ static TileArgs<T, NM> tile_args(ss_sim* s) {
     a.rec = (const T*)s->rec;
     a.rs = s->rs;
     a.seed = s->p.seed;
-    a.step = s->timestep;
+    a.step = s->d_step;
+    a.step_off = s->step_off;
     a.noise = (const T*)s->noise;
     a.kfix = s->kfix;
     a.main = s->mom_main;
@@ -387,7 +401,7 @@ static TileArgs<T, NM> tile_args(ss_sim* s) {
 // the current state (initial sampling).
 template <typename T, bool MOVE>
 static hipError_t launch_tile_update(ss_sim* s) {
-    if (s->hist_next) (void)hipMemsetAsync(s->hist_next, 0, (size_t)s->ns * s->na * 4, s->stream);
+    if (s->hist_next) fill_u32(s, s->hist_next, (size_t)s->ns * s->na, 0u);
     if (s->nm == 6)
         launch_tile_nm<T, 6, MOVE>(s, tile_args<T, 6>(s));
     else if (s->nm == 9)
@@ -406,10 +420,10 @@ static hipError_t launch_update(ss_sim* s) {
     dim3 grid(blocks_for(s->n));
     if (s->p.noise_mode == SS_NOISE_EXTERNAL)
         hipLaunchKernelGGL((k_update_plain<T, true>), grid, dim3(256), 0, s->stream, st, s->n, g, P,
-                           (const T*)s->rec, s->rs, s->p.seed, s->timestep, (const T*)s->noise);
+                           (const T*)s->rec, s->rs, s->p.seed, s->d_step, s->step_off, (const T*)s->noise);
     else
         hipLaunchKernelGGL((k_update_plain<T, false>), grid, dim3(256), 0, s->stream, st, s->n, g, P,
-                           (const T*)s->rec, s->rs, s->p.seed, s->timestep, (const T*)s->noise);
+                           (const T*)s->rec, s->rs, s->p.seed, s->d_step, s->step_off, (const T*)s->noise);
     return hipGetLastError();
 }
 
@@ -419,7 +433,7 @@ template <typename T>
 static hipError_t launch_sort(ss_sim* s, bool from_state) {
     const int nt = s->tg.ntiles;
     if (from_state) {
-        (void)hipMemsetAsync(s->tcnt, 0, sizeof(uint32_t) * nt, s->stream);
+        fill_u32(s, s->tcnt, (size_t)nt, 0u);
         hipLaunchKernelGGL(k_tile_count_state<T>, dim3(blocks_for(s->n)), dim3(256), 0, s->stream,
                            state_a<T>(s), s->n, geo<T>(s), s->tg, s->tcnt);
     } else {
@@ -487,6 +501,11 @@ static int step_once(ss_sim* s, hipEvent_t* ev) {
         s->swap_hist = s->hist_next != nullptr;
     }
     mark(ST_SORT);
+    if (s->capturing) {
+        s->step_off += 1;
+    } else {
+        hipLaunchKernelGGL(k_add_step, dim3(1), dim3(64), 0, s->stream, s->d_step, 1u);
+    }
     s->timestep += 1;
     s->have_fields = true;
     s->have_hist = !s->sorted || (s->p.flags & SS_FLAG_KEEP_DISTRIBUTION);
@@ -558,6 +577,74 @@ static int copy_real(ss_sim* s, const void* spec, int comp0, int ncomp, double* 
     return SS_OK;
 }
 
+// Steps per captured graph: one re-sort period on the tiled path (the sort,
+// and with it the swap of the state buffers, is the period's last node), 8
+// steps on the plain path.
+static int graph_period(const ss_sim* s) { return s->sorted ? s->resort : 8; }
+
+// Capture one period of steps into graph[parity].  Capturing enqueues nothing;
+// it advances the host-side bookkeeping exactly as running the period would,
+// so the caller must launch the graph once afterwards.
+template <typename T>
+static int capture_period(ss_sim* s, int parity) {
+    const int K = graph_period(s);
+    SS_HIP_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    s->capturing = true;
+    s->step_off = 0;
+    int rc = SS_OK;
+    for (int k = 0; k < K && rc == SS_OK; ++k) rc = step_once<T>(s, nullptr);
+    if (rc == SS_OK) hipLaunchKernelGGL(k_add_step, dim3(1), dim3(64), 0, s->stream, s->d_step, (uint32_t)K);
+    s->capturing = false;
+    s->step_off = 0;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s->stream, &g);
+    if (rc != SS_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(SS_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&s->graph[parity], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        s->graph[parity] = nullptr;
+        return fail(SS_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+    s->graph_steps = K;
+    return SS_OK;
+}
+
+// nsteps steps: eager steps up to the next period boundary, whole periods as
+// graph launches, eager steps for the rest.
+template <typename T>
+static int run_steps(ss_sim* s, uint64_t nsteps) {
+    const int K = graph_period(s);
+    while (nsteps > 0) {
+        const bool aligned = !s->sorted || s->steps_since_sort == 0;
+        if (s->use_graph && aligned && nsteps >= (uint64_t)K) {
+            const int parity = s->x == s->x_buf0 ? 0 : 1;
+            if (!s->graph[parity]) {
+                int rc = capture_period<T>(s, parity);
+                if (rc != SS_OK) return rc;
+            } else {
+                // advance the bookkeeping the captured period performs
+                for (int k = 0; k < K; ++k) s->timestep += 1;
+                if (s->sorted) {
+                    swap_ab(s);
+                    s->steps_since_sort = 0;
+                }
+                s->have_fields = true;
+            }
+            SS_HIP_CHECK(hipGraphLaunch(s->graph[parity], s->stream));
+            nsteps -= (uint64_t)K;
+        } else {
+            int rc = step_once<T>(s, nullptr);
+            if (rc != SS_OK) return rc;
+            nsteps -= 1;
+        }
+    }
+    return SS_OK;
+}
+
 static int alloc(ss_sim* s, void** p, size_t bytes) {
     hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 16));
     if (e != hipSuccess) return fail(SS_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
@@ -565,7 +652,16 @@ static int alloc(ss_sim* s, void** p, size_t bytes) {
     return SS_OK;
 }
 
+static void drop_graphs(ss_sim* s) {
+    for (int k = 0; k < 2; ++k)
+        if (s->graph[k]) {
+            (void)hipGraphExecDestroy(s->graph[k]);
+            s->graph[k] = nullptr;
+        }
+}
+
 static void free_all(ss_sim* s) {
+    drop_graphs(s);
     for (void* p : s->owned) (void)hipFree(p);
     s->owned.clear();
     s->fft.release();
@@ -620,6 +716,9 @@ static int create_impl(ss_sim* s) {
     A(s->spec_out, cs * s->nout);
     A(s->spec_full, cs * (12 + (s->mag ? 12 : 0)));
     if (s->use_rho) A(s->rho, (size_t)s->ns * sizeof(T));
+    s->x_buf0 = s->x;
+    A(s->d_step, sizeof(uint64_t));
+    SS_HIP_CHECK(hipMemset(s->d_step, 0, sizeof(uint64_t)));
     A(s->rec, (size_t)s->ns * s->rs * sizeof(T));
     SS_HIP_CHECK(hipMemset(s->rec, 0, (size_t)s->ns * s->rs * sizeof(T)));
     SS_HIP_CHECK(hipMemset(s->spec_in, 0, cs * s->nm));
@@ -800,6 +899,7 @@ int ss_create(const ss_params* params, ss_sim** out) {
     s->na = s->np * s->nt;
     s->use_e = p.shape != 0.0;
     s->use_rho = p.volume_exclusion != 0.0;
+    s->use_graph = !(p.flags & (SS_FLAG_KEEP_DISTRIBUTION | SS_FLAG_NO_GRAPH));
     s->use_b = p.magnetic_dipole_dipole != 0.0;
     s->use_gb = p.magnetic_drag != 0.0;
     s->mag = s->use_b || s->use_gb || (p.flags & SS_FLAG_MAGNETIC_ALWAYS);
@@ -855,6 +955,10 @@ int ss_init(ss_sim* s, const double* particles, uint64_t n) {
     if (rc != SS_OK) return rc;
     SS_HIP_CHECK(hipStreamSynchronize(s->stream));
     s->timestep = 0;
+    {
+        const uint64_t z = 0;
+        SS_HIP_CHECK(hipMemcpy(s->d_step, &z, sizeof(z), hipMemcpyHostToDevice));
+    }
     s->initialised = true;
     s->have_fields = false;
     return SS_OK;
@@ -884,10 +988,8 @@ int ss_step(ss_sim* s, uint64_t nsteps, uint64_t* timestep_out) {
     if (s->p.noise_mode == SS_NOISE_EXTERNAL && !s->have_noise)
         return fail(SS_ERR_STATE, "SS_NOISE_EXTERNAL needs ss_set_noise before ss_step");
     SS_HIP_CHECK(hipSetDevice(s->p.device));
-    for (uint64_t k = 0; k < nsteps; ++k) {
-        int rc = s->prec == SS_FP64 ? step_once<double>(s, nullptr) : step_once<float>(s, nullptr);
-        if (rc != SS_OK) return rc;
-    }
+    int rc = s->prec == SS_FP64 ? run_steps<double>(s, nsteps) : run_steps<float>(s, nsteps);
+    if (rc != SS_OK) return rc;
     SS_HIP_CHECK(hipStreamSynchronize(s->stream));
     if (timestep_out) *timestep_out = s->timestep;
     return SS_OK;
@@ -998,6 +1100,7 @@ int ss_resume(ss_sim* s, const double* particles, const double* compensation, ui
     if (rc != SS_OK) return rc;
     SS_HIP_CHECK(hipStreamSynchronize(s->stream));
     s->timestep = timestep;
+    SS_HIP_CHECK(hipMemcpy(s->d_step, &timestep, sizeof(timestep), hipMemcpyHostToDevice));
     s->initialised = true;
     s->have_fields = false;
     return SS_OK;

@@ -64,7 +64,9 @@ struct TileArgs {
     UpdateParams<T> P;
     const T* rec;
     int rs;
-    uint64_t seed, step;
+    uint64_t seed;
+    const uint64_t* step;   // device-resident step counter
+    uint32_t step_off;      // offset added to *step
     const T* noise;
     const long long* kfix;  // [NM][na] fixed-point moment kernels
     long long* main;        // [ns][NM+1] moments of each tile's own cells
@@ -96,7 +98,7 @@ __global__ void __launch_bounds__(256) k_tile_update(TileArgs<T, NM> a) {
         if (MOVE) {
             const uint32_t id = a.s.id[i];
             Noise<T> rv;
-            draw_noise<T>(a.P, a.seed, a.step, id, a.n, a.noise, EXT_NOISE, rv);
+            draw_noise<T>(a.P, a.seed, *a.step + a.step_off, id, a.n, a.noise, EXT_NOISE, rv);
             move_particle<T>(a.s, i, id, g, a.P, a.rec, a.rs, rv, nx, ny, nz, ophi, otheta);
         } else {
             nx = a.s.x[i];

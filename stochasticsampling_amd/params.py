@@ -19,6 +19,7 @@ FFT_DFT = 2
 FLAG_QUASI2D = 1
 FLAG_MAGNETIC_ALWAYS = 2
 FLAG_KEEP_DISTRIBUTION = 4
+FLAG_NO_GRAPH = 8
 
 SS_OK = 0
 ERRORS = {
