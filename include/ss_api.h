@@ -188,6 +188,17 @@ int ss_resume(ss_sim* sim, const double* particles, const double* compensation, 
 int ss_profile_steps(ss_sim* sim, uint64_t nsteps, double* ms_per_step, char* names,
                      int name_stride, int max_stages);
 
+/* Global diagnostics of the current particles, reduced in FP64 (exact
+   64-bit fixed-point sums, independent of the particle order), 15 doubles:
+   out[0..2]  mean orientation vector n over the particles;
+   out[3..5]  mean bin-centre orientation = the sum over cells of the mean
+              polarisation field of tools/DataStreamer.py:177-196;
+   out[6..14] mean stress [3][3] = (1/Ns) sum over cells of average_stress
+              (stress.rs:60-96) for the current binning.
+   No reference counterpart inside the step (the reference computes these in
+   Python post-processing). */
+int ss_get_diagnostics(ss_sim* sim, double* out);
+
 /* Wait for all work of the handle. */
 int ss_synchronize(ss_sim* sim);
 

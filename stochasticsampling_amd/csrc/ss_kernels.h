@@ -374,6 +374,44 @@ __global__ void __launch_bounds__(256)
     move_particle<T>(s, i, (uint32_t)i, g, P, rec, rs, rv, nx, ny, nz, ophi, otheta);
 }
 
+// Global diagnostics in FP64 (mean orientation of the particles, mean
+// bin-centre orientation = spatial sum of the polarisation field of
+// tools/DataStreamer.py:177-196, mean stress = (1/Ns) sum over cells of
+// average_stress).  All are particle averages; each thread converts its
+// values to 64-bit fixed point, so the sums are exact and independent of the
+// particle order.
+constexpr int DIAG_N = 15;
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_diag(State<T> s, uint64_t n, Geo<T> g, const double* __restrict__ table /*[12][na]*/, double scale,
+           unsigned long long* __restrict__ acc) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    long long v[DIAG_N];
+#pragma unroll
+    for (int q = 0; q < DIAG_N; ++q) v[q] = 0;
+    if (i < n) {
+        const double ph = (double)s.phi[i], th = (double)s.theta[i];
+        double sp, cp, st, ct;
+        ::sincos(ph, &sp, &cp);
+        ::sincos(th, &st, &ct);
+        v[0] = llrint(st * cp * scale);
+        v[1] = llrint(st * sp * scale);
+        v[2] = llrint(ct * scale);
+        const int a = angle_of(g, s.phi[i], s.theta[i]);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) v[3 + q] = llrint(table[(size_t)(9 + q) * g.na + a] * scale);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) v[6 + q] = llrint(table[(size_t)q * g.na + a] * scale);
+    }
+#pragma unroll
+    for (int q = 0; q < DIAG_N; ++q) {
+        long long x = v[q];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+        if (__lane_id() == 0) atomicAdd(acc + q, (unsigned long long)x);
+    }
+}
+
 // Fill and counter kernels (graph-capturable replacements of memset).
 __global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)

@@ -186,6 +186,8 @@ struct ss_sim {
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     int graph_steps = 0;
     void* kern = nullptr;
+    double* diag_table = nullptr;       // [12][na] FP64: full stress kernel (9) + bin-centre n (3)
+    unsigned long long* diag_acc = nullptr;
     void* spec_in = nullptr;
     void* spec_out = nullptr;
     void* spec_full = nullptr;
@@ -754,6 +756,13 @@ static int create_impl(ss_sim* s) {
     s->owned.push_back(s->ik2);
     if ((rc = upload<T>(&s->kinv, kinv)) != SS_OK) return rc;
     s->owned.push_back(s->kinv);
+    {
+        std::vector<double> full;
+        moment_kernels(p, 9, true, full);  // 9 stress components + orientation
+        if ((rc = alloc(s, (void**)&s->diag_table, full.size() * 8)) != SS_OK) return rc;
+        SS_HIP_CHECK(hipMemcpy(s->diag_table, full.data(), full.size() * 8, hipMemcpyHostToDevice));
+        if ((rc = alloc(s, (void**)&s->diag_acc, DIAG_N * 8)) != SS_OK) return rc;
+    }
     std::vector<double> kern;
     moment_kernels(p, s->nsig, s->mag, kern);
     if ((rc = upload<T>(&s->kern, kern)) != SS_OK) return rc;
@@ -1104,6 +1113,35 @@ int ss_resume(ss_sim* s, const double* particles, const double* compensation, ui
     s->initialised = true;
     s->have_fields = false;
     return SS_OK;
+}
+
+template <typename T>
+static int diagnostics(ss_sim* s, double* out) {
+    // |value| <= max(1, max|K|) per particle; N of them stay below 2^62
+    double vmax = 1.0;
+    {
+        std::vector<double> full;
+        moment_kernels(s->p, 9, true, full);
+        for (double v : full) vmax = std::max(vmax, std::fabs(v));
+    }
+    const int shift = (int)std::floor(62.0 - std::log2((double)s->n * vmax));
+    const double scale = std::ldexp(1.0, shift);
+    SS_HIP_CHECK(hipMemsetAsync(s->diag_acc, 0, DIAG_N * 8, s->stream));
+    hipLaunchKernelGGL(k_diag<T>, dim3(blocks_for(s->n)), dim3(256), 0, s->stream, state_a<T>(s), s->n, geo<T>(s),
+                       (const double*)s->diag_table, scale, s->diag_acc);
+    SS_HIP_CHECK(hipGetLastError());
+    long long acc[DIAG_N];
+    SS_HIP_CHECK(hipMemcpyAsync(acc, s->diag_acc, DIAG_N * 8, hipMemcpyDeviceToHost, s->stream));
+    SS_HIP_CHECK(hipStreamSynchronize(s->stream));
+    for (int q = 0; q < DIAG_N; ++q) out[q] = std::ldexp((double)acc[q], -shift) / (double)s->n;
+    return SS_OK;
+}
+
+int ss_get_diagnostics(ss_sim* s, double* out) {
+    if (!s || !out) return fail(SS_ERR_INVALID, "NULL argument");
+    if (!s->initialised) return fail(SS_ERR_STATE, "not initialised");
+    SS_HIP_CHECK(hipSetDevice(s->p.device));
+    return s->prec == SS_FP64 ? diagnostics<double>(s, out) : diagnostics<float>(s, out);
 }
 
 int ss_synchronize(ss_sim* s) {

@@ -47,6 +47,7 @@ def _declare(L):
         "ss_resume": (C.c_int, [vp, _d, _d, C.c_uint64, C.c_uint64]),
         "ss_profile_steps": (C.c_int, [vp, C.c_uint64, _d, C.c_char_p, C.c_int, C.c_int]),
         "ss_synchronize": (C.c_int, [vp]),
+        "ss_get_diagnostics": (C.c_int, [vp, _d]),
         "ss_initial_condition": (C.c_int, [pp, C.c_double, C.c_uint64, _d, C.c_uint64]),
         "ss_fft_plan_create": (
             C.c_int,

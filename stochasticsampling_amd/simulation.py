@@ -152,6 +152,17 @@ class Simulation:
         _l.check(self.L.ss_get_magnetic_field(self._h, _l.dptr(b), None if g is None else _l.dptr(g)))
         return (b, g) if with_gradient else b
 
+    def get_diagnostics(self):
+        """FP64 global diagnostics: dict with 'mean_orientation' (3),
+        'mean_polarisation' (3, bin centres) and 'mean_stress' (3x3)."""
+        out = np.zeros(15)
+        _l.check(self.L.ss_get_diagnostics(self._h, _l.dptr(out)))
+        return {
+            "mean_orientation": out[0:3],
+            "mean_polarisation": out[3:6],
+            "mean_stress": out[6:15].reshape(3, 3),
+        }
+
     def profile_steps(self, nsteps, max_stages=16):
         """Runs nsteps steps timing every stage with HIP events on the
         simulation's stream; returns {stage: mean ms per step}."""
