@@ -85,6 +85,49 @@ def cpu_baseline(parts, threads, steps):
     return parts.shape[0] / per_step, per_step
 
 
+def ensemble_configs():
+    """48 independent systems: alpha uniformly spaced in [-2, 2], D_r cycling
+    over {0.1, 0.5, 1.0}, grids cycling over 32^3x8x8, 48^3x12x12, 64^3x16x16
+    with box length 0.78 x grid points, particle counts log-uniform in
+    [2^18, 2^22] (numpy seed 3), seeds 100-147."""
+    import stochasticsampling_amd as ss
+
+    rng = np.random.default_rng(3)
+    grids = [(32, 32, 32, 8, 8), (48, 48, 48, 12, 12), (64, 64, 64, 16, 16)]
+    alphas = np.linspace(-2.0, 2.0, 48)
+    out = []
+    for j in range(48):
+        g = grids[j % 3]
+        L = 0.78 * g[0]
+        n = int(round(2.0 ** rng.uniform(18.0, 22.0)))
+        out.append(ss.make_params(box=(L, L, L), grid=g, n=n, timestep=1e-3, seed=100 + j,
+                                  diffusion_translational=0.1, diffusion_rotational=(0.1, 0.5, 1.0)[j % 3],
+                                  stress_active=float(alphas[j]), precision=ss.FP32, binning=ss.BIN_SORTED))
+    return out
+
+
+def time_ensemble(steps, warmup):
+    import stochasticsampling_amd as ss
+
+    plist = ensemble_configs()
+    ens = ss.Ensemble(plist)
+    total = 0
+    for p, m in zip(plist, ens.members):
+        m.init(ss.initial_condition(p, amplitude=0.05, seed=p.seed))
+        total += p.number_of_particles
+    ens.step(warmup)
+    t0 = time.perf_counter()
+    ens.step(steps)
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / steps
+    ens.close()
+    return {"value": total / (ms * 1e-3), "unit": "particle-steps/s", "ms_per_step": ms, "steps": steps,
+            "systems": len(plist), "particles": total, "dtype": "f32",
+            "workload": "48 systems, alpha in [-2,2], D_r in {0.1,0.5,1}, grids 32^3x8x8/48^3x12x12/"
+                        "64^3x16x16, L = 0.78 x grid points, N log-uniform in [2^18, 2^22], seeds 100-147, "
+                        "one persistent update launch per step, periods replayed from hipGraphs"}
+
+
 def copy_probe(gib=1.0, reps=10):
     """Device-to-device copy of a known size (torch), for the bandwidth
     reference and the calibration of the HBM byte counters."""
@@ -115,6 +158,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="only run a 1 GiB device copy")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=500)
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -143,6 +188,11 @@ def main():
                               min(args.profile_steps, 10))
         fp64 = {"value": n / (ms64 * 1e-3), "ms_per_step": ms64, "steps": max(1, args.steps // 4),
                 "dtype": "f64", "binning": "histogram", "stages_ms": st64}
+
+    ens = None
+    if not args.no_ensemble:
+        print("[bench] batched ensemble of 48 systems", file=sys.stderr, flush=True)
+        ens = time_ensemble(args.ensemble_steps, 16)
 
     # roofline of the dominant kernel (the particle update); traffic comes from
     # the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this kernel
@@ -200,6 +250,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "fp64_plain": fp64,
+        "ensemble": ens,
         "stages_ms": st32,
     }
     if fp64:

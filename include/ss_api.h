@@ -203,6 +203,31 @@ int ss_get_diagnostics(ss_sim* sim, double* out);
 int ss_synchronize(ss_sim* sim);
 
 /* ---------------------------------------------------------------------------
+ * Batched ensemble -- no reference counterpart (the reference runs one system
+ * per process).  For sweeps of many independent systems that differ in grid,
+ * box, particle count and parameters.  Members share precision, noise mode,
+ * moment layout (magnetic coefficients zero or not, magnetic stress zero or
+ * not) and the re-sort interval, and always use the tiled path.  Every step
+ * runs the members' spectral solves as parallel branches and one persistent
+ * update launch over all members' tiles; whole re-sort periods replay from a
+ * captured hipGraph.
+ * ------------------------------------------------------------------------- */
+typedef struct ss_ensemble ss_ensemble;
+
+/* params: nsys entries, one per member. */
+int ss_ensemble_create(const ss_params* params, int nsys, ss_ensemble** out);
+int ss_ensemble_destroy(ss_ensemble* ens);
+
+/* Borrowed handle of member `sys` for ss_init / ss_set_noise / getters (do
+   not ss_step or ss_destroy it).  All members must be initialised together
+   before ss_ensemble_step. */
+int ss_ensemble_member(ss_ensemble* ens, int sys, ss_sim** out);
+
+/* nsteps steps of every member; timestep_out (may be NULL) receives the
+   members' common step counter. */
+int ss_ensemble_step(ss_ensemble* ens, uint64_t nsteps, uint64_t* timestep_out);
+
+/* ---------------------------------------------------------------------------
  * Initial conditions (no reference counterpart for the perturbed state; the
  * uniform part mirrors Particle::create_isotropic, particle.rs:224-256).
  * Positions uniform in the box; cos(theta) drawn by exact inverse CDF from

@@ -19,6 +19,7 @@ from .params import (  # noqa: F401
 )
 from .lib import SSError, device_count, load  # noqa: F401
 from .simulation import (  # noqa: F401
+    Ensemble,
     FFTPlan,
     Simulation,
     Snapshot,

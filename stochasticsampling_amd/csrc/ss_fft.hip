@@ -159,7 +159,7 @@ __device__ inline void stage_generic(const cplx<T>* src, cplx<T>* dst, const cpl
 template <typename T, bool GENERIC>
 __global__ void __launch_bounds__(256) k_fft_axis(cplx<T>* __restrict__ data, AxisDesc d,
                                                   const cplx<T>* __restrict__ twg, int sign) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* buf0 = tw + d.n;
     cplx<T>* buf1 = buf0 + d.tl * d.n;

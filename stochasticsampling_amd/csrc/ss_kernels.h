@@ -108,7 +108,7 @@ template <typename T, int NM>
 __global__ void __launch_bounds__(256)
     k_contract_hist(const uint32_t* __restrict__ hist, Geo<T> g, const T* __restrict__ kern_g, T scale,
                     T rho_scale, cplx<T>* __restrict__ spec, T* __restrict__ rho) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     T* kern = reinterpret_cast<T*>(smem_raw);
     for (int t = threadIdx.x; t < NM * g.na; t += blockDim.x) kern[t] = kern_g[t];
     __syncthreads();

@@ -1369,3 +1369,312 @@ int ss_initial_condition(const ss_params* params, double amplitude, uint64_t see
     }
     return SS_OK;
 }
+
+// ===========================================================================
+// Batched ensemble: independent systems stepped together.  The per-system
+// bin conversion, FFTs, k-space solve and packing run on NSIDE side streams
+// (parallel branches of the captured graph); the particle update of all
+// systems is one persistent launch over the concatenated tile ranges.
+// ===========================================================================
+static const int NSIDE = 4;
+
+struct ss_ensemble {
+    std::vector<ss_sim*> m;
+    hipStream_t stream = nullptr;
+    hipStream_t side[NSIDE] = {nullptr};
+    hipEvent_t fork = nullptr, join[NSIDE] = {nullptr};
+    uint64_t* d_step = nullptr;
+    int prec = SS_FP32;
+    int nm = 6;
+    int resort = 16;
+    bool ext = false;
+    // persistent-update job tables: [slot 0/1][k in period] -> nsys TileArgs
+    void* d_jobs = nullptr;
+    int* d_start = nullptr;
+    size_t job_bytes = 0;
+    std::vector<int> built_for;  // member parities slot 0 was built for
+    int total_blocks = 0;
+    size_t lds = 0;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    uint64_t timestep = 0;
+    uint64_t steps_since_sort = 0;
+};
+
+namespace ss {
+
+static int member_parity(const ss_sim* s) { return s->x == s->x_buf0 ? 0 : 1; }
+
+template <typename T, int NM>
+static int ens_build_jobs(ss_ensemble* e) {
+    const int n = (int)e->m.size();
+    const int K = e->resort;
+    std::vector<TileArgs<T, NM>> h((size_t)2 * K * n);
+    for (int slot = 0; slot < 2; ++slot) {
+        for (ss_sim* s : e->m)
+            if (slot == 1) swap_ab(s);  // the flipped configuration, restored below
+        for (int k = 0; k < K; ++k)
+            for (int j = 0; j < n; ++j) {
+                ss_sim* s = e->m[j];
+                const uint32_t off = s->step_off;
+                s->step_off = (uint32_t)k;
+                h[((size_t)slot * K + k) * n + j] = tile_args<T, NM>(s);
+                s->step_off = off;
+            }
+        for (ss_sim* s : e->m)
+            if (slot == 1) swap_ab(s);
+    }
+    SS_HIP_CHECK(hipMemcpy(e->d_jobs, h.data(), h.size() * sizeof(TileArgs<T, NM>), hipMemcpyHostToDevice));
+    e->built_for.clear();
+    for (ss_sim* s : e->m) e->built_for.push_back(member_parity(s));
+    return SS_OK;
+}
+
+template <typename T, int NM>
+static void ens_launch_update(ss_ensemble* e, int slot, int k) {
+    const int n = (int)e->m.size();
+    const TileArgs<T, NM>* jobs = reinterpret_cast<const TileArgs<T, NM>*>(e->d_jobs) + ((size_t)slot * e->resort + k) * n;
+    const int grid = std::max(1, std::min(e->total_blocks, 256 * 8));
+    const size_t lds = e->lds + sizeof(int) * (n + 1);
+    if (e->ext)
+        hipLaunchKernelGGL((k_tile_update_batched<T, NM, true, true>), dim3(grid), dim3(256), lds, e->stream, jobs,
+                           e->d_start, n, (int)e->lds);
+    else
+        hipLaunchKernelGGL((k_tile_update_batched<T, NM, true, false>), dim3(grid), dim3(256), lds, e->stream,
+                           jobs, e->d_start, n, (int)e->lds);
+}
+
+// One ensemble step; `k` is the step's index within the re-sort period,
+// `slot` the job table matching the members' state buffers.
+template <typename T>
+static int ens_step_once(ss_ensemble* e, int slot, int k, bool capturing) {
+    hipError_t err;
+    SS_HIP_CHECK(hipEventRecord(e->fork, e->stream));
+    for (int q = 0; q < NSIDE; ++q) SS_HIP_CHECK(hipStreamWaitEvent(e->side[q], e->fork, 0));
+    for (ss_sim* s : e->m) {
+        if ((err = launch_bin<T>(s)) != hipSuccess) return fail(SS_ERR_DEVICE, hipGetErrorString(err));
+        int rc = s->fft.run(s->spec_in, s->nm, -1, s->stream);
+        if (rc != SS_OK) return rc;
+        if ((err = launch_solve<T>(s, false, s->spec_out)) != hipSuccess) return fail(SS_ERR_DEVICE, hipGetErrorString(err));
+        rc = s->fft.run(s->spec_out, s->nout, +1, s->stream);
+        if (rc != SS_OK) return rc;
+        if ((err = launch_pack<T>(s)) != hipSuccess) return fail(SS_ERR_DEVICE, hipGetErrorString(err));
+    }
+    for (int q = 0; q < NSIDE; ++q) {
+        SS_HIP_CHECK(hipEventRecord(e->join[q], e->side[q]));
+        SS_HIP_CHECK(hipStreamWaitEvent(e->stream, e->join[q], 0));
+    }
+    if (e->nm == 6)
+        ens_launch_update<T, 6>(e, slot, k);
+    else if (e->nm == 9)
+        ens_launch_update<T, 9>(e, slot, k);
+    else
+        ens_launch_update<T, 12>(e, slot, k);
+    SS_HIP_CHECK(hipGetLastError());
+    e->steps_since_sort += 1;
+    for (ss_sim* s : e->m) {
+        s->steps_since_sort += 1;
+        s->timestep += 1;
+        s->have_fields = true;
+    }
+    if (e->steps_since_sort >= (uint64_t)e->resort) {
+        // the sorts run on the side streams, then join
+        SS_HIP_CHECK(hipEventRecord(e->fork, e->stream));
+        for (int q = 0; q < NSIDE; ++q) SS_HIP_CHECK(hipStreamWaitEvent(e->side[q], e->fork, 0));
+        for (ss_sim* s : e->m)
+            if ((err = launch_sort<T>(s, false)) != hipSuccess) return fail(SS_ERR_DEVICE, hipGetErrorString(err));
+        for (int q = 0; q < NSIDE; ++q) {
+            SS_HIP_CHECK(hipEventRecord(e->join[q], e->side[q]));
+            SS_HIP_CHECK(hipStreamWaitEvent(e->stream, e->join[q], 0));
+        }
+        e->steps_since_sort = 0;
+    }
+    if (!capturing) hipLaunchKernelGGL(k_add_step, dim3(1), dim3(64), 0, e->stream, e->d_step, 1u);
+    e->timestep += 1;
+    return SS_OK;
+}
+
+template <typename T>
+static int ens_jobs_for_current(ss_ensemble* e) {
+    bool same = e->built_for.size() == e->m.size();
+    for (size_t j = 0; same && j < e->m.size(); ++j) same = e->built_for[j] == member_parity(e->m[j]);
+    if (same) return 0;
+    bool flipped = e->built_for.size() == e->m.size();
+    for (size_t j = 0; flipped && j < e->m.size(); ++j) flipped = e->built_for[j] != member_parity(e->m[j]);
+    if (flipped) return 1;
+    for (int k = 0; k < 2; ++k)
+        if (e->graph[k]) {
+            (void)hipGraphExecDestroy(e->graph[k]);
+            e->graph[k] = nullptr;
+        }
+    int rc = e->nm == 6 ? ens_build_jobs<T, 6>(e) : e->nm == 9 ? ens_build_jobs<T, 9>(e) : ens_build_jobs<T, 12>(e);
+    return rc == SS_OK ? 0 : rc;
+}
+
+template <typename T>
+static int ens_run(ss_ensemble* e, uint64_t nsteps) {
+    const int K = e->resort;
+    while (nsteps > 0) {
+        const int slot = ens_jobs_for_current<T>(e);
+        if (slot < 0) return slot;
+        if (e->steps_since_sort == 0 && nsteps >= (uint64_t)K) {
+            if (!e->graph[slot]) {
+                SS_HIP_CHECK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+                int rc = SS_OK;
+                for (int k = 0; k < K && rc == SS_OK; ++k) rc = ens_step_once<T>(e, slot, k, true);
+                if (rc == SS_OK)
+                    hipLaunchKernelGGL(k_add_step, dim3(1), dim3(64), 0, e->stream, e->d_step, (uint32_t)K);
+                hipGraph_t g = nullptr;
+                hipError_t err = hipStreamEndCapture(e->stream, &g);
+                if (rc != SS_OK) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc;
+                }
+                if (err != hipSuccess) return fail(SS_ERR_DEVICE, std::string("ensemble capture: ") + hipGetErrorString(err));
+                err = hipGraphInstantiate(&e->graph[slot], g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (err != hipSuccess) {
+                    e->graph[slot] = nullptr;
+                    return fail(SS_ERR_DEVICE, std::string("ensemble instantiate: ") + hipGetErrorString(err));
+                }
+            } else {
+                for (ss_sim* s : e->m) {
+                    s->timestep += K;
+                    swap_ab(s);
+                    s->steps_since_sort = 0;
+                    s->have_fields = true;
+                }
+                e->timestep += K;
+            }
+            SS_HIP_CHECK(hipGraphLaunch(e->graph[slot], e->stream));
+            nsteps -= (uint64_t)K;
+        } else {
+            // eager step: the job table's offsets are relative to the counter
+            int rc = ens_step_once<T>(e, slot, 0, false);
+            if (rc != SS_OK) return rc;
+            nsteps -= 1;
+        }
+    }
+    return SS_OK;
+}
+
+}  // namespace ss
+
+int ss_ensemble_create(const ss_params* params, int nsys, ss_ensemble** out) {
+    if (!params || !out) return fail(SS_ERR_INVALID, "NULL argument");
+    *out = nullptr;
+    if (nsys < 1 || nsys > MAX_BATCH) return fail(SS_ERR_INVALID, "nsys must be in [1, 256]");
+    ss_ensemble* e = new ss_ensemble();
+    auto bail = [&](int rc) {
+        std::string msg = ss_last_error();
+        ss_ensemble_destroy(e);
+        set_error(msg);
+        return rc;
+    };
+    for (int j = 0; j < nsys; ++j) {
+        ss_params p = params[j];
+        p.binning = SS_BIN_SORTED;
+        p.flags &= ~(SS_FLAG_KEEP_DISTRIBUTION);
+        p.flags |= SS_FLAG_NO_GRAPH;
+        if (p.precision != params[0].precision || p.noise_mode != params[0].noise_mode || p.device != params[0].device)
+            return bail(fail(SS_ERR_INVALID, "ensemble members must share precision, noise mode and device"));
+        if (p.resort_interval != params[0].resort_interval)
+            return bail(fail(SS_ERR_INVALID, "ensemble members must share the re-sort interval"));
+        ss_sim* s = nullptr;
+        int rc = ss_create(&p, &s);
+        if (rc != SS_OK) return bail(rc);
+        e->m.push_back(s);
+        if (!s->sorted) return bail(fail(SS_ERR_UNSUPPORTED, "member " + std::to_string(j) + " cannot use the tiled path"));
+        if (s->nm != e->m[0]->nm) return bail(fail(SS_ERR_INVALID, "ensemble members must share the moment layout"));
+    }
+    ss_sim* s0 = e->m[0];
+    e->prec = s0->prec;
+    e->nm = s0->nm;
+    e->resort = s0->resort;
+    e->ext = s0->p.noise_mode == SS_NOISE_EXTERNAL;
+    if (hipSetDevice(s0->p.device) != hipSuccess) return bail(fail(SS_ERR_DEVICE, "hipSetDevice"));
+    if (hipStreamCreate(&e->stream) != hipSuccess || hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(SS_ERR_DEVICE, "stream/event creation failed"));
+    for (int q = 0; q < NSIDE; ++q)
+        if (hipStreamCreate(&e->side[q]) != hipSuccess ||
+            hipEventCreateWithFlags(&e->join[q], hipEventDisableTiming) != hipSuccess)
+            return bail(fail(SS_ERR_DEVICE, "stream/event creation failed"));
+    if (hipMalloc(&e->d_step, sizeof(uint64_t)) != hipSuccess || hipMemset(e->d_step, 0, sizeof(uint64_t)) != hipSuccess)
+        return bail(fail(SS_ERR_ALLOC, "hipMalloc"));
+    // members run on the side streams and count steps on the shared counter
+    std::vector<int> start(1, 0);
+    size_t lds = 0;
+    for (int j = 0; j < nsys; ++j) {
+        ss_sim* s = e->m[j];
+        (void)hipStreamDestroy(s->stream);
+        s->stream = e->side[j % NSIDE];
+        s->d_step = e->d_step;
+        const int per = (s->tg.ntiles + s->tg.nxcd - 1) / s->tg.nxcd;
+        start.push_back(start.back() + per * s->tg.nxcd);
+        lds = std::max(lds, (size_t)s->tg.ws * (s->nm + 1) * 8 + (size_t)s->nm * s->na * 8);
+    }
+    e->total_blocks = start.back();
+    e->lds = (lds + 15) / 16 * 16;
+    e->job_bytes = e->prec == SS_FP32 ? (e->nm == 6 ? sizeof(TileArgs<float, 6>) : e->nm == 9 ? sizeof(TileArgs<float, 9>) : sizeof(TileArgs<float, 12>))
+                                      : (e->nm == 6 ? sizeof(TileArgs<double, 6>) : e->nm == 9 ? sizeof(TileArgs<double, 9>) : sizeof(TileArgs<double, 12>));
+    if (hipMalloc(&e->d_jobs, e->job_bytes * 2 * e->resort * nsys) != hipSuccess ||
+        hipMalloc(&e->d_start, sizeof(int) * (nsys + 1)) != hipSuccess)
+        return bail(fail(SS_ERR_ALLOC, "hipMalloc"));
+    if (hipMemcpy(e->d_start, start.data(), sizeof(int) * (nsys + 1), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(SS_ERR_DEVICE, "copy"));
+    *out = e;
+    return SS_OK;
+}
+
+int ss_ensemble_destroy(ss_ensemble* e) {
+    if (!e) return SS_OK;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (int k = 0; k < 2; ++k)
+        if (e->graph[k]) (void)hipGraphExecDestroy(e->graph[k]);
+    for (ss_sim* s : e->m) {
+        if (s->d_step == e->d_step) s->d_step = nullptr;
+        bool shared = false;
+        for (int q = 0; q < NSIDE; ++q) shared = shared || s->stream == e->side[q];
+        if (shared) s->stream = nullptr;
+        ss_destroy(s);
+    }
+    for (int q = 0; q < NSIDE; ++q) {
+        if (e->side[q]) (void)hipStreamDestroy(e->side[q]);
+        if (e->join[q]) (void)hipEventDestroy(e->join[q]);
+    }
+    if (e->fork) (void)hipEventDestroy(e->fork);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    if (e->d_step) (void)hipFree(e->d_step);
+    if (e->d_jobs) (void)hipFree(e->d_jobs);
+    if (e->d_start) (void)hipFree(e->d_start);
+    delete e;
+    return SS_OK;
+}
+
+int ss_ensemble_member(ss_ensemble* e, int sys, ss_sim** out) {
+    if (!e || !out) return fail(SS_ERR_INVALID, "NULL argument");
+    if (sys < 0 || sys >= (int)e->m.size()) return fail(SS_ERR_INVALID, "member index out of range");
+    *out = e->m[sys];
+    return SS_OK;
+}
+
+int ss_ensemble_step(ss_ensemble* e, uint64_t nsteps, uint64_t* timestep_out) {
+    if (!e) return fail(SS_ERR_INVALID, "NULL handle");
+    for (ss_sim* s : e->m) {
+        if (!s->initialised) return fail(SS_ERR_STATE, "every member needs ss_init before ss_ensemble_step");
+        if (s->p.noise_mode == SS_NOISE_EXTERNAL && !s->have_noise)
+            return fail(SS_ERR_STATE, "SS_NOISE_EXTERNAL needs ss_set_noise for every member");
+        if (s->timestep != e->m[0]->timestep || s->steps_since_sort != e->m[0]->steps_since_sort)
+            return fail(SS_ERR_STATE, "ensemble members are out of step: initialise them together");
+    }
+    SS_HIP_CHECK(hipSetDevice(e->m[0]->p.device));
+    e->timestep = e->m[0]->timestep;
+    e->steps_since_sort = e->m[0]->steps_since_sort;
+    // members' own eager work (init, getters) ran on the side streams
+    for (int q = 0; q < NSIDE; ++q) SS_HIP_CHECK(hipStreamSynchronize(e->side[q]));
+    SS_HIP_CHECK(hipMemcpy(e->d_step, &e->timestep, sizeof(uint64_t), hipMemcpyHostToDevice));
+    int rc = e->prec == SS_FP64 ? ens_run<double>(e, nsteps) : ens_run<float>(e, nsteps);
+    if (rc != SS_OK) return rc;
+    SS_HIP_CHECK(hipStreamSynchronize(e->stream));
+    if (timestep_out) *timestep_out = e->timestep;
+    return SS_OK;
+}

@@ -74,16 +74,17 @@ struct TileArgs {
     uint32_t* hist;         // optional [ns][na] counts (kept distribution)
 };
 
-// MOVE = false bins the current state without moving (initial sampling).
+// One tile of one system: move its particles (MOVE) and bin them into the
+// next step's moments.  MOVE = false bins the current state (initial
+// sampling).  `block` is the workgroup's index within the system's launch.
 template <typename T, int NM, bool MOVE, bool EXT_NOISE, bool KEEP>
-__global__ void __launch_bounds__(256) k_tile_update(TileArgs<T, NM> a) {
-    extern __shared__ unsigned char smem_raw[];
+__device__ inline void tile_update_body(const TileArgs<T, NM>& a, int block, unsigned char* smem_raw) {
     constexpr int NC = NM + 1;
     long long* acc = reinterpret_cast<long long*>(smem_raw);  // [ws][NC]
     long long* kf = acc + (size_t)a.tg.ws * NC;               // [NM][na]
     const Geo<T>& g = a.g;
     const TileGeo& tg = a.tg;
-    const int tile = tile_for_block(tg, blockIdx.x);
+    const int tile = tile_for_block(tg, block);
     if (tile >= tg.ntiles) return;
     const int tz = tile % tg.nt[2], ty = (tile / tg.nt[2]) % tg.nt[1], tx = tile / (tg.nt[2] * tg.nt[1]);
     const int ox = tx * tg.t[0], oy = ty * tg.t[1], oz = tz * tg.t[2];
@@ -155,6 +156,45 @@ __global__ void __launch_bounds__(256) k_tile_update(TileArgs<T, NM> a) {
             for (int q = 0; q < NC; ++q)
                 atomicAdd(reinterpret_cast<unsigned long long*>(a.inflow + cell * NC + q), (unsigned long long)src[q]);
         }
+    }
+}
+
+template <typename T, int NM, bool MOVE, bool EXT_NOISE, bool KEEP>
+__global__ void __launch_bounds__(256) k_tile_update(TileArgs<T, NM> a) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    tile_update_body<T, NM, MOVE, EXT_NOISE, KEEP>(a, blockIdx.x, smem_raw);
+}
+
+// Many systems in one persistent launch: work item w of the concatenated
+// per-system workgroup ranges [start[j], start[j+1]) is workgroup
+// w - start[j] of system j.  The grid strides over all work items.
+constexpr int MAX_BATCH = 256;
+
+// Dynamic LDS: [lds_tile bytes for the tile body][njobs + 1 ints of start]; no
+// static LDS, so the 64-bit accumulators at the region's start stay 8-byte
+// aligned.
+template <typename T, int NM, bool MOVE, bool EXT_NOISE>
+__global__ void __launch_bounds__(256)
+    k_tile_update_batched(const TileArgs<T, NM>* __restrict__ jobs, const int* __restrict__ start, int njobs,
+                          int lds_tile) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    int* sstart = reinterpret_cast<int*>(smem_raw + lds_tile);
+    for (int k = threadIdx.x; k <= njobs; k += blockDim.x) sstart[k] = start[k];
+    __syncthreads();
+    const int total = sstart[njobs];
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        int lo = 0, hi = njobs;  // sstart[lo] <= w < sstart[lo + 1]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (sstart[mid] <= w)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        lo = __builtin_amdgcn_readfirstlane(lo);  // uniform: keeps the job in scalar registers
+        const TileArgs<T, NM> a = jobs[lo];
+        tile_update_body<T, NM, MOVE, EXT_NOISE, false>(a, w - sstart[lo], smem_raw);
+        __syncthreads();
     }
 }
 

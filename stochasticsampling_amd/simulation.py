@@ -178,6 +178,59 @@ class Simulation:
         return out
 
 
+class _Member(Simulation):
+    """Borrowed member handle of an Ensemble (not destroyed by Python)."""
+
+    def __init__(self, ensemble, params, handle):
+        self.L = _l.load()
+        self.params = params
+        self._h = handle
+        self.n = params.number_of_particles
+        self._ens = ensemble
+
+    def close(self):
+        self._h = None
+
+    def do_timestep(self):
+        raise RuntimeError("step an ensemble member through Ensemble.step")
+
+    step = do_timestep
+
+
+class Ensemble:
+    """Independent systems stepped together (ss_ensemble_*): one persistent
+    update launch over all members per step, the members' spectral solves as
+    parallel graph branches.  ``members[j]`` has the Simulation getters."""
+
+    def __init__(self, params_list):
+        self.L = _l.load()
+        self.params = list(params_list)
+        arr = (Params * len(self.params))(*self.params)
+        h = C.c_void_p()
+        _l.check(self.L.ss_ensemble_create(arr, len(self.params), C.byref(h)))
+        self._h = h
+        self.members = []
+        for j, p in enumerate(self.params):
+            mh = C.c_void_p()
+            _l.check(self.L.ss_ensemble_member(self._h, j, C.byref(mh)))
+            self.members.append(_Member(self, p, mh))
+
+    def step(self, nsteps=1):
+        t = C.c_uint64()
+        _l.check(self.L.ss_ensemble_step(self._h, nsteps, C.byref(t)))
+        return t.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for m in self.members:
+                m._h = None
+            self.L.ss_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
 def initial_condition(params, amplitude=0.05, seed=None, n=None):
     """Uniform positions, isotropic orientations with a polar perturbation
     amplitude*sin(2 pi x / Lx) along z (N x 5, serde order)."""

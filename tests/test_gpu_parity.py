@@ -319,6 +319,39 @@ def test_graph_replays_match_eager_steps(precision, binning):
     np.testing.assert_array_equal(g.get_flow_field(), e.get_flow_field())
 
 
+@pytest.mark.parametrize("precision", [ss.FP32, ss.FP64])
+def test_ensemble_members_equal_single_systems(precision):
+    """Members of a batched ensemble (different grids, boxes, particle counts,
+    alpha and D_r) evolve bit for bit like the same systems stepped alone."""
+    specs = [((16, 16, 16, 8, 8), 12.5, 5000, -2.0, 0.1), ((12, 12, 12, 6, 6), 9.4, 3000, 0.5, 1.0),
+             ((8, 8, 8, 4, 4), 6.2, 700, 1.7, 0.5), ((20, 16, 12, 8, 8), 15.6, 9000, -0.3, 0.5),
+             ((16, 16, 16, 8, 8), 12.5, 1, 1.0, 0.1)]
+    plist, parts = [], []
+    for k, (grid, L, n, alpha, dr) in enumerate(specs):
+        p = ss.make_params(box=(L, L * grid[1] / grid[0], L * grid[2] / grid[0]), grid=grid, n=n, seed=100 + k,
+                           stress_active=alpha, diffusion_rotational=dr, precision=precision,
+                           binning=ss.BIN_SORTED, resort_interval=4)
+        plist.append(p)
+        parts.append(ss.initial_condition(p, amplitude=0.05, seed=100 + k))
+    ens = ss.Ensemble(plist)
+    for m, x in zip(ens.members, parts):
+        m.init(x)
+    singles = []
+    for p, x in zip(plist, parts):
+        sim = ss.Simulation(p)
+        sim.init(x)
+        singles.append(sim)
+    for chunk in [3, 4, 8, 1, 6]:
+        t = ens.step(chunk)
+        for sim in singles:
+            assert sim.step(chunk) == t
+        for m, sim in zip(ens.members, singles):
+            np.testing.assert_array_equal(m.get_particles(), sim.get_particles())
+    for m, sim in zip(ens.members, singles):
+        np.testing.assert_array_equal(m.get_flow_field(), sim.get_flow_field())
+        assert m.get_timestep() == sim.get_timestep()
+
+
 def test_external_noise_equals_philox_fp64(oracle):
     p = small_params(n=1024)
     parts = ss.initial_condition(p, amplitude=0.05, seed=3)
