@@ -312,22 +312,42 @@ __device__ inline float add_wrap<float>(float x, float d, float L, float* lo) {
     return t;
 }
 
-// Update of every particle, plain path (particles in their original order):
-// gather the cell record, draw the noise, run the LangevinBuilder chain,
-// move, wrap.
+// One particle's state as loaded from the arrays (the low-order position
+// parts only on the FP32 path).
 template <typename T>
-__device__ inline void move_particle(State<T> s, uint64_t i, uint32_t id, const Geo<T>& g,
-                                     const UpdateParams<T>& P, const T* __restrict__ rec, int rs,
-                                     const Noise<T>& rv, T& nx, T& ny, T& nz, T& ophi, T& otheta) {
-    const T x = s.x[i], y = s.y[i], z = s.z[i];
-    const int c = cell_of(g, x, y, z);
-    T dp[3];
-    langevin<T>(P, s.phi[i], s.theta[i], rec + (size_t)c * rs, rv, dp, ophi, otheta);
+struct Loaded {
+    T x, y, z, phi, theta;
+    float lx, ly, lz;
+};
+
+template <typename T>
+__device__ inline void load_particle(const State<T>& s, uint64_t i, Loaded<T>& q) {
+    q.x = s.x[i];
+    q.y = s.y[i];
+    q.z = s.z[i];
+    q.phi = s.phi[i];
+    q.theta = s.theta[i];
     if constexpr (sizeof(T) == 4) {
-        float lx = s.lx[i], ly = s.ly[i], lz = s.lz[i];
-        nx = add_wrap<T>(x, dp[0], P.box[0], &lx);
-        ny = add_wrap<T>(y, dp[1], P.box[1], &ly);
-        nz = add_wrap<T>(z, dp[2], P.box[2], &lz);
+        q.lx = s.lx[i];
+        q.ly = s.ly[i];
+        q.lz = s.lz[i];
+    } else {
+        q.lx = q.ly = q.lz = 0.0f;
+    }
+}
+
+// Update of one loaded particle with its cell record (global memory or
+// registers): run the LangevinBuilder chain, move, wrap, store at slot i.
+template <typename T>
+__device__ inline void move_loaded(const State<T>& s, uint64_t i, const Loaded<T>& q, const UpdateParams<T>& P,
+                                   const T* rec, const Noise<T>& rv, T& nx, T& ny, T& nz, T& ophi, T& otheta) {
+    T dp[3];
+    langevin<T>(P, q.phi, q.theta, rec, rv, dp, ophi, otheta);
+    if constexpr (sizeof(T) == 4) {
+        float lx = q.lx, ly = q.ly, lz = q.lz;
+        nx = add_wrap<T>(q.x, dp[0], P.box[0], &lx);
+        ny = add_wrap<T>(q.y, dp[1], P.box[1], &ly);
+        nz = add_wrap<T>(q.z, dp[2], P.box[2], &lz);
         if (P.quasi2d) {
             nz = T(0);
             lz = 0.0f;
@@ -336,9 +356,9 @@ __device__ inline void move_particle(State<T> s, uint64_t i, uint32_t id, const 
         s.ly[i] = ly;
         s.lz[i] = lz;
     } else {
-        nx = add_wrap<T>(x, dp[0], P.box[0], nullptr);
-        ny = add_wrap<T>(y, dp[1], P.box[1], nullptr);
-        nz = add_wrap<T>(z, dp[2], P.box[2], nullptr);
+        nx = add_wrap<T>(q.x, dp[0], P.box[0], nullptr);
+        ny = add_wrap<T>(q.y, dp[1], P.box[1], nullptr);
+        nz = add_wrap<T>(q.z, dp[2], P.box[2], nullptr);
         if (P.quasi2d) nz = T(0);
     }
     s.x[i] = nx;
@@ -346,6 +366,33 @@ __device__ inline void move_particle(State<T> s, uint64_t i, uint32_t id, const 
     s.z[i] = nz;
     s.phi[i] = ophi;
     s.theta[i] = otheta;
+}
+
+// FP32 cell record into registers with 16-byte loads: the first 8 values
+// (u, W) always, the rest when the record is the full one.
+__device__ inline void gather_record(const float* __restrict__ src, int rs, float r[REC_FULL]) {
+    const float4* v = reinterpret_cast<const float4*>(src);
+    const int nv = rs == REC_FULL ? REC_FULL / 4 : REC_SMALL / 4;
+#pragma unroll
+    for (int k = 0; k < REC_FULL / 4; ++k) {
+        float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (k < nv) t = v[k];
+        r[4 * k + 0] = t.x;
+        r[4 * k + 1] = t.y;
+        r[4 * k + 2] = t.z;
+        r[4 * k + 3] = t.w;
+    }
+}
+
+// Update of every particle, plain path (particles in their original order).
+template <typename T>
+__device__ inline void move_particle(const State<T>& s, uint64_t i, const Geo<T>& g, const UpdateParams<T>& P,
+                                     const T* __restrict__ rec, int rs, const Noise<T>& rv, T& nx, T& ny, T& nz,
+                                     T& ophi, T& otheta) {
+    Loaded<T> q;
+    load_particle(s, i, q);
+    const int c = cell_of(g, q.x, q.y, q.z);
+    move_loaded(s, i, q, P, rec + (size_t)c * rs, rv, nx, ny, nz, ophi, otheta);
 }
 
 template <typename T>
@@ -371,7 +418,7 @@ __global__ void __launch_bounds__(256)
     Noise<T> rv;
     draw_noise<T>(P, seed, *step + step_off, (uint32_t)i, n, noise, EXT_NOISE, rv);
     T nx, ny, nz, ophi, otheta;
-    move_particle<T>(s, i, (uint32_t)i, g, P, rec, rs, rv, nx, ny, nz, ophi, otheta);
+    move_particle<T>(s, i, g, P, rec, rs, rv, nx, ny, nz, ophi, otheta);
 }
 
 // Global diagnostics in FP64 (mean orientation of the particles, mean

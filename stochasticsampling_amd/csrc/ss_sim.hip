@@ -170,7 +170,10 @@ struct ss_sim {
     TileGeo tg;
     int resort = 16;             // steps between tile sorts
     uint64_t steps_since_sort = 0;
-    uint32_t *tcnt = nullptr, *toff = nullptr, *cursor = nullptr, *part = nullptr;
+    // a re-sort is due: the next update writes the particles to state B in
+    // the order of offsets toff2 (computed from the last binning's counts)
+    bool pending = false;
+    uint32_t *tcnt = nullptr, *toff = nullptr, *toff2 = nullptr, *cursor = nullptr, *part = nullptr;
     int nparts = 0;
     long long *mom_main = nullptr, *mom_inflow = nullptr, *kfix = nullptr;
     double fixed_scale = 1.0;    // value of one fixed-point unit
@@ -183,7 +186,8 @@ struct ss_sim {
     bool capturing = false;
     bool use_graph = true;
     void* x_buf0 = nullptr;       // the first x buffer: tells which state buffer is current
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    uint32_t* toff_buf0 = nullptr;  // the first offsets buffer (current with x_buf0)
+    hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};  // [pending][parity]
     int graph_steps = 0;
     void* kern = nullptr;
     double* diag_table = nullptr;       // [12][na] FP64: full stress kernel (9) + bin-centre n (3)
@@ -360,8 +364,11 @@ template <typename T, int NM, bool MOVE, bool EXT, bool KEEP>
 static void launch_tile_kernel(ss_sim* s, const TileArgs<T, NM>& a) {
     const int per = (s->tg.ntiles + s->tg.nxcd - 1) / s->tg.nxcd;
     const dim3 grid((unsigned)(per * s->tg.nxcd));
-    const size_t lds = (size_t)s->tg.ws * (NM + 1) * 8 + (size_t)NM * s->na * 8;
-    hipLaunchKernelGGL((k_tile_update<T, NM, MOVE, EXT, KEEP>), grid, dim3(256), lds, s->stream, a);
+    const size_t lds = tile_lds_bytes(s->tg.ws, NM, s->na);
+    if (MOVE && a.dtoff)
+        hipLaunchKernelGGL((k_tile_update<T, NM, MOVE, EXT, KEEP, MOVE>), grid, dim3(256), lds, s->stream, a);
+    else
+        hipLaunchKernelGGL((k_tile_update<T, NM, MOVE, EXT, KEEP, false>), grid, dim3(256), lds, s->stream, a);
 }
 
 template <typename T, int NM, bool MOVE>
@@ -396,20 +403,32 @@ static TileArgs<T, NM> tile_args(ss_sim* s) {
     a.main = s->mom_main;
     a.inflow = s->mom_inflow;
     a.hist = (s->p.flags & SS_FLAG_KEEP_DISTRIBUTION) ? s->hist_next : nullptr;
+    a.d = state_b<T>(s);
+    a.dtoff = s->pending ? s->toff2 : nullptr;
+    a.cross = s->cursor;
     return a;
 }
 
+static void swap_sorted(ss_sim* s) {
+    swap_ab(s);
+    std::swap(s->toff, s->toff2);
+    s->pending = false;
+}
+
 // Tiled path: move every particle and bin the new state (MOVE), or only bin
-// the current state (initial sampling).
+// the current state (initial sampling).  A pending re-sort makes the move
+// write state B in the new order, which then becomes current.
 template <typename T, bool MOVE>
 static hipError_t launch_tile_update(ss_sim* s) {
     if (s->hist_next) fill_u32(s, s->hist_next, (size_t)s->ns * s->na, 0u);
+    if (!MOVE && s->pending) return hipErrorInvalidValue;
     if (s->nm == 6)
         launch_tile_nm<T, 6, MOVE>(s, tile_args<T, 6>(s));
     else if (s->nm == 9)
         launch_tile_nm<T, 9, MOVE>(s, tile_args<T, 9>(s));
     else
         launch_tile_nm<T, 12, MOVE>(s, tile_args<T, 12>(s));
+    if (MOVE && s->pending) swap_sorted(s);
     return hipGetLastError();
 }
 
@@ -429,39 +448,49 @@ static hipError_t launch_update(ss_sim* s) {
     return hipGetLastError();
 }
 
-// Counting sort of the particles by tile.  from_state: count from the
-// particle state (initial sort) instead of the moment arrays.
-template <typename T>
-static hipError_t launch_sort(ss_sim* s, bool from_state) {
+// Offsets of the next re-sort: particles per tile from the count
+// component of the moments just binned, exclusive scan into toff2, crossing
+// counters cleared.  The next update places the particles (pending).
+static hipError_t launch_count_scan(ss_sim* s) {
     const int nt = s->tg.ntiles;
-    if (from_state) {
-        fill_u32(s, s->tcnt, (size_t)nt, 0u);
-        hipLaunchKernelGGL(k_tile_count_state<T>, dim3(blocks_for(s->n)), dim3(256), 0, s->stream,
-                           state_a<T>(s), s->n, geo<T>(s), s->tg, s->tcnt);
-    } else {
-        const dim3 grid(blocks_for(nt));
-        if (s->nm == 6)
-            hipLaunchKernelGGL(k_tile_counts<6>, grid, dim3(256), 0, s->stream, s->mom_main, s->mom_inflow,
-                               s->nx, s->ny, s->nz, s->tg, s->tcnt);
-        else if (s->nm == 9)
-            hipLaunchKernelGGL(k_tile_counts<9>, grid, dim3(256), 0, s->stream, s->mom_main, s->mom_inflow,
-                               s->nx, s->ny, s->nz, s->tg, s->tcnt);
-        else
-            hipLaunchKernelGGL(k_tile_counts<12>, grid, dim3(256), 0, s->stream, s->mom_main, s->mom_inflow,
-                               s->nx, s->ny, s->nz, s->tg, s->tcnt);
-    }
+    const dim3 grid((unsigned)((nt + 3) / 4));
+    if (s->nm == 6)
+        hipLaunchKernelGGL(k_tile_counts<6>, grid, dim3(256), 0, s->stream, s->mom_main, s->mom_inflow, s->nx,
+                           s->ny, s->nz, s->tg, s->tcnt);
+    else if (s->nm == 9)
+        hipLaunchKernelGGL(k_tile_counts<9>, grid, dim3(256), 0, s->stream, s->mom_main, s->mom_inflow, s->nx,
+                           s->ny, s->nz, s->tg, s->tcnt);
+    else
+        hipLaunchKernelGGL(k_tile_counts<12>, grid, dim3(256), 0, s->stream, s->mom_main, s->mom_inflow, s->nx,
+                           s->ny, s->nz, s->tg, s->tcnt);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(s->nparts), dim3(256), 0, s->stream, s->tcnt, nt, s->part);
+    hipLaunchKernelGGL(k_scan_apply, dim3(s->nparts), dim3(256), 0, s->stream, s->tcnt, nt, s->part, s->nparts,
+                       s->toff2, s->cursor);
+    s->pending = true;
+    s->steps_since_sort = 0;
+    return hipGetLastError();
+}
+
+// Counting sort of particles in arbitrary order by tile (ss_init, resume).
+template <typename T>
+static hipError_t launch_initial_sort(ss_sim* s) {
+    const int nt = s->tg.ntiles;
+    fill_u32(s, s->tcnt, (size_t)nt, 0u);
+    hipLaunchKernelGGL(k_tile_count_state<T>, dim3(blocks_for(s->n)), dim3(256), 0, s->stream, state_a<T>(s),
+                       s->n, geo<T>(s), s->tg, s->tcnt);
+    // the offsets buffer that becomes current pairs with the state buffer that
+    // becomes current, as every later re-sort swaps both
+    const bool to_b0 = s->x != s->x_buf0;
+    if ((s->toff == s->toff_buf0) != to_b0) std::swap(s->toff, s->toff2);
     hipLaunchKernelGGL(k_scan_reduce, dim3(s->nparts), dim3(256), 0, s->stream, s->tcnt, nt, s->part);
     hipLaunchKernelGGL(k_scan_apply, dim3(s->nparts), dim3(256), 0, s->stream, s->tcnt, nt, s->part, s->nparts,
                        s->toff, s->cursor);
-    if (from_state)
-        hipLaunchKernelGGL(k_tile_permute<T>, dim3(blocks_for(s->n)), dim3(256), 0, s->stream, state_a<T>(s),
-                           state_b<T>(s), s->n, geo<T>(s), s->tg, s->toff, s->cursor);
-    else
-        hipLaunchKernelGGL(k_tile_permute_agg<T>, dim3(blocks_for(s->n, 4 * 64 * PW_ITER)), dim3(256), 0,
-                           s->stream, state_a<T>(s), state_b<T>(s), s->n, geo<T>(s), s->tg, s->toff, s->cursor);
+    hipLaunchKernelGGL(k_tile_permute<T>, dim3(blocks_for(s->n)), dim3(256), 0, s->stream, state_a<T>(s),
+                       state_b<T>(s), s->n, geo<T>(s), s->tg, s->toff, s->cursor);
     hipError_t e = hipGetLastError();
     swap_ab(s);
     s->steps_since_sort = 0;
+    s->pending = false;
     return e;
 }
 
@@ -497,7 +526,7 @@ static int step_once(ss_sim* s, hipEvent_t* ev) {
     if (s->sorted) {
         s->steps_since_sort += 1;
         if (s->steps_since_sort >= (uint64_t)s->resort) {
-            if ((e = launch_sort<T>(s, false)) != hipSuccess)
+            if ((e = launch_count_scan(s)) != hipSuccess)
                 return fail(SS_ERR_DEVICE, std::string("sort: ") + hipGetErrorString(e));
         }
         s->swap_hist = s->hist_next != nullptr;
@@ -521,7 +550,7 @@ template <typename T>
 static int initial_binning(ss_sim* s) {
     hipError_t e;
     if (s->sorted) {
-        if ((e = launch_sort<T>(s, true)) != hipSuccess) return fail(SS_ERR_DEVICE, hipGetErrorString(e));
+        if ((e = launch_initial_sort<T>(s)) != hipSuccess) return fail(SS_ERR_DEVICE, hipGetErrorString(e));
         SS_HIP_CHECK(hipMemsetAsync(s->mom_inflow, 0, (size_t)s->ns * (s->nm + 1) * 8, s->stream));
         if ((e = launch_tile_update<T, false>(s)) != hipSuccess) return fail(SS_ERR_DEVICE, hipGetErrorString(e));
         if (s->hist_next) std::swap(s->hist, s->hist_next);
@@ -579,16 +608,16 @@ static int copy_real(ss_sim* s, const void* spec, int comp0, int ncomp, double* 
     return SS_OK;
 }
 
-// Steps per captured graph: one re-sort period on the tiled path (the sort,
-// and with it the swap of the state buffers, is the period's last node), 8
-// steps on the plain path.
+// Steps per captured graph: one re-sort period on the tiled path (its first
+// update carries out the re-sort prepared at the end of the previous period,
+// swapping the state buffers), 8 steps on the plain path.
 static int graph_period(const ss_sim* s) { return s->sorted ? s->resort : 8; }
 
-// Capture one period of steps into graph[parity].  Capturing enqueues nothing;
+// Capture one period of steps into graph[key].  Capturing enqueues nothing;
 // it advances the host-side bookkeeping exactly as running the period would,
 // so the caller must launch the graph once afterwards.
 template <typename T>
-static int capture_period(ss_sim* s, int parity) {
+static int capture_period(ss_sim* s, int key) {
     const int K = graph_period(s);
     SS_HIP_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     s->capturing = true;
@@ -605,10 +634,10 @@ static int capture_period(ss_sim* s, int parity) {
         return rc;
     }
     if (e != hipSuccess) return fail(SS_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
-    e = hipGraphInstantiate(&s->graph[parity], g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&s->graph[key], g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) {
-        s->graph[parity] = nullptr;
+        s->graph[key] = nullptr;
         return fail(SS_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
     }
     s->graph_steps = K;
@@ -623,20 +652,21 @@ static int run_steps(ss_sim* s, uint64_t nsteps) {
     while (nsteps > 0) {
         const bool aligned = !s->sorted || s->steps_since_sort == 0;
         if (s->use_graph && aligned && nsteps >= (uint64_t)K) {
-            const int parity = s->x == s->x_buf0 ? 0 : 1;
-            if (!s->graph[parity]) {
-                int rc = capture_period<T>(s, parity);
+            const int key = (s->pending ? 2 : 0) + (s->x == s->x_buf0 ? 0 : 1);
+            if (!s->graph[key]) {
+                int rc = capture_period<T>(s, key);
                 if (rc != SS_OK) return rc;
             } else {
                 // advance the bookkeeping the captured period performs
                 for (int k = 0; k < K; ++k) s->timestep += 1;
                 if (s->sorted) {
-                    swap_ab(s);
+                    if (s->pending) swap_sorted(s);
+                    s->pending = true;
                     s->steps_since_sort = 0;
                 }
                 s->have_fields = true;
             }
-            SS_HIP_CHECK(hipGraphLaunch(s->graph[parity], s->stream));
+            SS_HIP_CHECK(hipGraphLaunch(s->graph[key], s->stream));
             nsteps -= (uint64_t)K;
         } else {
             int rc = step_once<T>(s, nullptr);
@@ -655,7 +685,7 @@ static int alloc(ss_sim* s, void** p, size_t bytes) {
 }
 
 static void drop_graphs(ss_sim* s) {
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < 4; ++k)
         if (s->graph[k]) {
             (void)hipGraphExecDestroy(s->graph[k]);
             s->graph[k] = nullptr;
@@ -704,6 +734,7 @@ static int create_impl(ss_sim* s) {
         const int nt = s->tg.ntiles;
         A(s->tcnt, (size_t)nt * 4);
         A(s->toff, ((size_t)nt + 1) * 4);
+        A(s->toff2, ((size_t)nt + 1) * 4);
         A(s->cursor, (size_t)nt * 4);
         s->nparts = (nt + SCAN_CHUNK - 1) / SCAN_CHUNK;
         A(s->part, (size_t)s->nparts * 4);
@@ -719,6 +750,7 @@ static int create_impl(ss_sim* s) {
     A(s->spec_full, cs * (12 + (s->mag ? 12 : 0)));
     if (s->use_rho) A(s->rho, (size_t)s->ns * sizeof(T));
     s->x_buf0 = s->x;
+    s->toff_buf0 = s->toff;
     A(s->d_step, sizeof(uint64_t));
     SS_HIP_CHECK(hipMemset(s->d_step, 0, sizeof(uint64_t)));
     A(s->rec, (size_t)s->ns * s->rs * sizeof(T));
@@ -1388,14 +1420,14 @@ struct ss_ensemble {
     int nm = 6;
     int resort = 16;
     bool ext = false;
-    // persistent-update job tables: [slot 0/1][k in period] -> nsys TileArgs
+    // persistent-update job tables: [slot 0/1][in place / re-sort] -> nsys TileArgs
     void* d_jobs = nullptr;
     int* d_start = nullptr;
     size_t job_bytes = 0;
     std::vector<int> built_for;  // member parities slot 0 was built for
     int total_blocks = 0;
     size_t lds = 0;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};  // [pending][slot]
     uint64_t timestep = 0;
     uint64_t steps_since_sort = 0;
 };
@@ -1404,24 +1436,32 @@ namespace ss {
 
 static int member_parity(const ss_sim* s) { return s->x == s->x_buf0 ? 0 : 1; }
 
+// Job tables for the members' current buffers (slot 0) and the swapped ones
+// (slot 1), each for an update in place and for an update that carries out
+// a pending re-sort.
 template <typename T, int NM>
 static int ens_build_jobs(ss_ensemble* e) {
     const int n = (int)e->m.size();
-    const int K = e->resort;
-    std::vector<TileArgs<T, NM>> h((size_t)2 * K * n);
+    std::vector<TileArgs<T, NM>> h((size_t)4 * n);
     for (int slot = 0; slot < 2; ++slot) {
         for (ss_sim* s : e->m)
-            if (slot == 1) swap_ab(s);  // the flipped configuration, restored below
-        for (int k = 0; k < K; ++k)
+            if (slot == 1) {  // the swapped configuration, restored below
+                swap_ab(s);
+                std::swap(s->toff, s->toff2);
+            }
+        for (int sc = 0; sc < 2; ++sc)
             for (int j = 0; j < n; ++j) {
                 ss_sim* s = e->m[j];
-                const uint32_t off = s->step_off;
-                s->step_off = (uint32_t)k;
-                h[((size_t)slot * K + k) * n + j] = tile_args<T, NM>(s);
-                s->step_off = off;
+                const bool pend = s->pending;
+                s->pending = sc == 1;
+                h[((size_t)slot * 2 + sc) * n + j] = tile_args<T, NM>(s);
+                s->pending = pend;
             }
         for (ss_sim* s : e->m)
-            if (slot == 1) swap_ab(s);
+            if (slot == 1) {
+                swap_ab(s);
+                std::swap(s->toff, s->toff2);
+            }
     }
     SS_HIP_CHECK(hipMemcpy(e->d_jobs, h.data(), h.size() * sizeof(TileArgs<T, NM>), hipMemcpyHostToDevice));
     e->built_for.clear();
@@ -1430,23 +1470,38 @@ static int ens_build_jobs(ss_ensemble* e) {
 }
 
 template <typename T, int NM>
-static void ens_launch_update(ss_ensemble* e, int slot, int k) {
+static void ens_launch_update(ss_ensemble* e, int slot, bool resort, int k) {
     const int n = (int)e->m.size();
-    const TileArgs<T, NM>* jobs = reinterpret_cast<const TileArgs<T, NM>*>(e->d_jobs) + ((size_t)slot * e->resort + k) * n;
+    const TileArgs<T, NM>* jobs =
+        reinterpret_cast<const TileArgs<T, NM>*>(e->d_jobs) + ((size_t)slot * 2 + (resort ? 1 : 0)) * n;
     const int grid = std::max(1, std::min(e->total_blocks, 256 * 8));
     const size_t lds = e->lds + sizeof(int) * (n + 1);
-    if (e->ext)
-        hipLaunchKernelGGL((k_tile_update_batched<T, NM, true, true>), dim3(grid), dim3(256), lds, e->stream, jobs,
-                           e->d_start, n, (int)e->lds);
-    else
-        hipLaunchKernelGGL((k_tile_update_batched<T, NM, true, false>), dim3(grid), dim3(256), lds, e->stream,
-                           jobs, e->d_start, n, (int)e->lds);
+    const int nj = (int)e->lds;
+    const uint32_t ko = (uint32_t)k;
+    if (e->ext) {
+        if (resort)
+            hipLaunchKernelGGL((k_tile_update_batched<T, NM, true, true, true>), dim3(grid), dim3(256), lds,
+                               e->stream, jobs, e->d_start, n, nj, ko);
+        else
+            hipLaunchKernelGGL((k_tile_update_batched<T, NM, true, true, false>), dim3(grid), dim3(256), lds,
+                               e->stream, jobs, e->d_start, n, nj, ko);
+    } else {
+        if (resort)
+            hipLaunchKernelGGL((k_tile_update_batched<T, NM, true, false, true>), dim3(grid), dim3(256), lds,
+                               e->stream, jobs, e->d_start, n, nj, ko);
+        else
+            hipLaunchKernelGGL((k_tile_update_batched<T, NM, true, false, false>), dim3(grid), dim3(256), lds,
+                               e->stream, jobs, e->d_start, n, nj, ko);
+    }
 }
 
-// One ensemble step; `k` is the step's index within the re-sort period,
-// `slot` the job table matching the members' state buffers.
+// Job-table slot matching the members' current buffers.
+static int ens_slot(const ss_ensemble* e) { return member_parity(e->m[0]) == e->built_for[0] ? 0 : 1; }
+
+// One ensemble step; `k` is the step's index within the re-sort period (the
+// step counter offset while capturing).
 template <typename T>
-static int ens_step_once(ss_ensemble* e, int slot, int k, bool capturing) {
+static int ens_step_once(ss_ensemble* e, int k, bool capturing) {
     hipError_t err;
     SS_HIP_CHECK(hipEventRecord(e->fork, e->stream));
     for (int q = 0; q < NSIDE; ++q) SS_HIP_CHECK(hipStreamWaitEvent(e->side[q], e->fork, 0));
@@ -1463,25 +1518,28 @@ static int ens_step_once(ss_ensemble* e, int slot, int k, bool capturing) {
         SS_HIP_CHECK(hipEventRecord(e->join[q], e->side[q]));
         SS_HIP_CHECK(hipStreamWaitEvent(e->stream, e->join[q], 0));
     }
+    const int slot = ens_slot(e);
+    const bool resort = e->m[0]->pending;
     if (e->nm == 6)
-        ens_launch_update<T, 6>(e, slot, k);
+        ens_launch_update<T, 6>(e, slot, resort, k);
     else if (e->nm == 9)
-        ens_launch_update<T, 9>(e, slot, k);
+        ens_launch_update<T, 9>(e, slot, resort, k);
     else
-        ens_launch_update<T, 12>(e, slot, k);
+        ens_launch_update<T, 12>(e, slot, resort, k);
     SS_HIP_CHECK(hipGetLastError());
     e->steps_since_sort += 1;
     for (ss_sim* s : e->m) {
+        if (resort) swap_sorted(s);
         s->steps_since_sort += 1;
         s->timestep += 1;
         s->have_fields = true;
     }
     if (e->steps_since_sort >= (uint64_t)e->resort) {
-        // the sorts run on the side streams, then join
+        // the next re-sort's offsets, on the side streams, then join
         SS_HIP_CHECK(hipEventRecord(e->fork, e->stream));
         for (int q = 0; q < NSIDE; ++q) SS_HIP_CHECK(hipStreamWaitEvent(e->side[q], e->fork, 0));
         for (ss_sim* s : e->m)
-            if ((err = launch_sort<T>(s, false)) != hipSuccess) return fail(SS_ERR_DEVICE, hipGetErrorString(err));
+            if ((err = launch_count_scan(s)) != hipSuccess) return fail(SS_ERR_DEVICE, hipGetErrorString(err));
         for (int q = 0; q < NSIDE; ++q) {
             SS_HIP_CHECK(hipEventRecord(e->join[q], e->side[q]));
             SS_HIP_CHECK(hipStreamWaitEvent(e->stream, e->join[q], 0));
@@ -1501,7 +1559,7 @@ static int ens_jobs_for_current(ss_ensemble* e) {
     bool flipped = e->built_for.size() == e->m.size();
     for (size_t j = 0; flipped && j < e->m.size(); ++j) flipped = e->built_for[j] != member_parity(e->m[j]);
     if (flipped) return 1;
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < 4; ++k)
         if (e->graph[k]) {
             (void)hipGraphExecDestroy(e->graph[k]);
             e->graph[k] = nullptr;
@@ -1517,10 +1575,12 @@ static int ens_run(ss_ensemble* e, uint64_t nsteps) {
         const int slot = ens_jobs_for_current<T>(e);
         if (slot < 0) return slot;
         if (e->steps_since_sort == 0 && nsteps >= (uint64_t)K) {
-            if (!e->graph[slot]) {
+            const bool pending = e->m[0]->pending;
+            const int key = (pending ? 2 : 0) + slot;
+            if (!e->graph[key]) {
                 SS_HIP_CHECK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
                 int rc = SS_OK;
-                for (int k = 0; k < K && rc == SS_OK; ++k) rc = ens_step_once<T>(e, slot, k, true);
+                for (int k = 0; k < K && rc == SS_OK; ++k) rc = ens_step_once<T>(e, k, true);
                 if (rc == SS_OK)
                     hipLaunchKernelGGL(k_add_step, dim3(1), dim3(64), 0, e->stream, e->d_step, (uint32_t)K);
                 hipGraph_t g = nullptr;
@@ -1530,26 +1590,28 @@ static int ens_run(ss_ensemble* e, uint64_t nsteps) {
                     return rc;
                 }
                 if (err != hipSuccess) return fail(SS_ERR_DEVICE, std::string("ensemble capture: ") + hipGetErrorString(err));
-                err = hipGraphInstantiate(&e->graph[slot], g, nullptr, nullptr, 0);
+                err = hipGraphInstantiate(&e->graph[key], g, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(g);
                 if (err != hipSuccess) {
-                    e->graph[slot] = nullptr;
+                    e->graph[key] = nullptr;
                     return fail(SS_ERR_DEVICE, std::string("ensemble instantiate: ") + hipGetErrorString(err));
                 }
             } else {
+                // advance the bookkeeping the captured period performs
                 for (ss_sim* s : e->m) {
                     s->timestep += K;
-                    swap_ab(s);
+                    if (pending) swap_sorted(s);
+                    s->pending = true;
                     s->steps_since_sort = 0;
                     s->have_fields = true;
                 }
                 e->timestep += K;
             }
-            SS_HIP_CHECK(hipGraphLaunch(e->graph[slot], e->stream));
+            SS_HIP_CHECK(hipGraphLaunch(e->graph[key], e->stream));
             nsteps -= (uint64_t)K;
         } else {
             // eager step: the job table's offsets are relative to the counter
-            int rc = ens_step_once<T>(e, slot, 0, false);
+            int rc = ens_step_once<T>(e, 0, false);
             if (rc != SS_OK) return rc;
             nsteps -= 1;
         }
@@ -1610,13 +1672,13 @@ int ss_ensemble_create(const ss_params* params, int nsys, ss_ensemble** out) {
         s->d_step = e->d_step;
         const int per = (s->tg.ntiles + s->tg.nxcd - 1) / s->tg.nxcd;
         start.push_back(start.back() + per * s->tg.nxcd);
-        lds = std::max(lds, (size_t)s->tg.ws * (s->nm + 1) * 8 + (size_t)s->nm * s->na * 8);
+        lds = std::max(lds, tile_lds_bytes(s->tg.ws, s->nm, s->na));
     }
     e->total_blocks = start.back();
     e->lds = (lds + 15) / 16 * 16;
     e->job_bytes = e->prec == SS_FP32 ? (e->nm == 6 ? sizeof(TileArgs<float, 6>) : e->nm == 9 ? sizeof(TileArgs<float, 9>) : sizeof(TileArgs<float, 12>))
                                       : (e->nm == 6 ? sizeof(TileArgs<double, 6>) : e->nm == 9 ? sizeof(TileArgs<double, 9>) : sizeof(TileArgs<double, 12>));
-    if (hipMalloc(&e->d_jobs, e->job_bytes * 2 * e->resort * nsys) != hipSuccess ||
+    if (hipMalloc(&e->d_jobs, e->job_bytes * 4 * nsys) != hipSuccess ||
         hipMalloc(&e->d_start, sizeof(int) * (nsys + 1)) != hipSuccess)
         return bail(fail(SS_ERR_ALLOC, "hipMalloc"));
     if (hipMemcpy(e->d_start, start.data(), sizeof(int) * (nsys + 1), hipMemcpyHostToDevice) != hipSuccess)
@@ -1628,7 +1690,7 @@ int ss_ensemble_create(const ss_params* params, int nsys, ss_ensemble** out) {
 int ss_ensemble_destroy(ss_ensemble* e) {
     if (!e) return SS_OK;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < 4; ++k)
         if (e->graph[k]) (void)hipGraphExecDestroy(e->graph[k]);
     for (ss_sim* s : e->m) {
         if (s->d_step == e->d_step) s->d_step = nullptr;
@@ -1663,7 +1725,8 @@ int ss_ensemble_step(ss_ensemble* e, uint64_t nsteps, uint64_t* timestep_out) {
         if (!s->initialised) return fail(SS_ERR_STATE, "every member needs ss_init before ss_ensemble_step");
         if (s->p.noise_mode == SS_NOISE_EXTERNAL && !s->have_noise)
             return fail(SS_ERR_STATE, "SS_NOISE_EXTERNAL needs ss_set_noise for every member");
-        if (s->timestep != e->m[0]->timestep || s->steps_since_sort != e->m[0]->steps_since_sort)
+        if (s->timestep != e->m[0]->timestep || s->steps_since_sort != e->m[0]->steps_since_sort ||
+            s->pending != e->m[0]->pending)
             return fail(SS_ERR_STATE, "ensemble members are out of step: initialise them together");
     }
     SS_HIP_CHECK(hipSetDevice(e->m[0]->p.device));

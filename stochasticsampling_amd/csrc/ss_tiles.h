@@ -72,16 +72,29 @@ struct TileArgs {
     long long* main;        // [ns][NM+1] moments of each tile's own cells
     long long* inflow;      // [ns][NM+1] contributions from other tiles
     uint32_t* hist;         // optional [ns][na] counts (kept distribution)
+    // re-sort fused into the update: when dtoff is set, every particle is
+    // written to state d at a slot of the segment of the tile it occupied at
+    // the start of the step (offsets dtoff, from that tile's count)
+    State<T> d;
+    const uint32_t* dtoff;  // [ntiles + 1] or nullptr (update in place)
+    uint32_t* cross;        // [ntiles] particles placed from a segment's end
 };
+
+// Dynamic LDS of the tile update: window accumulators, moment kernels and
+// the counter of a block's particles that stay in its tile.
+inline size_t tile_lds_bytes(int ws, int nm, int na) {
+    return ((size_t)ws * (nm + 1) * 8 + (size_t)nm * na * 8 + 16 + 15) / 16 * 16;
+}
 
 // One tile of one system: move its particles (MOVE) and bin them into the
 // next step's moments.  MOVE = false bins the current state (initial
 // sampling).  `block` is the workgroup's index within the system's launch.
-template <typename T, int NM, bool MOVE, bool EXT_NOISE, bool KEEP>
+template <typename T, int NM, bool MOVE, bool EXT_NOISE, bool KEEP, bool SCATTER>
 __device__ inline void tile_update_body(const TileArgs<T, NM>& a, int block, unsigned char* smem_raw) {
     constexpr int NC = NM + 1;
     long long* acc = reinterpret_cast<long long*>(smem_raw);  // [ws][NC]
     long long* kf = acc + (size_t)a.tg.ws * NC;               // [NM][na]
+    unsigned* home = reinterpret_cast<unsigned*>(kf + (size_t)NM * a.g.na);
     const Geo<T>& g = a.g;
     const TileGeo& tg = a.tg;
     const int tile = tile_for_block(tg, block);
@@ -91,16 +104,68 @@ __device__ inline void tile_update_body(const TileArgs<T, NM>& a, int block, uns
 
     for (int k = threadIdx.x; k < tg.ws * NC; k += blockDim.x) acc[k] = 0;
     for (int k = threadIdx.x; k < NM * g.na; k += blockDim.x) kf[k] = a.kfix[k];
+    if (threadIdx.x == 0) *home = 0;
     __syncthreads();
 
     const uint32_t p0 = a.toff[tile], p1 = a.toff[tile + 1];
-    for (uint32_t i = p0 + threadIdx.x; i < p1; i += blockDim.x) {
+    const uint64_t step = MOVE ? *a.step + a.step_off : 0;
+    constexpr bool scatter = MOVE && SCATTER;
+    const uint32_t hbase = scatter ? a.dtoff[tile] : 0u;
+    // Software pipeline: the next particle's state is loaded before the
+    // current one is updated, so that each wavefront has two iterations of
+    // loads in flight.
+    uint32_t i = p0 + threadIdx.x;
+    Loaded<T> cur{};
+    uint32_t cid = 0;
+    if (MOVE && i < p1) {
+        cid = a.s.id[i];
+        load_particle(a.s, i, cur);
+    }
+    for (; i < p1; i += blockDim.x) {
         T nx, ny, nz, ophi, otheta;
         if (MOVE) {
-            const uint32_t id = a.s.id[i];
+            // the cell record is requested before the next particle, so that
+            // waiting for it leaves the next particle's loads in flight
+            const int cx = grid_index(cur.x, g.gx, g.nx), cy = grid_index(cur.y, g.gy, g.ny),
+                      cz = grid_index(cur.z, g.gz, g.nz);
+            const T* recp = a.rec + (size_t)((cx * g.ny + cy) * g.nz + cz) * a.rs;
+            float r[REC_FULL];
+            if constexpr (sizeof(T) == 4) gather_record(recp, a.rs, r);
+            // unconditional (clamped to the segment's last slot): a branch
+            // here would make the wait for the record a wait for everything
+            const uint32_t j = min(i + blockDim.x, p1 - 1);
+            Loaded<T> nxt;
+            const uint32_t nid = a.s.id[j];
+            load_particle(a.s, j, nxt);
             Noise<T> rv;
-            draw_noise<T>(a.P, a.seed, *a.step + a.step_off, id, a.n, a.noise, EXT_NOISE, rv);
-            move_particle<T>(a.s, i, id, g, a.P, a.rec, a.rs, rv, nx, ny, nz, ophi, otheta);
+            draw_noise<T>(a.P, a.seed, step, cid, a.n, a.noise, EXT_NOISE, rv);
+            State<T> out = a.s;
+            uint32_t o = i;
+            if constexpr (scatter) {
+                // particles still in this tile fill its new segment from the
+                // front (one LDS reservation per wavefront), the others fill
+                // their tile's segment from the back
+                const uint32_t ts = (uint32_t)tile_of(tg, cx, cy, cz);
+                const bool h = ts == (uint32_t)tile;
+                const uint64_t m = __ballot(h);
+                const int lane = __lane_id();
+                if (m != 0) {
+                    const int leader = __ffsll((unsigned long long)m) - 1;
+                    uint32_t base = 0;
+                    if (lane == leader) base = atomicAdd(home, (unsigned)__popcll(m));
+                    base = __shfl(base, leader);
+                    if (h) o = hbase + base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                }
+                if (!h) o = a.dtoff[ts + 1] - 1u - atomicAdd(a.cross + ts, 1u);
+                out = a.d;
+                out.id[o] = cid;
+            }
+            if constexpr (sizeof(T) == 4)
+                move_loaded<T>(out, o, cur, a.P, reinterpret_cast<const T*>(r), rv, nx, ny, nz, ophi, otheta);
+            else
+                move_loaded<T>(out, o, cur, a.P, recp, rv, nx, ny, nz, ophi, otheta);
+            cur = nxt;
+            cid = nid;
         } else {
             nx = a.s.x[i];
             ny = a.s.y[i];
@@ -159,10 +224,10 @@ __device__ inline void tile_update_body(const TileArgs<T, NM>& a, int block, uns
     }
 }
 
-template <typename T, int NM, bool MOVE, bool EXT_NOISE, bool KEEP>
+template <typename T, int NM, bool MOVE, bool EXT_NOISE, bool KEEP, bool SCATTER>
 __global__ void __launch_bounds__(256) k_tile_update(TileArgs<T, NM> a) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-    tile_update_body<T, NM, MOVE, EXT_NOISE, KEEP>(a, blockIdx.x, smem_raw);
+    tile_update_body<T, NM, MOVE, EXT_NOISE, KEEP, SCATTER>(a, blockIdx.x, smem_raw);
 }
 
 // Many systems in one persistent launch: work item w of the concatenated
@@ -173,10 +238,10 @@ constexpr int MAX_BATCH = 256;
 // Dynamic LDS: [lds_tile bytes for the tile body][njobs + 1 ints of start]; no
 // static LDS, so the 64-bit accumulators at the region's start stay 8-byte
 // aligned.
-template <typename T, int NM, bool MOVE, bool EXT_NOISE>
+template <typename T, int NM, bool MOVE, bool EXT_NOISE, bool SCATTER>
 __global__ void __launch_bounds__(256)
     k_tile_update_batched(const TileArgs<T, NM>* __restrict__ jobs, const int* __restrict__ start, int njobs,
-                          int lds_tile) {
+                          int lds_tile, uint32_t step_off) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     int* sstart = reinterpret_cast<int*>(smem_raw + lds_tile);
     for (int k = threadIdx.x; k <= njobs; k += blockDim.x) sstart[k] = start[k];
@@ -192,8 +257,9 @@ __global__ void __launch_bounds__(256)
                 hi = mid;
         }
         lo = __builtin_amdgcn_readfirstlane(lo);  // uniform: keeps the job in scalar registers
-        const TileArgs<T, NM> a = jobs[lo];
-        tile_update_body<T, NM, MOVE, EXT_NOISE, false>(a, w - sstart[lo], smem_raw);
+        TileArgs<T, NM> a = jobs[lo];
+        a.step_off = step_off;
+        tile_update_body<T, NM, MOVE, EXT_NOISE, false, SCATTER>(a, w - sstart[lo], smem_raw);
         __syncthreads();
     }
 }
@@ -218,26 +284,27 @@ __global__ void __launch_bounds__(256)
     if (rho) rho[c] = (T)((double)cntv * rho_scale);
 }
 
-// Particles per tile from the moment arrays' count component (sort after a
-// step) -- one thread per tile.
+// Particles per tile from the moment arrays' count component (the next
+// re-sort) -- one wavefront per tile, its lanes over the tile's cells.
 template <int NM>
 __global__ void __launch_bounds__(256)
     k_tile_counts(const long long* __restrict__ main, const long long* __restrict__ inflow, int nx, int ny,
                   int nz, TileGeo tg, uint32_t* __restrict__ tcnt) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (t >= tg.ntiles) return;
     constexpr int NC = NM + 1;
     const int tz = t % tg.nt[2], ty = (t / tg.nt[2]) % tg.nt[1], tx = t / (tg.nt[2] * tg.nt[1]);
     const int ox = tx * tg.t[0], oy = ty * tg.t[1], oz = tz * tg.t[2];
     const int ex = min(tg.t[0], nx - ox), ey = min(tg.t[1], ny - oy), ez = min(tg.t[2], nz - oz);
+    const int nc = ex * ey * ez;
     long long sum = 0;
-    for (int i = 0; i < ex; ++i)
-        for (int j = 0; j < ey; ++j)
-            for (int k = 0; k < ez; ++k) {
-                const size_t cell = (size_t)((ox + i) * ny + oy + j) * nz + oz + k;
-                sum += main[cell * NC + NM] + inflow[cell * NC + NM];
-            }
-    tcnt[t] = (uint32_t)sum;
+    for (int c = __lane_id(); c < nc; c += 64) {
+        const int k = c % ez, j = (c / ez) % ey, i = c / (ez * ey);
+        const size_t cell = (size_t)((ox + i) * ny + oy + j) * nz + oz + k;
+        sum += main[cell * NC + NM] + inflow[cell * NC + NM];
+    }
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+    if (__lane_id() == 0) tcnt[t] = (uint32_t)sum;
 }
 
 // Particles per tile from the particle state (initial sort).
@@ -290,82 +357,6 @@ __global__ void __launch_bounds__(256)
     base = __shfl(base, r.head);
     if (!valid) return;
     copy_particle(a, b, i, toff[t] + base + (uint32_t)(__lane_id() - r.head));
-}
-
-// Re-sort of particles that are already grouped by tile up to the few that
-// crossed a tile face since the last sort.  Each wavefront owns PW_ITER * 64
-// consecutive particles and first counts them per tile in a wave-private LDS
-// table, so that a tile's cursor receives one atomic per wavefront and tile
-// instead of one per 64 particles; a table that fills up falls back to one
-// atomic per run.  A second pass scatters the particles.
-constexpr int PW_ITER = 16;
-constexpr int PW_TAB = 64;
-
-template <typename T>
-__global__ void __launch_bounds__(256)
-    k_tile_permute_agg(State<T> a, State<T> b, uint64_t n, Geo<T> g, TileGeo tg, const uint32_t* __restrict__ toff,
-                       uint32_t* __restrict__ cursor) {
-    __shared__ uint32_t tab_t[4][PW_TAB];
-    __shared__ uint32_t tab_c[4][PW_TAB];
-    const int wave = threadIdx.x >> 6, lane = __lane_id();
-    const uint64_t w0 = ((uint64_t)blockIdx.x * 4 + wave) * (PW_ITER * 64);
-    uint32_t* tt = tab_t[wave];
-    uint32_t* tc = tab_c[wave];
-    int nent = 0;  // wave-uniform
-    uint32_t rank[PW_ITER];  // rank within the table entry, or absolute slot (bit 31)
-#pragma unroll
-    for (int it = 0; it < PW_ITER; ++it) {
-        const uint64_t i = w0 + (uint64_t)it * 64 + lane;
-        const bool valid = i < n;
-        const uint32_t t = valid ? tile_of_particle(a, i, g, tg) : INVALID_KEY;
-        Run r = wave_run(t);
-        unsigned long long heads = __ballot(r.is_head && valid);
-        uint32_t my = 0;
-        while (heads) {
-            const int h = __builtin_ctzll(heads);
-            heads &= heads - 1;
-            const uint32_t th = __shfl(t, h);
-            const uint32_t len = (uint32_t)__shfl(r.len, h);
-            const unsigned long long hit = __ballot(lane < nent && tt[lane] == th);
-            uint32_t before;
-            int e;
-            if (hit) {
-                e = __builtin_ctzll(hit);
-                before = tc[e];
-                if (lane == 0) tc[e] = before + len;
-            } else if (nent < PW_TAB) {
-                e = nent;
-                before = 0;
-                if (lane == 0) {
-                    tt[e] = th;
-                    tc[e] = len;
-                }
-                nent += 1;
-            } else {
-                e = -1;
-                before = 0;
-                if (lane == h) before = atomicAdd(cursor + th, len);
-                before = __shfl(before, h);
-            }
-            __builtin_amdgcn_wave_barrier();
-            if (r.head == h) my = (e >= 0) ? (before + (uint32_t)(lane - h)) | ((uint32_t)e << 24)
-                                          : (toff[th] + before + (uint32_t)(lane - h)) | 0x80000000u;
-        }
-        rank[it] = my;
-    }
-    // one reservation per table entry
-    uint32_t ebase = 0;
-    if (lane < nent) ebase = atomicAdd(cursor + tt[lane], tc[lane]) + toff[tt[lane]];
-#pragma unroll
-    for (int it = 0; it < PW_ITER; ++it) {
-        const uint64_t i = w0 + (uint64_t)it * 64 + lane;
-        const uint32_t rk = rank[it];
-        const int e = (int)((rk >> 24) & 0x7F);
-        const uint32_t eb = __shfl(ebase, e);
-        if (i >= n) continue;
-        const uint32_t dst = (rk & 0x80000000u) ? (rk & 0x7FFFFFFFu) : eb + (rk & 0xFFFFFFu);
-        copy_particle(a, b, i, dst);
-    }
 }
 
 }  // namespace ss
