@@ -55,6 +55,9 @@ extern "C" {
 #define SS_FLAG_KEEP_DISTRIBUTION 4u  /* keep the full count grid of every step's binning         */
 #define SS_FLAG_NO_GRAPH 8u           /* enqueue every step eagerly instead of replaying captured
                                          hipGraphs of whole re-sort periods                       */
+#define SS_FLAG_AXIS_PASSES 16u       /* tiled path: run the spectral stage as separate kernels
+                                         (convert, one pass per FFT axis, solve, pack) instead of
+                                         the three fused plane/pencil kernels (same results)      */
 
 /* Size of the simulation box (lib.rs BoxSize). */
 typedef struct ss_box {
@@ -201,6 +204,19 @@ int ss_get_diagnostics(ss_sim* sim, double* out);
 
 /* Wait for all work of the handle. */
 int ss_synchronize(ss_sim* sim);
+
+/* Execution plan chosen at ss_create (no reference counterpart; for tests and
+   reports). */
+typedef struct ss_info {
+    int32_t sorted;          /* tiled binning path (particles grouped by tile) */
+    int32_t fused_spectral;  /* spectral stage as the three fused kernels      */
+    int32_t tile[3];         /* tile extent in cells (tiled path)              */
+    int32_t ntiles;
+    int32_t resort_interval; /* steps between re-sorts (tiled path)            */
+    int32_t pencils;         /* (y, z) pencils per workgroup (fused spectral)  */
+    int32_t graphs;          /* whole periods replayed from captured hipGraphs */
+} ss_info;
+int ss_get_info(const ss_sim* sim, ss_info* out);
 
 /* ---------------------------------------------------------------------------
  * Batched ensemble -- no reference counterpart (the reference runs one system

@@ -6,6 +6,7 @@ from .params import (  # noqa: F401
     BIN_SORTED,
     FFT_AUTO,
     FFT_RADIX,
+    FLAG_AXIS_PASSES,
     FLAG_KEEP_DISTRIBUTION,
     FLAG_MAGNETIC_ALWAYS,
     FLAG_NO_GRAPH,

@@ -11,6 +11,7 @@
 
 #include <cstring>
 
+#include "ss_fft_device.h"
 #include "ss_internal.h"
 
 namespace ss {
@@ -41,49 +42,6 @@ struct AxisDesc {
     int contiguous;    // lines are contiguous runs of n elements
 };
 
-template <typename T>
-__device__ inline cplx<T> cmul(cplx<T> a, cplx<T> b) {
-    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
-}
-
-// Radix-R DFT of v in place; root(t) = exp(sign 2 pi i t / R) read from the
-// twiddle table of the full length n at index t*(n/R).
-template <typename T, int R>
-__device__ inline void dft_small(cplx<T>* v, const cplx<T>* tw, int n, int sign) {
-    if constexpr (R == 2) {
-        cplx<T> a = v[0], b = v[1];
-        v[0] = {a.x + b.x, a.y + b.y};
-        v[1] = {a.x - b.x, a.y - b.y};
-    } else if constexpr (R == 4) {
-        cplx<T> a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3];
-        cplx<T> s02 = {a0.x + a2.x, a0.y + a2.y}, d02 = {a0.x - a2.x, a0.y - a2.y};
-        cplx<T> s13 = {a1.x + a3.x, a1.y + a3.y}, d13 = {a1.x - a3.x, a1.y - a3.y};
-        // multiply d13 by (sign * i)
-        cplx<T> jd13 = sign < 0 ? cplx<T>{d13.y, -d13.x} : cplx<T>{-d13.y, d13.x};
-        v[0] = {s02.x + s13.x, s02.y + s13.y};
-        v[2] = {s02.x - s13.x, s02.y - s13.y};
-        v[1] = {d02.x + jd13.x, d02.y + jd13.y};
-        v[3] = {d02.x - jd13.x, d02.y - jd13.y};
-    } else {
-        cplx<T> w[R];
-        const int step = n / R;
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-            T sr = T(0), si = T(0);
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                cplx<T> c = tw[((r * q) % R) * step];
-                if (sign > 0) c.y = -c.y;
-                sr += v[r].x * c.x - v[r].y * c.y;
-                si += v[r].x * c.y + v[r].y * c.x;
-            }
-            w[q] = {sr, si};
-        }
-#pragma unroll
-        for (int q = 0; q < R; ++q) v[q] = w[q];
-    }
-}
-
 // Generic radix for odd primes above 5 (test sizes such as 7, 11, 13).
 template <typename T>
 __device__ inline void dft_generic(cplx<T>* v, int R, const cplx<T>* tw, int n, int sign) {
@@ -100,34 +58,6 @@ __device__ inline void dft_generic(cplx<T>* v, int R, const cplx<T>* tw, int n, 
         w[q] = {sr, si};
     }
     for (int q = 0; q < R; ++q) v[q] = w[q];
-}
-
-template <typename T, int R>
-__device__ inline void stage(const cplx<T>* src, cplx<T>* dst, const cplx<T>* tw, int n, int Ns,
-                             int nlines, int sign) {
-    const int m = n / R;
-    const int tstep = n / (Ns * R);
-    for (int b = threadIdx.x; b < nlines * m; b += blockDim.x) {
-        const int l = b / m, j = b - l * m;
-        const int k = j % Ns;
-        const cplx<T>* s = src + l * n;
-        cplx<T> v[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            cplx<T> x = s[j + r * m];
-            if (r > 0 && k > 0) {
-                cplx<T> c = tw[(r * k * tstep) % n];
-                if (sign > 0) c.y = -c.y;
-                x = cmul(x, c);
-            }
-            v[r] = x;
-        }
-        dft_small<T, R>(v, tw, n, sign);
-        const int idxD = (j / Ns) * Ns * R + k;
-        cplx<T>* d = dst + l * n;
-#pragma unroll
-        for (int r = 0; r < R; ++r) d[idxD + r * Ns] = v[r];
-    }
 }
 
 template <typename T>
@@ -191,10 +121,10 @@ __global__ void __launch_bounds__(256) k_fft_axis(cplx<T>* __restrict__ data, Ax
     for (int s = 0; s < d.nf; ++s) {
         const int R = d.fac[s];
         switch (R) {
-            case 4: stage<T, 4>(src, dst, tw, d.n, Ns, nl, sign); break;
-            case 2: stage<T, 2>(src, dst, tw, d.n, Ns, nl, sign); break;
-            case 3: stage<T, 3>(src, dst, tw, d.n, Ns, nl, sign); break;
-            case 5: stage<T, 5>(src, dst, tw, d.n, Ns, nl, sign); break;
+            case 4: stage<T, 4, false>(src, dst, tw, d.n, Ns, nl, d.n, 1, sign); break;
+            case 2: stage<T, 2, false>(src, dst, tw, d.n, Ns, nl, d.n, 1, sign); break;
+            case 3: stage<T, 3, false>(src, dst, tw, d.n, Ns, nl, d.n, 1, sign); break;
+            case 5: stage<T, 5, false>(src, dst, tw, d.n, Ns, nl, d.n, 1, sign); break;
             default:
                 if constexpr (GENERIC) stage_generic<T>(src, dst, tw, d.n, Ns, R, nl, sign);
                 break;
@@ -322,18 +252,28 @@ static int run_axis(cplx<T>* data, const FftAxis& fa, int axis, int nx, int ny, 
     return SS_OK;
 }
 
+int Fft3d::run_axis(void* data, int batch, int sign, int axis, hipStream_t stream) const {
+    if (precision == SS_FP64)
+        return ss::run_axis<double>(reinterpret_cast<cplx<double>*>(data), ax[axis], axis, nx, ny, nz, batch, sign,
+                                    stream);
+    return ss::run_axis<float>(reinterpret_cast<cplx<float>*>(data), ax[axis], axis, nx, ny, nz, batch, sign,
+                               stream);
+}
+
 int Fft3d::run(void* data, int batch, int sign, hipStream_t stream) const {
-    int rc;
-    for (int axis = 2; axis >= 0; --axis) {
-        if (precision == SS_FP64)
-            rc = run_axis<double>(reinterpret_cast<cplx<double>*>(data), ax[axis], axis, nx, ny, nz,
-                                  batch, sign, stream);
-        else
-            rc = run_axis<float>(reinterpret_cast<cplx<float>*>(data), ax[axis], axis, nx, ny, nz,
-                                 batch, sign, stream);
+    for (int k = 0; k < 3; ++k) {
+        const int axis = sign < 0 ? 2 - k : k;
+        const int rc = run_axis(data, batch, sign, axis, stream);
         if (rc != SS_OK) return rc;
     }
     return SS_OK;
+}
+
+bool Fft3d::small_radices() const {
+    for (int a = 0; a < 3; ++a)
+        for (int s = 0; s < ax[a].nf; ++s)
+            if (ax[a].fac[s] > 5) return false;
+    return true;
 }
 
 }  // namespace ss

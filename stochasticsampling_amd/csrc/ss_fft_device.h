@@ -1,0 +1,125 @@
+// ss_fft_device.h -- radix stages of the LDS Stockham FFT, shared by the
+// axis-pass kernels (ss_fft.hip) and the fused spectral-step kernels
+// (ss_spectral.h), so that both run the same arithmetic on each line.
+//
+// Element e of line l lives at buf[l * ls + e * es].  Convention of FFTW
+// (crates/fftw3/src/fft.rs:156-231): sign -1 forward, unnormalised.
+#pragma once
+#include "ss_device.h"
+
+namespace ss {
+
+template <typename T>
+__device__ inline cplx<T> cmul(cplx<T> a, cplx<T> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+// Radix-R DFT of v in place; root(t) = exp(sign 2 pi i t / R) read from the
+// twiddle table of the full length n at index t*(n/R).
+template <typename T, int R>
+__device__ inline void dft_small(cplx<T>* v, const cplx<T>* tw, int n, int sign) {
+    if constexpr (R == 2) {
+        cplx<T> a = v[0], b = v[1];
+        v[0] = {a.x + b.x, a.y + b.y};
+        v[1] = {a.x - b.x, a.y - b.y};
+    } else if constexpr (R == 4) {
+        cplx<T> a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3];
+        cplx<T> s02 = {a0.x + a2.x, a0.y + a2.y}, d02 = {a0.x - a2.x, a0.y - a2.y};
+        cplx<T> s13 = {a1.x + a3.x, a1.y + a3.y}, d13 = {a1.x - a3.x, a1.y - a3.y};
+        // multiply d13 by (sign * i)
+        cplx<T> jd13 = sign < 0 ? cplx<T>{d13.y, -d13.x} : cplx<T>{-d13.y, d13.x};
+        v[0] = {s02.x + s13.x, s02.y + s13.y};
+        v[2] = {s02.x - s13.x, s02.y - s13.y};
+        v[1] = {d02.x + jd13.x, d02.y + jd13.y};
+        v[3] = {d02.x - jd13.x, d02.y - jd13.y};
+    } else {
+        cplx<T> w[R];
+        const int step = n / R;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            T sr = T(0), si = T(0);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                cplx<T> c = tw[((r * q) % R) * step];
+                if (sign > 0) c.y = -c.y;
+                sr += v[r].x * c.x - v[r].y * c.y;
+                si += v[r].x * c.y + v[r].y * c.x;
+            }
+            w[q] = {sr, si};
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[q] = w[q];
+    }
+}
+
+// One radix-R Stockham stage over nlines lines of length n (Ns = product of
+// the radices of the earlier stages).  LINES_FAST: consecutive threads take
+// consecutive lines (lines interleaved in memory, es > 1); otherwise
+// consecutive butterflies of one line (contiguous lines, es = 1).
+template <typename T, int R, bool LINES_FAST>
+__device__ inline void stage(const cplx<T>* src, cplx<T>* dst, const cplx<T>* tw, int n, int Ns, int nlines,
+                             int ls, int es, int sign) {
+    const int m = n / R;
+    const int tstep = n / (Ns * R);
+    for (int b = threadIdx.x; b < nlines * m; b += blockDim.x) {
+        int l, j;
+        if (LINES_FAST) {
+            l = b % nlines;
+            j = b / nlines;
+        } else {
+            l = b / m;
+            j = b - l * m;
+        }
+        const int k = j % Ns;
+        const cplx<T>* s = src + l * ls;
+        cplx<T> v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            cplx<T> x = s[(j + r * m) * es];
+            if (r > 0 && k > 0) {
+                cplx<T> c = tw[(r * k * tstep) % n];
+                if (sign > 0) c.y = -c.y;
+                x = cmul(x, c);
+            }
+            v[r] = x;
+        }
+        dft_small<T, R>(v, tw, n, sign);
+        const int idxD = (j / Ns) * Ns * R + k;
+        cplx<T>* d = dst + l * ls;
+#pragma unroll
+        for (int r = 0; r < R; ++r) d[(idxD + r * Ns) * es] = v[r];
+    }
+}
+
+// Radix plan of one axis (factors 4, 2, 3, 5 only).
+struct SpecAxis {
+    int n, nf;
+    int fac[16];
+};
+
+// All stages of one axis, ping-ponging between a and b; returns the buffer
+// holding the result.  Ends with a barrier.
+template <typename T, bool LINES_FAST>
+__device__ inline cplx<T>* fft_lines(cplx<T>* a, cplx<T>* b, const cplx<T>* tw, const SpecAxis& ax, int nlines,
+                                     int ls, int es, int sign) {
+    cplx<T>* src = a;
+    cplx<T>* dst = b;
+    int Ns = 1;
+    for (int s = 0; s < ax.nf; ++s) {
+        const int R = ax.fac[s];
+        switch (R) {
+            case 4: stage<T, 4, LINES_FAST>(src, dst, tw, ax.n, Ns, nlines, ls, es, sign); break;
+            case 2: stage<T, 2, LINES_FAST>(src, dst, tw, ax.n, Ns, nlines, ls, es, sign); break;
+            case 3: stage<T, 3, LINES_FAST>(src, dst, tw, ax.n, Ns, nlines, ls, es, sign); break;
+            default: stage<T, 5, LINES_FAST>(src, dst, tw, ax.n, Ns, nlines, ls, es, sign); break;
+        }
+        __syncthreads();
+        cplx<T>* t = src;
+        src = dst;
+        dst = t;
+        Ns *= R;
+    }
+    return src;
+}
+
+}  // namespace ss

@@ -40,8 +40,13 @@ struct Fft3d {
     FftAxis ax[3];
     int init(int nx, int ny, int nz, int precision);
     void release();
-    // In-place, unnormalised; sign -1 forward, +1 backward.
+    // In-place, unnormalised; sign -1 forward, +1 backward.  The forward
+    // transform runs the axes z, y, x, the backward one x, y, z.
     int run(void* data, int batch, int sign, hipStream_t stream) const;
+    // One axis (0 = x, 1 = y, 2 = z) of the same transform.
+    int run_axis(void* data, int batch, int sign, int axis, hipStream_t stream) const;
+    // True when every axis factors into radices 2, 3, 4 and 5.
+    bool small_radices() const;
 };
 
 int factorize(int n, int* fac);

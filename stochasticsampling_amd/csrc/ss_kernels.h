@@ -141,6 +141,62 @@ struct SolveFlags {
     int want_e, want_b, want_gb;
 };
 
+// Stokes solve at one wave vector (spectral_solver.rs:133-189): v = sigma k,
+// v -= kn (kn . v), u = i v (1/k^2 + screening) / N.  sig(q) returns stress
+// component q: the 6 of a symmetric sigma (xx xy xz yy yz zz) or all 9.
+template <typename T, typename Sig>
+__device__ inline void flow_point(Sig sig, int nsig, const T k[3], T ki, T ik2, T screening, T norm, T ur[3],
+                                  T ui[3]) {
+    const T kn[3] = {k[0] * ki, k[1] * ki, k[2] * ki};
+    T sr[3], si[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        T ar = T(0), ai = T(0);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int q = nsig == 6 ? (i <= j ? 3 * i + j - (i * (i + 1)) / 2 : 3 * j + i - (j * (j + 1)) / 2)
+                                    : 3 * i + j;
+            const cplx<T> s = sig(q);
+            ar = ar + s.x * k[j];
+            ai = ai + s.y * k[j];
+        }
+        sr[i] = ar;
+        si[i] = ai;
+    }
+    T kr = (kn[0] * sr[0] + kn[1] * sr[1]) + kn[2] * sr[2];
+    T kim = (kn[0] * si[0] + kn[1] * si[1]) + kn[2] * si[2];
+    const T fac = (ik2 + screening) / norm;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        T ar = (sr[i] - kn[i] * kr) * fac;
+        T ai = (si[i] - kn[i] * kim) * fac;
+        ur[i] = -ai;
+        ui[i] = ar;
+    }
+}
+
+// G_ab = i k_a u_b (spectral_solver.rs:192-229)
+template <typename T>
+__device__ inline cplx<T> grad_u(const T k[3], const T ur[3], const T ui[3], int a, int b) {
+    return cplx<T>{-(k[a] * ui[b]), k[a] * ur[b]};
+}
+
+// W = (G - G^T) / 2 as W01, W02, W12 (w = 0..2) and E = (G + G^T) / 2 as E00,
+// E01, E02, E11, E12, E22 (w = 0..5) (src/bin/simulation/mod.rs:302-308).
+template <typename T>
+__device__ inline cplx<T> vort_part(const T k[3], const T ur[3], const T ui[3], int w) {
+    const int a = w == 2 ? 1 : 0, b = w == 0 ? 1 : 2;
+    const cplx<T> gab = grad_u(k, ur, ui, a, b), gba = grad_u(k, ur, ui, b, a);
+    return cplx<T>{(gab.x - gba.x) * T(0.5), (gab.y - gba.y) * T(0.5)};
+}
+template <typename T>
+__device__ inline cplx<T> strain_part(const T k[3], const T ur[3], const T ui[3], int w) {
+    const int a = w < 3 ? 0 : (w < 5 ? 1 : 2);
+    const int b = w < 3 ? w : (w < 5 ? w - 2 : 2);
+    const cplx<T> gab = grad_u(k, ur, ui, a, b), gba = grad_u(k, ur, ui, b, a);
+    return cplx<T>{(gab.x + gba.x) * T(0.5), (gab.y + gba.y) * T(0.5)};
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_solve(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, Geo<T> g,
@@ -153,62 +209,24 @@ __global__ void __launch_bounds__(256)
     const T k[3] = {kx[ix], ky[iy], kz[iz]};
     const T ki = kinv[idx];
     const T kn[3] = {k[0] * ki, k[1] * ki, k[2] * ki};
-    // sigma k, projection, Oseen factor, times i
     T ur[3], ui[3];
-    {
-        T sr[3], si[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            T ar = T(0), ai = T(0);
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                const int q = f.nsig == 6 ? (i <= j ? 3 * i + j - (i * (i + 1)) / 2
-                                                    : 3 * j + i - (j * (j + 1)) / 2)
-                                          : 3 * i + j;
-                const cplx<T> s = in[(size_t)q * ns + idx];
-                ar = ar + s.x * k[j];
-                ai = ai + s.y * k[j];
-            }
-            sr[i] = ar;
-            si[i] = ai;
-        }
-        T kr = (kn[0] * sr[0] + kn[1] * sr[1]) + kn[2] * sr[2];
-        T kim = (kn[0] * si[0] + kn[1] * si[1]) + kn[2] * si[2];
-        const T fac = (ik2[idx] + screening) / norm;
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            T ar = (sr[i] - kn[i] * kr) * fac;
-            T ai = (si[i] - kn[i] * kim) * fac;
-            ur[i] = -ai;
-            ui[i] = ar;
-        }
-    }
+    flow_point<T>([&](int q) { return in[(size_t)q * ns + idx]; }, f.nsig, k, ki, ik2[idx], screening, norm, ur,
+                  ui);
     int o = 0;
 #pragma unroll
     for (int i = 0; i < 3; ++i) out[(size_t)(o + i) * ns + idx] = cplx<T>{ur[i], ui[i]};
     o += 3;
-    // G_ab = i k_a u_b
-    auto G = [&](int a, int b) { return cplx<T>{-(k[a] * ui[b]), k[a] * ur[b]}; };
     if (f.full) {
         for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b) out[(size_t)(o + 3 * a + b) * ns + idx] = G(a, b);
+            for (int b = 0; b < 3; ++b) out[(size_t)(o + 3 * a + b) * ns + idx] = grad_u(k, ur, ui, a, b);
         o += 9;
     } else {
-        const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
 #pragma unroll
-        for (int w = 0; w < 3; ++w) {
-            cplx<T> gab = G(pa[w], pb[w]), gba = G(pb[w], pa[w]);
-            out[(size_t)(o + w) * ns + idx] = cplx<T>{(gab.x - gba.x) * T(0.5), (gab.y - gba.y) * T(0.5)};
-        }
+        for (int w = 0; w < 3; ++w) out[(size_t)(o + w) * ns + idx] = vort_part(k, ur, ui, w);
         o += 3;
         if (f.want_e) {
-            const int ea[6] = {0, 0, 0, 1, 1, 2}, eb[6] = {0, 1, 2, 1, 2, 2};
 #pragma unroll
-            for (int w = 0; w < 6; ++w) {
-                cplx<T> gab = G(ea[w], eb[w]), gba = G(eb[w], ea[w]);
-                out[(size_t)(o + w) * ns + idx] =
-                    cplx<T>{(gab.x + gba.x) * T(0.5), (gab.y + gba.y) * T(0.5)};
-            }
+            for (int w = 0; w < 6; ++w) out[(size_t)(o + w) * ns + idx] = strain_part(k, ur, ui, w);
             o += 6;
         }
     }

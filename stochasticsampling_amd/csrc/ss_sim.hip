@@ -19,6 +19,7 @@
 
 #include "ss_internal.h"
 #include "ss_kernels.h"
+#include "ss_spectral.h"
 #include "ss_tiles.h"
 
 namespace ss {
@@ -145,6 +146,11 @@ struct ss_sim {
     ss_params p;
     int prec = SS_FP64;
     bool sorted = false;
+    // spectral stage as three fused kernels (tiled path, no magnetic solve,
+    // radices 2-5); spec_in then holds moments transformed along z and y only
+    bool fused = false;
+    bool spec_partial = false;
+    int pencils = 8;  // (y, z) pencils per workgroup of the pencil kernel
     bool mag = false;  // magnetic solve runs
     bool use_e = false, use_rho = false, use_b = false, use_gb = false;
     int nsig = 6;
@@ -360,6 +366,59 @@ static hipError_t launch_pack(ss_sim* s) {
     return hipGetLastError();
 }
 
+static SpecAxis spec_axis(const FftAxis& a) {
+    SpecAxis r;
+    r.n = a.n;
+    r.nf = a.nf;
+    for (int k = 0; k < 16; ++k) r.fac[k] = a.fac[k];
+    return r;
+}
+
+template <typename T>
+static hipError_t launch_spec_planes_fwd(ss_sim* s) {
+    double ms, ds;
+    scales(s, ms, ds);
+    const size_t lds = spec_plane_lds(s->ny, s->nz, sizeof(T));
+    const dim3 grid((unsigned)(s->nx * s->nm));
+    T* rho = s->use_rho ? (T*)s->rho : nullptr;
+    const SpecAxis ay = spec_axis(s->fft.ax[1]), az = spec_axis(s->fft.ax[2]);
+    const cplx<T>* twy = (const cplx<T>*)s->fft.ax[1].tw;
+    const cplx<T>* twz = (const cplx<T>*)s->fft.ax[2].tw;
+    if (s->nm == 6)
+        hipLaunchKernelGGL((k_spec_planes_fwd<T, 6>), grid, dim3(256), lds, s->stream, s->mom_main, s->mom_inflow,
+                           s->nx, s->fixed_scale * ms, ds, ay, az, twy, twz, (cplx<T>*)s->spec_in, rho, (T*)s->rec,
+                           s->rs);
+    else
+        hipLaunchKernelGGL((k_spec_planes_fwd<T, 9>), grid, dim3(256), lds, s->stream, s->mom_main, s->mom_inflow,
+                           s->nx, s->fixed_scale * ms, ds, ay, az, twy, twz, (cplx<T>*)s->spec_in, rho, (T*)s->rec,
+                           s->rs);
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_spec_pencils(ss_sim* s) {
+    const int np = s->ny * s->nz;
+    const int nout = s->use_e ? 12 : 6;
+    const size_t lds = spec_pencil_lds(s->nx, std::max(s->nsig, nout) * s->pencils, sizeof(T));
+    const dim3 grid((unsigned)((np + s->pencils - 1) / s->pencils));
+    hipLaunchKernelGGL(k_spec_pencils<T>, grid, dim3(256), lds, s->stream, (const cplx<T>*)s->spec_in,
+                       (cplx<T>*)s->spec_out, np, s->nz, s->pencils, s->nsig, nout, spec_axis(s->fft.ax[0]),
+                       (const cplx<T>*)s->fft.ax[0].tw, (const T*)s->kx, (const T*)s->ky, (const T*)s->kz,
+                       (const T*)s->ik2, (const T*)s->kinv, (T)s->p.hydro_screening, (T)(double)s->ns);
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_spec_planes_inv(ss_sim* s) {
+    const int nout = s->use_e ? 12 : 6;
+    const size_t lds = spec_plane_lds(s->ny, s->nz, sizeof(T));
+    const dim3 grid((unsigned)(s->nx * nout));
+    hipLaunchKernelGGL(k_spec_planes_inv<T>, grid, dim3(256), lds, s->stream, (const cplx<T>*)s->spec_out, s->nx,
+                       spec_axis(s->fft.ax[1]), spec_axis(s->fft.ax[2]), (const cplx<T>*)s->fft.ax[1].tw,
+                       (const cplx<T>*)s->fft.ax[2].tw, (T*)s->rec, s->rs);
+    return hipGetLastError();
+}
+
 template <typename T, int NM, bool MOVE, bool EXT, bool KEEP>
 static void launch_tile_kernel(ss_sim* s, const TileArgs<T, NM>& a) {
     const int per = (s->tg.ntiles + s->tg.nxcd - 1) / s->tg.nxcd;
@@ -498,6 +557,49 @@ enum Stage { ST_BIN = 0, ST_FWD, ST_SOLVE, ST_INV, ST_PACK, ST_UPDATE, ST_SORT, 
 static const char* STAGE_NAMES[ST_COUNT] = {"bin", "fft_forward", "solve", "fft_inverse",
                                             "pack", "update", "sort"};
 
+// Moments -> cell records (bin/convert, forward FFT, solve, inverse FFT,
+// pack).  ev (may be null): events recorded after each stage.
+template <typename T>
+static int spectral_stage(ss_sim* s, hipEvent_t* ev) {
+    hipError_t e;
+    auto mark = [&](int st) {
+        if (ev) (void)hipEventRecord(ev[st + 1], s->stream);
+    };
+    if (s->fused) {
+        // stages: bin = moments -> z, y transforms; solve = x transforms and
+        // solve; fft_inverse = y, z inverse transforms into the records
+        if ((e = launch_spec_planes_fwd<T>(s)) != hipSuccess)
+            return fail(SS_ERR_DEVICE, std::string("spectral planes: ") + hipGetErrorString(e));
+        mark(ST_BIN);
+        mark(ST_FWD);
+        if ((e = launch_spec_pencils<T>(s)) != hipSuccess)
+            return fail(SS_ERR_DEVICE, std::string("spectral pencils: ") + hipGetErrorString(e));
+        mark(ST_SOLVE);
+        if ((e = launch_spec_planes_inv<T>(s)) != hipSuccess)
+            return fail(SS_ERR_DEVICE, std::string("spectral planes: ") + hipGetErrorString(e));
+        mark(ST_INV);
+        mark(ST_PACK);
+        s->spec_partial = true;
+    } else {
+        if ((e = launch_bin<T>(s)) != hipSuccess)
+            return fail(SS_ERR_DEVICE, std::string("bin: ") + hipGetErrorString(e));
+        mark(ST_BIN);
+        int rc = s->fft.run(s->spec_in, s->nm, -1, s->stream);
+        if (rc != SS_OK) return rc;
+        mark(ST_FWD);
+        if ((e = launch_solve<T>(s, false, s->spec_out)) != hipSuccess)
+            return fail(SS_ERR_DEVICE, std::string("solve: ") + hipGetErrorString(e));
+        mark(ST_SOLVE);
+        rc = s->fft.run(s->spec_out, s->nout, +1, s->stream);
+        if (rc != SS_OK) return rc;
+        mark(ST_INV);
+        if ((e = launch_pack<T>(s)) != hipSuccess)
+            return fail(SS_ERR_DEVICE, std::string("pack: ") + hipGetErrorString(e));
+        mark(ST_PACK);
+    }
+    return SS_OK;
+}
+
 // One do_timestep.  ev (may be null): events recorded after each stage.
 template <typename T>
 static int step_once(ss_sim* s, hipEvent_t* ev) {
@@ -507,19 +609,8 @@ static int step_once(ss_sim* s, hipEvent_t* ev) {
     };
     if (ev) (void)hipEventRecord(ev[0], s->stream);
     if (s->swap_hist) std::swap(s->hist, s->hist_next);
-    if ((e = launch_bin<T>(s)) != hipSuccess) return fail(SS_ERR_DEVICE, std::string("bin: ") + hipGetErrorString(e));
-    mark(ST_BIN);
-    int rc = s->fft.run(s->spec_in, s->nm, -1, s->stream);
+    int rc = spectral_stage<T>(s, ev);
     if (rc != SS_OK) return rc;
-    mark(ST_FWD);
-    if ((e = launch_solve<T>(s, false, s->spec_out)) != hipSuccess)
-        return fail(SS_ERR_DEVICE, std::string("solve: ") + hipGetErrorString(e));
-    mark(ST_SOLVE);
-    rc = s->fft.run(s->spec_out, s->nout, +1, s->stream);
-    if (rc != SS_OK) return rc;
-    mark(ST_INV);
-    if ((e = launch_pack<T>(s)) != hipSuccess) return fail(SS_ERR_DEVICE, std::string("pack: ") + hipGetErrorString(e));
-    mark(ST_PACK);
     if ((e = launch_update<T>(s)) != hipSuccess)
         return fail(SS_ERR_DEVICE, std::string("update: ") + hipGetErrorString(e));
     mark(ST_UPDATE);
@@ -590,6 +681,11 @@ static int export_particles(ss_sim* s, double* out, double* comp, uint64_t n) {
 // spec_full: u(3), G(9), [b(3), gradB(9)], inverse transformed.
 template <typename T>
 static int full_fields(ss_sim* s) {
+    if (s->spec_partial) {  // the fused step left the x transform of the moments undone
+        int rc = s->fft.run_axis(s->spec_in, s->nm, -1, 0, s->stream);
+        if (rc != SS_OK) return rc;
+        s->spec_partial = false;
+    }
     hipError_t e = launch_solve<T>(s, true, s->spec_full);
     if (e != hipSuccess) return fail(SS_ERR_DEVICE, hipGetErrorString(e));
     const int nfull = 12 + (s->mag ? 12 : 0);
@@ -665,6 +761,7 @@ static int run_steps(ss_sim* s, uint64_t nsteps) {
                     s->steps_since_sort = 0;
                 }
                 s->have_fields = true;
+                s->spec_partial = s->fused;
             }
             SS_HIP_CHECK(hipGraphLaunch(s->graph[key], s->stream));
             nsteps -= (uint64_t)K;
@@ -699,6 +796,36 @@ static void free_all(ss_sim* s) {
     s->fft.release();
     if (s->stream) (void)hipStreamDestroy(s->stream);
     s->stream = nullptr;
+}
+
+// Fused spectral stage when it applies: tiled path, no magnetic solve, FFT
+// radices 2-5, a y-z plane (two buffers) within the workgroup LDS limit (at
+// most 144 KB).  The pencil group is the largest power of two up to 16 whose
+// buffers fit in 64 KB.
+template <typename T>
+static int choose_spectral(ss_sim* s) {
+    s->fused = false;
+    if (!s->sorted || s->mag || (s->p.flags & SS_FLAG_AXIS_PASSES) || !s->fft.small_radices()) return SS_OK;
+    int lim = 0;
+    SS_HIP_CHECK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, s->p.device));
+    const size_t cap = std::min<size_t>((size_t)lim, 144 * 1024);
+    const size_t plane = spec_plane_lds(s->ny, s->nz, sizeof(T));
+    const int nmax = std::max(s->nsig, s->nout);
+    int pb = 16;
+    while (pb > 1 && spec_pencil_lds(s->nx, nmax * pb, sizeof(T)) > 64 * 1024) pb /= 2;
+    const size_t pencil = spec_pencil_lds(s->nx, nmax * pb, sizeof(T));
+    if (plane > cap || pencil > cap) return SS_OK;
+    SS_HIP_CHECK(hipFuncSetAttribute((const void*)k_spec_planes_fwd<T, 6>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane));
+    SS_HIP_CHECK(hipFuncSetAttribute((const void*)k_spec_planes_fwd<T, 9>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane));
+    SS_HIP_CHECK(hipFuncSetAttribute((const void*)k_spec_planes_inv<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)plane));
+    SS_HIP_CHECK(hipFuncSetAttribute((const void*)k_spec_pencils<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)pencil));
+    s->pencils = pb;
+    s->fused = true;
+    return SS_OK;
 }
 
 template <typename T>
@@ -811,7 +938,8 @@ static int create_impl(ss_sim* s) {
         if ((rc = alloc(s, (void**)&s->kfix, kf.size() * 8)) != SS_OK) return rc;
         SS_HIP_CHECK(hipMemcpy(s->kfix, kf.data(), kf.size() * 8, hipMemcpyHostToDevice));
     }
-    return s->fft.init(s->nx, s->ny, s->nz, s->prec);
+    if ((rc = s->fft.init(s->nx, s->ny, s->nz, s->prec)) != SS_OK) return rc;
+    return choose_spectral<T>(s);
 }
 
 // Tile geometry of the tiled path.  Unless given, tiles start at 4^3 cells and
@@ -1169,6 +1297,21 @@ static int diagnostics(ss_sim* s, double* out) {
     return SS_OK;
 }
 
+int ss_get_info(const ss_sim* s, ss_info* out) {
+    if (!s || !out) return fail(SS_ERR_INVALID, "NULL argument");
+    std::memset(out, 0, sizeof(*out));
+    out->sorted = s->sorted ? 1 : 0;
+    out->fused_spectral = s->fused ? 1 : 0;
+    if (s->sorted) {
+        for (int k = 0; k < 3; ++k) out->tile[k] = s->tg.t[k];
+        out->ntiles = s->tg.ntiles;
+        out->resort_interval = s->resort;
+    }
+    out->pencils = s->fused ? s->pencils : 0;
+    out->graphs = s->use_graph ? 1 : 0;
+    return SS_OK;
+}
+
 int ss_get_diagnostics(ss_sim* s, double* out) {
     if (!s || !out) return fail(SS_ERR_INVALID, "NULL argument");
     if (!s->initialised) return fail(SS_ERR_STATE, "not initialised");
@@ -1506,13 +1649,8 @@ static int ens_step_once(ss_ensemble* e, int k, bool capturing) {
     SS_HIP_CHECK(hipEventRecord(e->fork, e->stream));
     for (int q = 0; q < NSIDE; ++q) SS_HIP_CHECK(hipStreamWaitEvent(e->side[q], e->fork, 0));
     for (ss_sim* s : e->m) {
-        if ((err = launch_bin<T>(s)) != hipSuccess) return fail(SS_ERR_DEVICE, hipGetErrorString(err));
-        int rc = s->fft.run(s->spec_in, s->nm, -1, s->stream);
+        int rc = spectral_stage<T>(s, nullptr);
         if (rc != SS_OK) return rc;
-        if ((err = launch_solve<T>(s, false, s->spec_out)) != hipSuccess) return fail(SS_ERR_DEVICE, hipGetErrorString(err));
-        rc = s->fft.run(s->spec_out, s->nout, +1, s->stream);
-        if (rc != SS_OK) return rc;
-        if ((err = launch_pack<T>(s)) != hipSuccess) return fail(SS_ERR_DEVICE, hipGetErrorString(err));
     }
     for (int q = 0; q < NSIDE; ++q) {
         SS_HIP_CHECK(hipEventRecord(e->join[q], e->side[q]));
@@ -1604,6 +1742,7 @@ static int ens_run(ss_ensemble* e, uint64_t nsteps) {
                     s->pending = true;
                     s->steps_since_sort = 0;
                     s->have_fields = true;
+                    s->spec_partial = s->fused;
                 }
                 e->timestep += K;
             }

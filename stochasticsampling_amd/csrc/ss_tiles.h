@@ -69,8 +69,8 @@ struct TileArgs {
     uint32_t step_off;      // offset added to *step
     const T* noise;
     const long long* kfix;  // [NM][na] fixed-point moment kernels
-    long long* main;        // [ns][NM+1] moments of each tile's own cells
-    long long* inflow;      // [ns][NM+1] contributions from other tiles
+    long long* main;        // [NM+1][ns] moments of each tile's own cells (count last)
+    long long* inflow;      // [NM+1][ns] contributions from other tiles
     uint32_t* hist;         // optional [ns][na] counts (kept distribution)
     // re-sort fused into the update: when dtoff is set, every particle is
     // written to state d at a slot of the segment of the tile it occupied at
@@ -187,11 +187,12 @@ __device__ inline void tile_update_body(const TileArgs<T, NM>& a, int block, uns
             atomicAdd(reinterpret_cast<unsigned long long*>(dst + NM), 1ull);
         } else {
             const int cell = (ix * g.ny + iy) * g.nz + iz;
-            long long* dst = a.inflow + (size_t)cell * NC;
+            long long* dst = a.inflow + cell;
 #pragma unroll
             for (int q = 0; q < NM; ++q)
-                atomicAdd(reinterpret_cast<unsigned long long*>(dst + q), (unsigned long long)kf[q * g.na + ang]);
-            atomicAdd(reinterpret_cast<unsigned long long*>(dst + NM), 1ull);
+                atomicAdd(reinterpret_cast<unsigned long long*>(dst + (size_t)q * g.ns),
+                          (unsigned long long)kf[q * g.na + ang]);
+            atomicAdd(reinterpret_cast<unsigned long long*>(dst + (size_t)NM * g.ns), 1ull);
         }
         if (KEEP) {
             const int cell = (ix * g.ny + iy) * g.nz + iz;
@@ -215,11 +216,12 @@ __device__ inline void tile_update_body(const TileArgs<T, NM>& a, int block, uns
         const bool own = dx >= 0 && dx < ex && dy >= 0 && dy < ey && dz >= 0 && dz < ez;
         if (own) {
 #pragma unroll
-            for (int q = 0; q < NC; ++q) a.main[cell * NC + q] = src[q];
+            for (int q = 0; q < NC; ++q) a.main[(size_t)q * g.ns + cell] = src[q];
         } else if (src[NM] != 0) {
 #pragma unroll
             for (int q = 0; q < NC; ++q)
-                atomicAdd(reinterpret_cast<unsigned long long*>(a.inflow + cell * NC + q), (unsigned long long)src[q]);
+                atomicAdd(reinterpret_cast<unsigned long long*>(a.inflow + (size_t)q * g.ns + cell),
+                          (unsigned long long)src[q]);
         }
     }
 }
@@ -272,15 +274,14 @@ __global__ void __launch_bounds__(256)
                    double rho_scale, cplx<T>* __restrict__ spec, T* __restrict__ rho) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ns) return;
-    constexpr int NC = NM + 1;
 #pragma unroll
     for (int q = 0; q < NM; ++q) {
-        const long long v = main[(size_t)c * NC + q] + inflow[(size_t)c * NC + q];
+        const long long v = main[(size_t)q * ns + c] + inflow[(size_t)q * ns + c];
         spec[(size_t)q * ns + c] = cplx<T>{(T)((double)v * scale), T(0)};
-        inflow[(size_t)c * NC + q] = 0;
+        inflow[(size_t)q * ns + c] = 0;
     }
-    const long long cntv = main[(size_t)c * NC + NM] + inflow[(size_t)c * NC + NM];
-    inflow[(size_t)c * NC + NM] = 0;
+    const long long cntv = main[(size_t)NM * ns + c] + inflow[(size_t)NM * ns + c];
+    inflow[(size_t)NM * ns + c] = 0;
     if (rho) rho[c] = (T)((double)cntv * rho_scale);
 }
 
@@ -292,16 +293,16 @@ __global__ void __launch_bounds__(256)
                   int nz, TileGeo tg, uint32_t* __restrict__ tcnt) {
     const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (t >= tg.ntiles) return;
-    constexpr int NC = NM + 1;
     const int tz = t % tg.nt[2], ty = (t / tg.nt[2]) % tg.nt[1], tx = t / (tg.nt[2] * tg.nt[1]);
     const int ox = tx * tg.t[0], oy = ty * tg.t[1], oz = tz * tg.t[2];
     const int ex = min(tg.t[0], nx - ox), ey = min(tg.t[1], ny - oy), ez = min(tg.t[2], nz - oz);
     const int nc = ex * ey * ez;
+    const size_t cnt = (size_t)NM * nx * ny * nz;  // the count component
     long long sum = 0;
     for (int c = __lane_id(); c < nc; c += 64) {
         const int k = c % ez, j = (c / ez) % ey, i = c / (ez * ey);
         const size_t cell = (size_t)((ox + i) * ny + oy + j) * nz + oz + k;
-        sum += main[cell * NC + NM] + inflow[cell * NC + NM];
+        sum += main[cnt + cell] + inflow[cnt + cell];
     }
     for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
     if (__lane_id() == 0) tcnt[t] = (uint32_t)sum;

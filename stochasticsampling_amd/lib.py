@@ -7,7 +7,7 @@ if the library is missing, every entry point raises.
 import ctypes as C
 import os
 
-from .params import ERRORS, SS_OK, Params
+from .params import ERRORS, SS_OK, Info, Params
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libss.so")
@@ -48,6 +48,7 @@ def _declare(L):
         "ss_profile_steps": (C.c_int, [vp, C.c_uint64, _d, C.c_char_p, C.c_int, C.c_int]),
         "ss_synchronize": (C.c_int, [vp]),
         "ss_get_diagnostics": (C.c_int, [vp, _d]),
+        "ss_get_info": (C.c_int, [vp, _P(Info)]),
         "ss_ensemble_create": (C.c_int, [pp, C.c_int, _P(vp)]),
         "ss_ensemble_destroy": (C.c_int, [vp]),
         "ss_ensemble_member": (C.c_int, [vp, C.c_int, _P(vp)]),

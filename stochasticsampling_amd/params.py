@@ -20,6 +20,7 @@ FLAG_QUASI2D = 1
 FLAG_MAGNETIC_ALWAYS = 2
 FLAG_KEEP_DISTRIBUTION = 4
 FLAG_NO_GRAPH = 8
+FLAG_AXIS_PASSES = 16
 
 SS_OK = 0
 ERRORS = {
@@ -96,6 +97,20 @@ class Params(C.Structure):
     def dist_shape(self):
         g = self.grid_size
         return (g.x, g.y, g.z, g.phi, g.theta)
+
+
+class Info(C.Structure):
+    """ss_info: the execution plan chosen at ss_create."""
+
+    _fields_ = [
+        ("sorted", C.c_int32),
+        ("fused_spectral", C.c_int32),
+        ("tile", C.c_int32 * 3),
+        ("ntiles", C.c_int32),
+        ("resort_interval", C.c_int32),
+        ("pencils", C.c_int32),
+        ("graphs", C.c_int32),
+    ]
 
 
 def make_params(

@@ -13,7 +13,7 @@ import ctypes as C
 import numpy as np
 
 from . import lib as _l
-from .params import Params
+from .params import Info, Params
 
 _P = C.POINTER
 
@@ -161,6 +161,20 @@ class Simulation:
             "mean_orientation": out[0:3],
             "mean_polarisation": out[3:6],
             "mean_stress": out[6:15].reshape(3, 3),
+        }
+
+    def get_info(self):
+        """The execution plan chosen at creation (ss_get_info) as a dict."""
+        i = Info()
+        _l.check(self.L.ss_get_info(self._h, C.byref(i)))
+        return {
+            "sorted": bool(i.sorted),
+            "fused_spectral": bool(i.fused_spectral),
+            "tile": tuple(i.tile),
+            "ntiles": i.ntiles,
+            "resort_interval": i.resort_interval,
+            "pencils": i.pencils,
+            "graphs": bool(i.graphs),
         }
 
     def profile_steps(self, nsteps, max_stages=16):
