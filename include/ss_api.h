@@ -215,6 +215,7 @@ typedef struct ss_info {
     int32_t resort_interval; /* steps between re-sorts (tiled path)            */
     int32_t pencils;         /* (y, z) pencils per workgroup (fused spectral)  */
     int32_t graphs;          /* whole periods replayed from captured hipGraphs */
+    int32_t update_groups;   /* workgroups of the tile update (one per tile)   */
 } ss_info;
 int ss_get_info(const ss_sim* sim, ss_info* out);
 

@@ -151,6 +151,7 @@ struct ss_sim {
     bool fused = false;
     bool spec_partial = false;
     int pencils = 8;  // (y, z) pencils per workgroup of the pencil kernel
+    uint32_t* home_ctr = nullptr;  // re-sort: particles placed from a segment's front
     bool mag = false;  // magnetic solve runs
     bool use_e = false, use_rho = false, use_b = false, use_gb = false;
     int nsig = 6;
@@ -465,6 +466,7 @@ static TileArgs<T, NM> tile_args(ss_sim* s) {
     a.d = state_b<T>(s);
     a.dtoff = s->pending ? s->toff2 : nullptr;
     a.cross = s->cursor;
+    a.home = s->home_ctr;
     return a;
 }
 
@@ -525,6 +527,7 @@ static hipError_t launch_count_scan(ss_sim* s) {
     hipLaunchKernelGGL(k_scan_reduce, dim3(s->nparts), dim3(256), 0, s->stream, s->tcnt, nt, s->part);
     hipLaunchKernelGGL(k_scan_apply, dim3(s->nparts), dim3(256), 0, s->stream, s->tcnt, nt, s->part, s->nparts,
                        s->toff2, s->cursor);
+    fill_u32(s, s->home_ctr, (size_t)nt, 0u);
     s->pending = true;
     s->steps_since_sort = 0;
     return hipGetLastError();
@@ -863,6 +866,7 @@ static int create_impl(ss_sim* s) {
         A(s->toff, ((size_t)nt + 1) * 4);
         A(s->toff2, ((size_t)nt + 1) * 4);
         A(s->cursor, (size_t)nt * 4);
+        A(s->home_ctr, (size_t)nt * 4);
         s->nparts = (nt + SCAN_CHUNK - 1) / SCAN_CHUNK;
         A(s->part, (size_t)s->nparts * 4);
         A(s->mom_main, (size_t)s->ns * (s->nm + 1) * 8);
@@ -1309,6 +1313,7 @@ int ss_get_info(const ss_sim* s, ss_info* out) {
     }
     out->pencils = s->fused ? s->pencils : 0;
     out->graphs = s->use_graph ? 1 : 0;
+    if (s->sorted) out->update_groups = (s->tg.ntiles + s->tg.nxcd - 1) / s->tg.nxcd * s->tg.nxcd;
     return SS_OK;
 }
 

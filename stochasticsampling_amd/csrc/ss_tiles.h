@@ -78,36 +78,33 @@ struct TileArgs {
     State<T> d;
     const uint32_t* dtoff;  // [ntiles + 1] or nullptr (update in place)
     uint32_t* cross;        // [ntiles] particles placed from a segment's end
+    uint32_t* home;         // [ntiles] particles placed from a segment's front
 };
 
-// Dynamic LDS of the tile update: window accumulators, moment kernels and
-// the counter of a block's particles that stay in its tile.
+// Dynamic LDS of the tile update: window accumulators and moment kernels.
 inline size_t tile_lds_bytes(int ws, int nm, int na) {
-    return ((size_t)ws * (nm + 1) * 8 + (size_t)nm * na * 8 + 16 + 15) / 16 * 16;
+    return ((size_t)ws * (nm + 1) * 8 + (size_t)nm * na * 8 + 15) / 16 * 16;
 }
 
-// One tile of one system: move its particles (MOVE) and bin them into the
-// next step's moments.  MOVE = false bins the current state (initial
-// sampling).  `block` is the workgroup's index within the system's launch.
+// The particles in slots [q0, q1) of one tile: move them (MOVE) and bin them
+// into the next step's moments in the LDS window of the tile; MOVE = false
+// bins the current state (initial sampling).  `owner`: this pass stores the
+// tile's own cells (every tile has exactly one owning pass per step, which
+// also runs for a tile without particles); a pass that does not own the tile
+// adds its own cells to the inflow accumulators instead.  kf must hold the
+// fixed-point moment kernels; acc is the window.  Ends with a barrier.
 template <typename T, int NM, bool MOVE, bool EXT_NOISE, bool KEEP, bool SCATTER>
-__device__ inline void tile_update_body(const TileArgs<T, NM>& a, int block, unsigned char* smem_raw) {
+__device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0, uint32_t q1, bool owner,
+                                 long long* acc, const long long* kf) {
     constexpr int NC = NM + 1;
-    long long* acc = reinterpret_cast<long long*>(smem_raw);  // [ws][NC]
-    long long* kf = acc + (size_t)a.tg.ws * NC;               // [NM][na]
-    unsigned* home = reinterpret_cast<unsigned*>(kf + (size_t)NM * a.g.na);
     const Geo<T>& g = a.g;
     const TileGeo& tg = a.tg;
-    const int tile = tile_for_block(tg, block);
-    if (tile >= tg.ntiles) return;
     const int tz = tile % tg.nt[2], ty = (tile / tg.nt[2]) % tg.nt[1], tx = tile / (tg.nt[2] * tg.nt[1]);
     const int ox = tx * tg.t[0], oy = ty * tg.t[1], oz = tz * tg.t[2];
-
     for (int k = threadIdx.x; k < tg.ws * NC; k += blockDim.x) acc[k] = 0;
-    for (int k = threadIdx.x; k < NM * g.na; k += blockDim.x) kf[k] = a.kfix[k];
-    if (threadIdx.x == 0) *home = 0;
     __syncthreads();
 
-    const uint32_t p0 = a.toff[tile], p1 = a.toff[tile + 1];
+    const uint32_t p0 = q0, p1 = q1;
     const uint64_t step = MOVE ? *a.step + a.step_off : 0;
     constexpr bool scatter = MOVE && SCATTER;
     const uint32_t hbase = scatter ? a.dtoff[tile] : 0u;
@@ -152,7 +149,7 @@ __device__ inline void tile_update_body(const TileArgs<T, NM>& a, int block, uns
                 if (m != 0) {
                     const int leader = __ffsll((unsigned long long)m) - 1;
                     uint32_t base = 0;
-                    if (lane == leader) base = atomicAdd(home, (unsigned)__popcll(m));
+                    if (lane == leader) base = atomicAdd(a.home + tile, (unsigned)__popcll(m));
                     base = __shfl(base, leader);
                     if (h) o = hbase + base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
                 }
@@ -201,8 +198,9 @@ __device__ inline void tile_update_body(const TileArgs<T, NM>& a, int block, uns
     }
     __syncthreads();
 
-    // flush: own cells by plain stores (every cell of the grid is owned by
-    // exactly one tile), halo cells with atomics when they received particles
+    // flush: own cells by plain stores when this pass owns the tile (every
+    // cell of the grid is owned by exactly one tile, every tile by exactly
+    // one pass), other cells with atomics when they received particles
     const int ex = min(tg.t[0], g.nx - ox), ey = min(tg.t[1], g.ny - oy), ez = min(tg.t[2], g.nz - oz);
     for (int slot = threadIdx.x; slot < tg.ws; slot += blockDim.x) {
         const int lz = slot % tg.w[2], ly = (slot / tg.w[2]) % tg.w[1], lx = slot / (tg.w[2] * tg.w[1]);
@@ -214,7 +212,7 @@ __device__ inline void tile_update_body(const TileArgs<T, NM>& a, int block, uns
         const size_t cell = (size_t)(cx * g.ny + cy) * g.nz + cz;
         const long long* src = acc + (size_t)slot * NC;
         const bool own = dx >= 0 && dx < ex && dy >= 0 && dy < ey && dz >= 0 && dz < ez;
-        if (own) {
+        if (own && owner) {
 #pragma unroll
             for (int q = 0; q < NC; ++q) a.main[(size_t)q * g.ns + cell] = src[q];
         } else if (src[NM] != 0) {
@@ -224,12 +222,26 @@ __device__ inline void tile_update_body(const TileArgs<T, NM>& a, int block, uns
                           (unsigned long long)src[q]);
         }
     }
+    __syncthreads();
 }
 
+// Moment kernels into LDS (visible after the first barrier of tile_pass).
+template <int NM>
+__device__ inline void load_kf(const long long* __restrict__ kfix, int na, long long* kf) {
+    for (int k = threadIdx.x; k < NM * na; k += blockDim.x) kf[k] = kfix[k];
+}
+
+// One system: workgroup `blockIdx.x` moves and bins the particles of one
+// tile (tile_for_block).
 template <typename T, int NM, bool MOVE, bool EXT_NOISE, bool KEEP, bool SCATTER>
 __global__ void __launch_bounds__(256) k_tile_update(TileArgs<T, NM> a) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-    tile_update_body<T, NM, MOVE, EXT_NOISE, KEEP, SCATTER>(a, blockIdx.x, smem_raw);
+    const int tile = tile_for_block(a.tg, blockIdx.x);
+    if (tile >= a.tg.ntiles) return;
+    long long* acc = reinterpret_cast<long long*>(smem_raw);  // [ws][NM+1]
+    long long* kf = acc + (size_t)a.tg.ws * (NM + 1);         // [NM][na]
+    load_kf<NM>(a.kfix, a.g.na, kf);
+    tile_pass<T, NM, MOVE, EXT_NOISE, KEEP, SCATTER>(a, tile, a.toff[tile], a.toff[tile + 1], true, acc, kf);
 }
 
 // Many systems in one persistent launch: work item w of the concatenated
@@ -261,8 +273,14 @@ __global__ void __launch_bounds__(256)
         lo = __builtin_amdgcn_readfirstlane(lo);  // uniform: keeps the job in scalar registers
         TileArgs<T, NM> a = jobs[lo];
         a.step_off = step_off;
-        tile_update_body<T, NM, MOVE, EXT_NOISE, false, SCATTER>(a, w - sstart[lo], smem_raw);
-        __syncthreads();
+        const int tile = tile_for_block(a.tg, w - sstart[lo]);
+        if (tile < a.tg.ntiles) {
+            long long* acc = reinterpret_cast<long long*>(smem_raw);
+            long long* kf = acc + (size_t)a.tg.ws * (NM + 1);
+            load_kf<NM>(a.kfix, a.g.na, kf);
+            tile_pass<T, NM, MOVE, EXT_NOISE, false, SCATTER>(a, tile, a.toff[tile], a.toff[tile + 1], true, acc,
+                                                               kf);
+        }
     }
 }
 

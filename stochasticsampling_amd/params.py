@@ -110,6 +110,7 @@ class Info(C.Structure):
         ("resort_interval", C.c_int32),
         ("pencils", C.c_int32),
         ("graphs", C.c_int32),
+        ("update_groups", C.c_int32),
     ]
 
 

@@ -175,6 +175,7 @@ class Simulation:
             "resort_interval": i.resort_interval,
             "pencils": i.pencils,
             "graphs": bool(i.graphs),
+            "update_groups": i.update_groups,
         }
 
     def profile_steps(self, nsteps, max_stages=16):
