@@ -386,11 +386,11 @@ static hipError_t launch_spec_planes_fwd(ss_sim* s) {
     const cplx<T>* twy = (const cplx<T>*)s->fft.ax[1].tw;
     const cplx<T>* twz = (const cplx<T>*)s->fft.ax[2].tw;
     if (s->nm == 6)
-        hipLaunchKernelGGL((k_spec_planes_fwd<T, 6>), grid, dim3(256), lds, s->stream, s->mom_main, s->mom_inflow,
+        hipLaunchKernelGGL((k_spec_planes_fwd<T, 6>), grid, dim3(SPEC_PLANE_THREADS), lds, s->stream, s->mom_main, s->mom_inflow,
                            s->nx, s->fixed_scale * ms, ds, ay, az, twy, twz, (cplx<T>*)s->spec_in, rho, (T*)s->rec,
                            s->rs);
     else
-        hipLaunchKernelGGL((k_spec_planes_fwd<T, 9>), grid, dim3(256), lds, s->stream, s->mom_main, s->mom_inflow,
+        hipLaunchKernelGGL((k_spec_planes_fwd<T, 9>), grid, dim3(SPEC_PLANE_THREADS), lds, s->stream, s->mom_main, s->mom_inflow,
                            s->nx, s->fixed_scale * ms, ds, ay, az, twy, twz, (cplx<T>*)s->spec_in, rho, (T*)s->rec,
                            s->rs);
     return hipGetLastError();
@@ -402,7 +402,7 @@ static hipError_t launch_spec_pencils(ss_sim* s) {
     const int nout = s->use_e ? 12 : 6;
     const size_t lds = spec_pencil_lds(s->nx, std::max(s->nsig, nout) * s->pencils, sizeof(T));
     const dim3 grid((unsigned)((np + s->pencils - 1) / s->pencils));
-    hipLaunchKernelGGL(k_spec_pencils<T>, grid, dim3(256), lds, s->stream, (const cplx<T>*)s->spec_in,
+    hipLaunchKernelGGL(k_spec_pencils<T>, grid, dim3(SPEC_PENCIL_THREADS), lds, s->stream, (const cplx<T>*)s->spec_in,
                        (cplx<T>*)s->spec_out, np, s->nz, s->pencils, s->nsig, nout, spec_axis(s->fft.ax[0]),
                        (const cplx<T>*)s->fft.ax[0].tw, (const T*)s->kx, (const T*)s->ky, (const T*)s->kz,
                        (const T*)s->ik2, (const T*)s->kinv, (T)s->p.hydro_screening, (T)(double)s->ns);
@@ -414,7 +414,7 @@ static hipError_t launch_spec_planes_inv(ss_sim* s) {
     const int nout = s->use_e ? 12 : 6;
     const size_t lds = spec_plane_lds(s->ny, s->nz, sizeof(T));
     const dim3 grid((unsigned)(s->nx * nout));
-    hipLaunchKernelGGL(k_spec_planes_inv<T>, grid, dim3(256), lds, s->stream, (const cplx<T>*)s->spec_out, s->nx,
+    hipLaunchKernelGGL(k_spec_planes_inv<T>, grid, dim3(SPEC_PLANE_THREADS), lds, s->stream, (const cplx<T>*)s->spec_out, s->nx,
                        spec_axis(s->fft.ax[1]), spec_axis(s->fft.ax[2]), (const cplx<T>*)s->fft.ax[1].tw,
                        (const cplx<T>*)s->fft.ax[2].tw, (T*)s->rec, s->rs);
     return hipGetLastError();

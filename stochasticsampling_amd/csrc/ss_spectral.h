@@ -19,13 +19,19 @@
 
 namespace ss {
 
+// Workgroup sizes: the plane kernels keep one plane per workgroup, the
+// pencil kernel a group of pencils; more threads per workgroup give each
+// workgroup's LDS-bound stages more waves to overlap.
+constexpr int SPEC_PLANE_THREADS = 512;
+constexpr int SPEC_PENCIL_THREADS = 512;
+
 // LDS of one plane kernel: twiddles of y and z, two plane buffers.
 inline size_t spec_plane_lds(int ny, int nz, size_t esz) { return esz * 2 * ((size_t)ny + nz + 2 * (size_t)ny * nz); }
 
 // Forward: component q of plane x.  Blocks with q == 0 also convert the count
 // component (density, when wanted).  Clears the inflow entries it reads.
 template <typename T, int NM>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(SPEC_PLANE_THREADS)
     k_spec_planes_fwd(const long long* __restrict__ main, long long* __restrict__ inflow, int nx, double scale,
                       double rho_scale, SpecAxis ay, SpecAxis az, const cplx<T>* __restrict__ twy,
                       const cplx<T>* __restrict__ twz, cplx<T>* __restrict__ spec, T* __restrict__ rho,
@@ -78,7 +84,7 @@ inline size_t spec_pencil_lds(int nx, int nl, size_t esz) {
 // x transform, flow solve, inverse x transform of the nout = 6 (u, W) or 12
 // (u, W, E) outputs.
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(SPEC_PENCIL_THREADS)
     k_spec_pencils(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int np, int nz, int pb_max,
                    int nsig, int nout, SpecAxis ax, const cplx<T>* __restrict__ twx, const T* __restrict__ kx,
                    const T* __restrict__ ky, const T* __restrict__ kz, const T* __restrict__ ik2,
@@ -134,7 +140,7 @@ __global__ void __launch_bounds__(256)
 // Inverse: field component o of plane x; the real part goes to record slot
 // o (u, W) or REC_E + o - 6 (E).
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(SPEC_PLANE_THREADS)
     k_spec_planes_inv(const cplx<T>* __restrict__ spec, int nx, SpecAxis ay, SpecAxis az,
                       const cplx<T>* __restrict__ twy, const cplx<T>* __restrict__ twz, T* __restrict__ rec,
                       int rs) {
