@@ -53,7 +53,10 @@ def time_gpu(prec, parts, steps, warmup, profile_steps):
     sim.step(steps)
     sim.synchronize()
     t1 = time.perf_counter()
+    # per-stage HIP-event timing on the simulation's stream, right after the
+    # timed steps on the same state
     stages = sim.profile_steps(profile_steps) if profile_steps else {}
+    stages["plan"] = sim.get_info()
     parts_out = sim.get_particles_head(4)
     assert np.all(np.isfinite(parts_out))
     sim.close()
@@ -204,7 +207,7 @@ def main():
             traffic = json.load(f).get("traffic_bytes_per_launch")
     achieved = UPDATE_BYTES["f32"] * n / (upd_ms * 1e-3) / 1e9
     roofline = {
-        "kernel": "ss::k_tile_update<float, 6, true, false, false>",
+        "kernel": "ss::k_tile_update<float, 6, true, false, false, false>",
         "bound": "hbm",
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
