@@ -5,6 +5,7 @@ from .params import (  # noqa: F401
     BIN_HISTOGRAM,
     BIN_SORTED,
     FFT_AUTO,
+    FFT_DFT,
     FFT_RADIX,
     FLAG_AXIS_PASSES,
     FLAG_KEEP_DISTRIBUTION,

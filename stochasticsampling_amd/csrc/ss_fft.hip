@@ -149,11 +149,166 @@ __global__ void __launch_bounds__(256) k_fft_axis(cplx<T>* __restrict__ data, Ax
     }
 }
 
-int Fft3d::init(int nx_, int ny_, int nz_, int precision_) {
+// ---------------------------------------------------------------------------
+// Dense-DFT variant for axes of at most 64 points: each line is a complex
+// vector x of n points, stored as the real 2n-vector [Re x; Im x], and the
+// transform is the real matrix product [Re X; Im X] = M [Re x; Im x] with
+// M = [[C, -s S], [s S, C]], C = cos(2 pi jk / n), S = sin(2 pi jk / n).  It
+// runs on the matrix cores with v_mfma_f32_32x32x8bf16_1k in three passes,
+// M_hi x_hi + M_hi x_lo + M_lo x_hi, where hi = bf16(v) and lo = bf16(v - hi),
+// accumulated in FP32 (about 16 significant bits per product).  Lane layout
+// of the instruction (checked on gfx950): A[i][k] and B[k][j] hold
+// i (or j) = lane % 32, k = 4 (lane / 32) + t; D[i][j] in register v holds
+// j = lane % 32, i = 8 (v / 4) + 4 (lane / 32) + v % 4.
+// ---------------------------------------------------------------------------
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int DFT_LINES = 32;  // lines (matrix columns) per workgroup
+
+__device__ inline short bf16_rne(float x) {
+    uint32_t b = __float_as_uint(x);
+    b += 0x7FFFu + ((b >> 16) & 1u);
+    return (short)(b >> 16);
+}
+__device__ inline float bf16_value(short h) { return __uint_as_float(((uint32_t)(uint16_t)h) << 16); }
+
+// One tile of DFT_LINES lines; four wavefronts, wavefront w computes output
+// rows 32 w .. 32 w + 31 (rows <= 128).
+__global__ void __launch_bounds__(256) k_dft_axis(cplx<float>* __restrict__ data, AxisDesc d,
+                                                  const bf16x4* __restrict__ mh, const bf16x4* __restrict__ ml,
+                                                  int rows, int kdim) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    float* xs = reinterpret_cast<float*>(smem_raw);  // [DFT_LINES][kdim + 4]
+    const int n = d.n, ldx = kdim + 4;
+    const int outer = blockIdx.x / d.tiles;
+    const int tile = blockIdx.x - outer * d.tiles;
+    const int l0 = tile * DFT_LINES;
+    const int nl = min(DFT_LINES, d.lines - l0);
+    cplx<float>* base = data + (long long)outer * d.outer_stride + (long long)l0 * d.line_stride;
+    for (int f = threadIdx.x; f < DFT_LINES * n; f += blockDim.x) {
+        int l, i;
+        if (d.contiguous) {
+            l = f / n;
+            i = f - l * n;
+        } else {
+            i = f / DFT_LINES;
+            l = f - i * DFT_LINES;
+        }
+        cplx<float> v = {0.f, 0.f};
+        if (l < nl) v = base[(long long)l * d.line_stride + (long long)i * d.elem_stride];
+        xs[l * ldx + i] = v.x;
+        xs[l * ldx + n + i] = v.y;
+    }
+    for (int f = threadIdx.x; f < DFT_LINES * (kdim - 2 * n); f += blockDim.x) {
+        const int l = f / (kdim - 2 * n), e = 2 * n + f % (kdim - 2 * n);
+        xs[l * ldx + e] = 0.f;
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ksteps = kdim / 8;
+    f32x16 acc = {};
+    const bool active = wave * 32 < rows;
+    if (active) {
+        for (int ks = 0; ks < ksteps; ++ks) {
+            const bf16x4 ah = mh[(wave * ksteps + ks) * 64 + lane];
+            const bf16x4 al = ml[(wave * ksteps + ks) * 64 + lane];
+            const float* xp = xs + (lane & 31) * ldx + ks * 8 + 4 * (lane >> 5);
+            bf16x4 bh, bl;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const float x = xp[t];
+                const short h = bf16_rne(x);
+                bh[t] = h;
+                bl[t] = bf16_rne(x - bf16_value(h));
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, bh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, bl, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(al, bh, acc, 0, 0, 0);
+        }
+    }
+    __syncthreads();  // every wavefront has read its inputs
+    if (active) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int i = wave * 32 + (v / 4) * 8 + (lane >> 5) * 4 + (v % 4);
+            if (i < 2 * n) xs[(lane & 31) * ldx + i] = acc[v];
+        }
+    }
+    __syncthreads();
+    for (int f = threadIdx.x; f < DFT_LINES * n; f += blockDim.x) {
+        int l, i;
+        if (d.contiguous) {
+            l = f / n;
+            i = f - l * n;
+        } else {
+            i = f / DFT_LINES;
+            l = f - i * DFT_LINES;
+        }
+        if (l < nl)
+            base[(long long)l * d.line_stride + (long long)i * d.elem_stride] =
+                cplx<float>{xs[l * ldx + i], xs[l * ldx + n + i]};
+    }
+}
+
+static uint16_t bf16_bits_rne(float x) {
+    uint32_t b;
+    std::memcpy(&b, &x, 4);
+    b += 0x7FFFu + ((b >> 16) & 1u);
+    return (uint16_t)(b >> 16);
+}
+static double bf16_double(uint16_t h) {
+    const uint32_t b = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &b, 4);
+    return (double)f;
+}
+
+// DFT matrices of one axis for both signs, split into bf16 high and low parts
+// in fragment order [row tile][k step][lane][4].
+static int build_dft(FftAxis& a) {
+    const int n = a.n;
+    a.dft_rows = (2 * n + 31) / 32 * 32;
+    a.dft_k = (2 * n + 7) / 8 * 8;
+    const int rt = a.dft_rows / 32, ks = a.dft_k / 8;
+    const size_t cnt = (size_t)rt * ks * 64 * 4;
+    for (int sg = 0; sg < 2; ++sg) {
+        const double s = sg == 0 ? -1.0 : 1.0;
+        std::vector<uint16_t> hi(cnt), lo(cnt);
+        auto m = [&](int i, int k) -> double {
+            if (i >= 2 * n || k >= 2 * n) return 0.0;
+            const int e = i % n, j = k % n;
+            const double ang = 2.0 * M_PI * (double)((long long)e * j % n) / (double)n;
+            const bool im_out = i >= n, im_in = k >= n;
+            if (im_out == im_in) return cos(ang);
+            return im_out ? s * sin(ang) : -s * sin(ang);
+        };
+        for (int r = 0; r < rt; ++r)
+            for (int q = 0; q < ks; ++q)
+                for (int l = 0; l < 64; ++l)
+                    for (int t = 0; t < 4; ++t) {
+                        const double v = m(r * 32 + l % 32, q * 8 + 4 * (l / 32) + t);
+                        const uint16_t h = bf16_bits_rne((float)v);
+                        const size_t idx = (((size_t)r * ks + q) * 64 + l) * 4 + t;
+                        hi[idx] = h;
+                        lo[idx] = bf16_bits_rne((float)(v - bf16_double(h)));
+                    }
+        SS_HIP_CHECK(hipMalloc(&a.dft[sg][0], cnt * 2));
+        SS_HIP_CHECK(hipMalloc(&a.dft[sg][1], cnt * 2));
+        SS_HIP_CHECK(hipMemcpy(a.dft[sg][0], hi.data(), cnt * 2, hipMemcpyHostToDevice));
+        SS_HIP_CHECK(hipMemcpy(a.dft[sg][1], lo.data(), cnt * 2, hipMemcpyHostToDevice));
+    }
+    return SS_OK;
+}
+
+int Fft3d::init(int nx_, int ny_, int nz_, int precision_, int variant_) {
     nx = nx_;
     ny = ny_;
     nz = nz_;
     precision = precision_;
+    variant = variant_ == SS_FFT_DFT ? SS_FFT_DFT : SS_FFT_RADIX;
+    if (variant == SS_FFT_DFT && precision != SS_FP32)
+        return fail(SS_ERR_UNSUPPORTED, "the dense-DFT FFT variant is FP32 only");
     int dims[3] = {nx, ny, nz};
     for (int a = 0; a < 3; ++a) {
         int n = dims[a];
@@ -179,21 +334,75 @@ int Fft3d::init(int nx_, int ny_, int nz_, int precision_) {
         }
         SS_HIP_CHECK(hipMalloc(&ax[a].tw, esz * n));
         SS_HIP_CHECK(hipMemcpy(ax[a].tw, host.data(), esz * n, hipMemcpyHostToDevice));
+        // axes of 2..64 points run as dense DFTs in the DFT variant, longer
+        // ones keep the radix stages
+        if (variant == SS_FFT_DFT && n >= 2 && n <= 64) {
+            const int rc = build_dft(ax[a]);
+            if (rc != SS_OK) return rc;
+        }
     }
     return SS_OK;
 }
 
 void Fft3d::release() {
-    for (int a = 0; a < 3; ++a)
+    for (int a = 0; a < 3; ++a) {
         if (ax[a].tw) {
             (void)hipFree(ax[a].tw);
             ax[a].tw = nullptr;
         }
+        for (int sg = 0; sg < 2; ++sg)
+            for (int h = 0; h < 2; ++h)
+                if (ax[a].dft[sg][h]) {
+                    (void)hipFree(ax[a].dft[sg][h]);
+                    ax[a].dft[sg][h] = nullptr;
+                }
+    }
+}
+
+// One axis as dense DFTs on the matrix cores (FP32, 2..64 points).
+static int run_dft_axis(cplx<float>* data, const FftAxis& fa, int axis, int nx, int ny, int nz, int batch,
+                        int sign, hipStream_t stream) {
+    AxisDesc d;
+    std::memset(&d, 0, sizeof(d));
+    d.n = fa.n;
+    d.tl = DFT_LINES;
+    const long long nyz = (long long)ny * nz;
+    long long outers = 1;
+    if (axis == 2) {
+        d.contiguous = 1;
+        d.lines = (int)((long long)batch * nx * ny);
+        d.line_stride = nz;
+        d.elem_stride = 1;
+    } else if (axis == 1) {
+        d.lines = nz;
+        d.outer_stride = nyz;
+        d.line_stride = 1;
+        d.elem_stride = nz;
+        outers = (long long)batch * nx;
+    } else {
+        d.lines = (int)nyz;
+        d.outer_stride = (long long)nx * nyz;
+        d.line_stride = 1;
+        d.elem_stride = nyz;
+        outers = batch;
+    }
+    d.tiles = (d.lines + DFT_LINES - 1) / DFT_LINES;
+    const int sg = sign < 0 ? 0 : 1;
+    const size_t lds = sizeof(float) * DFT_LINES * (fa.dft_k + 4);
+    hipLaunchKernelGGL(k_dft_axis, dim3((unsigned)(outers * d.tiles)), dim3(256), lds, stream, data, d,
+                       reinterpret_cast<const bf16x4*>(fa.dft[sg][0]), reinterpret_cast<const bf16x4*>(fa.dft[sg][1]),
+                       fa.dft_rows, fa.dft_k);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(SS_ERR_DEVICE, std::string("dft launch: ") + hipGetErrorString(e));
+    return SS_OK;
 }
 
 template <typename T>
 static int run_axis(cplx<T>* data, const FftAxis& fa, int axis, int nx, int ny, int nz, int batch,
                     int sign, hipStream_t stream) {
+    if constexpr (sizeof(T) == 4) {
+        if (fa.dft[0][0]) return run_dft_axis(data, fa, axis, nx, ny, nz, batch, sign, stream);
+    }
     AxisDesc d;
     std::memset(&d, 0, sizeof(d));
     d.n = fa.n;
@@ -296,8 +505,8 @@ extern "C" int ss_fft_plan_create(int nx, int ny, int nz, int direction, int bat
     if (direction != -1 && direction != 1) return ss::fail(SS_ERR_INVALID, "direction must be -1 or +1");
     if (batch < 1) return ss::fail(SS_ERR_INVALID, "batch must be >= 1");
     if (precision != SS_FP64 && precision != SS_FP32) return ss::fail(SS_ERR_INVALID, "bad precision");
-    if (variant != SS_FFT_AUTO && variant != SS_FFT_RADIX)
-        return ss::fail(SS_ERR_UNSUPPORTED, "only the radix FFT variant is built");
+    if (variant != SS_FFT_AUTO && variant != SS_FFT_RADIX && variant != SS_FFT_DFT)
+        return ss::fail(SS_ERR_INVALID, "bad FFT variant");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return ss::fail(SS_ERR_DEVICE, "no GPU");
     if (device < 0 || device >= ndev) return ss::fail(SS_ERR_INVALID, "bad device ordinal");
@@ -306,7 +515,7 @@ extern "C" int ss_fft_plan_create(int nx, int ny, int nz, int direction, int bat
     p->direction = direction;
     p->batch = batch;
     p->device = device;
-    int rc = p->fft.init(nx, ny, nz, precision);
+    int rc = p->fft.init(nx, ny, nz, precision, variant);
     if (rc != SS_OK) {
         p->fft.release();
         delete p;

@@ -32,13 +32,19 @@ struct FftAxis {
     int nf = 0;
     int fac[16] = {0};
     void* tw = nullptr;  // n complex twiddles exp(-2 pi i t / n) in the plan precision
+    // dense-DFT variant: the real 2n x 2n DFT matrix of each sign split into
+    // bf16 high and low parts, in MFMA fragment order ([sign][hi/lo])
+    void* dft[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    int dft_rows = 0;  // 2n rounded up to 32 (output rows)
+    int dft_k = 0;     // 2n rounded up to 8 (reduction length)
 };
 
 struct Fft3d {
     int nx = 0, ny = 0, nz = 0;
     int precision = SS_FP64;
+    int variant = SS_FFT_RADIX;  // SS_FFT_RADIX or SS_FFT_DFT (FP32, axes of at most 64 points)
     FftAxis ax[3];
-    int init(int nx, int ny, int nz, int precision);
+    int init(int nx, int ny, int nz, int precision, int variant = SS_FFT_RADIX);
     void release();
     // In-place, unnormalised; sign -1 forward, +1 backward.  The forward
     // transform runs the axes z, y, x, the backward one x, y, z.

@@ -808,7 +808,9 @@ static void free_all(ss_sim* s) {
 template <typename T>
 static int choose_spectral(ss_sim* s) {
     s->fused = false;
-    if (!s->sorted || s->mag || (s->p.flags & SS_FLAG_AXIS_PASSES) || !s->fft.small_radices()) return SS_OK;
+    if (!s->sorted || s->mag || (s->p.flags & SS_FLAG_AXIS_PASSES) || s->fft.variant != SS_FFT_RADIX ||
+        !s->fft.small_radices())
+        return SS_OK;
     int lim = 0;
     SS_HIP_CHECK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, s->p.device));
     const size_t cap = std::min<size_t>((size_t)lim, 144 * 1024);
@@ -942,7 +944,7 @@ static int create_impl(ss_sim* s) {
         if ((rc = alloc(s, (void**)&s->kfix, kf.size() * 8)) != SS_OK) return rc;
         SS_HIP_CHECK(hipMemcpy(s->kfix, kf.data(), kf.size() * 8, hipMemcpyHostToDevice));
     }
-    if ((rc = s->fft.init(s->nx, s->ny, s->nz, s->prec)) != SS_OK) return rc;
+    if ((rc = s->fft.init(s->nx, s->ny, s->nz, s->prec, p.fft_variant)) != SS_OK) return rc;
     return choose_spectral<T>(s);
 }
 
@@ -1007,8 +1009,10 @@ static int validate(const ss_params* p) {
     if (p->noise_mode != SS_NOISE_PHILOX && p->noise_mode != SS_NOISE_EXTERNAL)
         return fail(SS_ERR_INVALID, "bad noise_mode");
     if (p->binning < SS_BIN_AUTO || p->binning > SS_BIN_SORTED) return fail(SS_ERR_INVALID, "bad binning");
-    if (p->fft_variant != SS_FFT_AUTO && p->fft_variant != SS_FFT_RADIX)
-        return fail(SS_ERR_UNSUPPORTED, "only the radix FFT variant is built");
+    if (p->fft_variant != SS_FFT_AUTO && p->fft_variant != SS_FFT_RADIX && p->fft_variant != SS_FFT_DFT)
+        return fail(SS_ERR_INVALID, "bad fft_variant");
+    if (p->fft_variant == SS_FFT_DFT && p->precision != SS_FP32)
+        return fail(SS_ERR_UNSUPPORTED, "the dense-DFT FFT variant is FP32 only");
     const double ns = (double)g.x * g.y * g.z, na = (double)g.phi * g.theta;
     if (ns * na >= 4294967295.0) return fail(SS_ERR_UNSUPPORTED, "grid has 2^32 or more bins");
     if (na > 65535) return fail(SS_ERR_UNSUPPORTED, "at most 65535 angle bins");

@@ -280,9 +280,12 @@ def mean_magnetic_field(params, dist):
 
 class FFTPlan:
     """FFTPlan (crates/fftw3/src/fft.rs:33-244): unnormalised in-place c2c 3-D,
-    ``direction`` -1 forward / +1 backward, batched over leading arrays."""
+    ``direction`` -1 forward / +1 backward, batched over leading arrays.
+    ``variant``: FFT_AUTO / FFT_RADIX (LDS Stockham) or FFT_DFT (FP32 only:
+    axes of 2..64 points as dense DFT products on the matrix cores, 3-pass
+    split bf16 with FP32 accumulation)."""
 
-    def __init__(self, shape, direction, batch=1, precision=0, device=0):
+    def __init__(self, shape, direction, batch=1, precision=0, device=0, variant=0):
         self.L = _l.load()
         self.shape = tuple(shape)
         self.batch = batch
@@ -290,7 +293,7 @@ class FFTPlan:
         h = C.c_void_p()
         nx, ny, nz = self.shape
         _l.check(
-            self.L.ss_fft_plan_create(nx, ny, nz, direction, batch, precision, 0, device, C.byref(h))
+            self.L.ss_fft_plan_create(nx, ny, nz, direction, batch, precision, variant, device, C.byref(h))
         )
         self._h = h
 

@@ -39,6 +39,29 @@ def test_fft_plan_matches_numpy(shape, precision, direction):
     assert rel_err(a, ref) < tol
 
 
+@pytest.mark.parametrize("shape", [(16, 16, 16), (64, 64, 64), (32, 48, 6), (8, 12, 10), (96, 16, 7)])
+@pytest.mark.parametrize("direction", [-1, 1])
+def test_fft_plan_dense_dft_variant(shape, direction):
+    """Dense DFT products on the matrix cores (3-pass split bf16, FP32
+    accumulation) for axes of at most 64 points, radix stages for longer axes:
+    the 3-pass split keeps about 16 significant bits per product, so the bound
+    is 1e-4 of the largest output."""
+    rng = np.random.default_rng(2)
+    batch = 2
+    x = rng.standard_normal((batch,) + shape) + 1j * rng.standard_normal((batch,) + shape)
+    a = np.ascontiguousarray(x.astype(np.complex64))
+    plan = ss.FFTPlan(shape, direction, batch=batch, precision=ss.FP32, variant=ss.FFT_DFT)
+    plan.reexecute3d(a)
+    axes = (1, 2, 3)
+    ref = np.fft.fftn(x, axes=axes) if direction == -1 else np.fft.ifftn(x, axes=axes) * np.prod(shape)
+    assert rel_err(a, ref) < 1e-4
+
+
+def test_fft_plan_dense_dft_variant_is_fp32_only():
+    with pytest.raises(ss.SSError):
+        ss.FFTPlan((16, 16, 16), -1, precision=ss.FP64, variant=ss.FFT_DFT)
+
+
 # ---------------------------------------------------------------------------
 # Binning rule (Distribution::coord_to_grid, distribution_test.rs:119-184)
 # ---------------------------------------------------------------------------
@@ -423,6 +446,28 @@ def test_fp32_fields_within_target_of_fp64(oracle):
     a, b = sim.get_particles(), osim.particles()
     assert np.median(periodic_diff(a[:, :3], b[:, :3], 10.0)) < 1e-6
     assert np.percentile(periodic_diff(a[:, :3], b[:, :3], 10.0), 99) < 1e-5
+
+
+def test_fp32_dense_dft_fields_against_fp64(oracle):
+    """The FP32 step with the dense-DFT FFT variant against the FP64 oracle:
+    u and grad u within 2e-4 relative (the radix variant meets 1e-5, so the
+    automatic choice keeps the radix stages)."""
+    kw = dict(grid=(16, 16, 16, 8, 8), n=1 << 15)
+    p32 = small_params(precision=ss.FP32, fft_variant=ss.FFT_DFT, flags=ss.FLAG_KEEP_DISTRIBUTION, **kw)
+    p64 = small_params(precision=ss.FP64, **kw)
+    parts = ss.initial_condition(p64, amplitude=0.05, seed=4)
+    sim = ss.Simulation(p32)
+    assert not sim.get_info()["fused_spectral"]
+    sim.init(parts)
+    sim.step(1)
+    osim = oracle.simulation(p64)
+    osim.init(parts)
+    osim.step(1)
+    u, g = sim.get_flow_field(with_gradient=True)
+    ue, ge = osim.flow_field()
+    if np.array_equal(sim.get_counts(), osim.counts()):
+        assert rel_err(u, ue) < 2e-4
+        assert rel_err(g, ge) < 2e-4
 
 
 def test_snapshot_resume_is_exact():
