@@ -28,10 +28,10 @@ HBM_PEAK_GBS = 8000.0
 UPDATE_BYTES = {"f32": 64.0, "f64": 80.0}
 
 
-def make(prec, n, binning=None):
+def make(prec, n, binning=None, **extra):
     import stochasticsampling_amd as ss
 
-    kw = {}
+    kw = dict(extra)
     if binning is not None:
         kw["binning"] = binning
     return ss.make_params(
@@ -62,6 +62,47 @@ def time_gpu(prec, parts, steps, warmup, profile_steps):
     sim.close()
     ms = (t1 - t0) * 1e3 / steps
     return ms, stages
+
+
+def fft_variant_study(parts, steps):
+    """The spectral-stage variants of the FP32 step on the same workload:
+    fused radix kernels (the default), radix axis passes, and dense DFT
+    products on the matrix cores (3-pass split bf16) in axis passes.  Timing:
+    mean ms per step of each stage over `steps` steps (HIP events).  Accuracy:
+    u and grad u of the spectral solve (SpectralSolver::mean_flow_field) in
+    FP32 with the variant's transforms against the FP64 solve, both from the
+    same distribution (the initial condition binned on the GPU in FP64),
+    max |a - b| / max |b|."""
+    import stochasticsampling_amd as ss
+
+    n = parts.shape[0]
+    ref = ss.Simulation(make(ss.FP64, n))
+    ref.init(parts)
+    dist = ref.get_distribution()
+    ref.close()
+    p64 = make(ss.FP64, n)
+    u64, g64 = ss.mean_flow_field(p64, dist)
+    out = {}
+    for name, kw in [("radix_fused", {}), ("radix_axis_passes", {"flags": ss.FLAG_AXIS_PASSES}),
+                     ("dft_mfma_axis_passes", {"fft_variant": ss.FFT_DFT})]:
+        p = make(ss.FP32, n, **kw)
+        u, g = ss.mean_flow_field(p, dist)
+        eu = float(np.max(np.abs(u - u64)) / np.max(np.abs(u64)))
+        eg = float(np.max(np.abs(g - g64)) / np.max(np.abs(g64)))
+        sim = ss.Simulation(p)
+        sim.init(parts)
+        sim.step(1)
+        stages = sim.profile_steps(steps)
+        sim.close()
+        spectral = sum(v for k, v in stages.items() if k not in ("update", "sort"))
+        out[name] = {"ms_per_step": sum(stages.values()), "spectral_ms": spectral, "stages_ms": stages,
+                     "u_rel_err_vs_fp64": eu, "grad_u_rel_err_vs_fp64": eg}
+    del dist
+    ok = {k: v for k, v in out.items() if v["u_rel_err_vs_fp64"] <= 1e-5 and v["grad_u_rel_err_vs_fp64"] <= 1e-5}
+    out["kept"] = min(ok, key=lambda k: ok[k]["spectral_ms"]) if ok else None
+    out["gate"] = ("u and grad u of the FP32 solve within 1e-5 of the FP64 solve on the same distribution; "
+                   "the fastest spectral stage among the variants passing is kept")
+    return out
 
 
 def cpu_baseline(parts, threads, steps):
@@ -163,6 +204,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true", help="only run a 1 GiB device copy")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--no-fft-study", action="store_true")
+    ap.add_argument("--fft-study-steps", type=int, default=32)
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -191,6 +234,11 @@ def main():
                               min(args.profile_steps, 10))
         fp64 = {"value": n / (ms64 * 1e-3), "ms_per_step": ms64, "steps": max(1, args.steps // 4),
                 "dtype": "f64", "binning": "histogram", "stages_ms": st64}
+
+    fftv = None
+    if not args.no_fft_study:
+        print("[bench] FFT variants", file=sys.stderr, flush=True)
+        fftv = fft_variant_study(parts, args.fft_study_steps)
 
     ens = None
     if not args.no_ensemble:
@@ -254,6 +302,7 @@ def main():
         "cpu_baseline": cpu,
         "fp64_plain": fp64,
         "ensemble": ens,
+        "fft_variants": fftv,
         "stages_ms": st32,
     }
     if fp64:
