@@ -40,10 +40,13 @@ def make(prec, n, binning=None, **extra):
     )
 
 
-def time_gpu(prec, parts, steps, warmup, profile_steps):
+def time_gpu(prec, parts, steps, warmup, profile_steps, tune=False):
     import stochasticsampling_amd as ss
+    from stochasticsampling_amd import autotune
 
     p = make(prec, parts.shape[0])
+    if tune:  # tile shape and re-sort interval of the size class (shipped table)
+        p = autotune.tuned(p)
     sim = ss.Simulation(p)
     sim.init(parts)
     if warmup:
@@ -151,9 +154,18 @@ def ensemble_configs():
 
 
 def time_ensemble(steps, warmup):
-    import stochasticsampling_amd as ss
+    import ctypes
 
-    plist = ensemble_configs()
+    import stochasticsampling_amd as ss
+    from stochasticsampling_amd import autotune
+
+    # tile shape per size class from the tuning table; the members share one
+    # re-sort interval (that of the first member's class)
+    plist = [autotune.tuned(p) for p in ensemble_configs()]
+    for p in plist:
+        p.resort_interval = plist[0].resort_interval
+        if not any(p.tile):
+            p.tile = (ctypes.c_int32 * 3)(4, 4, 4)
     ens = ss.Ensemble(plist)
     total = 0
     for p, m in zip(plist, ens.members):
@@ -226,7 +238,7 @@ def main():
     parts = ss.initial_condition(p64, amplitude=0.05, seed=2)
 
     print("[bench] FP32 sorted path", file=sys.stderr, flush=True)
-    ms32, st32 = time_gpu(ss.FP32, parts, args.steps, args.warmup, args.profile_steps)
+    ms32, st32 = time_gpu(ss.FP32, parts, args.steps, args.warmup, args.profile_steps, tune=True)
     fp64 = None
     if not args.no_fp64:
         print("[bench] FP64 plain path", file=sys.stderr, flush=True)
@@ -293,7 +305,8 @@ def main():
         "config": {
             "workload": "2^24 particles, grid 64^3 x 16 x 16, L=50, dt=1e-3, D_t=0.1, D_r=0.5, "
                         "alpha=-1, FP32 states with compensated positions, FP32 FFT, Philox noise "
-                        "seed 2, particles grouped by 4^3-cell tiles",
+                        "seed 2, particles grouped by tiles (tile shape and re-sort interval from "
+                        "the autotuner's table, see stages_ms.plan)",
             "particles": n,
             "grid": [64, 64, 64, 16, 16],
             "box": [50.0, 50.0, 50.0],
