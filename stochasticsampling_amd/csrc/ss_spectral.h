@@ -96,7 +96,12 @@ __global__ void __launch_bounds__(SPEC_PENCIL_THREADS)
     cplx<T>* A = tw + nx;
     cplx<T>* B = A + (size_t)nl * ls;
     const size_t ns = (size_t)nx * np;
-    const int p0 = blockIdx.x * pb_max;
+    // workgroups are dispatched round-robin over the XCDs: give each XCD a
+    // contiguous run of pencil groups, so that the two groups sharing a
+    // cache line of every x row are read through the same L2
+    const int groups = gridDim.x;
+    const int g = groups % 8 == 0 ? (blockIdx.x % 8) * (groups / 8) + blockIdx.x / 8 : blockIdx.x;
+    const int p0 = g * pb_max;
     const int pb = min(pb_max, np - p0);
     for (int t = threadIdx.x; t < nx; t += blockDim.x) tw[t] = twx[t];
     // line (c, p) at A[(c pb_max + p) ls]; lanes over p for coalesced loads
