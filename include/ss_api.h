@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define SS_API_VERSION 1
+#define SS_API_VERSION 2
 
 /* Return codes. */
 #define SS_OK 0
@@ -37,6 +37,11 @@ extern "C" {
 /* Source of the per-particle Gaussian / Rayleigh increments. */
 #define SS_NOISE_PHILOX 0   /* counter-based Philox4x32-10 keyed by (seed, step, particle id) */
 #define SS_NOISE_EXTERNAL 1 /* caller-supplied RandomVector buffer, see ss_set_noise */
+#define SS_NOISE_REFERENCE 2 /* the reference's own stream: noise_streams Pcg32 generators
+                                seeded alike by seed_from_u64(seed), chunk c of N/T + 1
+                                particles drawn from generator c (rand_distr ziggurat
+                                normals, rand uniforms), src/bin/simulation/mod.rs:111-128,
+                                259-286 */
 
 /* Binning strategy for particle -> (x,y,z,phi,theta) histogram. */
 #define SS_BIN_AUTO 0      /* SORTED for FP32, HISTOGRAM for FP64                        */
@@ -106,6 +111,9 @@ typedef struct ss_params {
        (0: 16). */
     int32_t tile[3];
     int32_t resort_interval;
+    /* SS_NOISE_REFERENCE: number of generator streams, the reference run's
+       RAYON_NUM_THREADS (mod.rs:113-128); required (>= 1) in that mode. */
+    int32_t noise_streams;
 } ss_params;
 
 /* Fills p with the reference's defaults: zero optional parameters, field along y,
@@ -176,13 +184,29 @@ int ss_get_magnetic_field(ss_sim* sim, double* b, double* grad_b);
 
 /* Simulation::get_snapshot (mod.rs:210-217).  particles: N x 5; compensation
    (may be NULL): N x 3 low-order position parts of the FP32 path (zeros in
-   FP64).  The noise generator state is (seed, timestep). */
+   FP64).  The Philox noise state is (seed, timestep); SS_NOISE_REFERENCE
+   adds the generators of ss_get_rng_state. */
 int ss_get_snapshot(ss_sim* sim, double* particles, double* compensation, uint64_t* timestep);
 
 /* Simulation::resume (mod.rs:199-207): init from the snapshot, restore the
    compensation (may be NULL) and the timestep counter. */
 int ss_resume(ss_sim* sim, const double* particles, const double* compensation, uint64_t n,
               uint64_t timestep);
+
+/* Snapshot::rng_state (mod.rs:80-86, 199-217) for SS_NOISE_REFERENCE: the
+   noise_streams generators as (state, increment) pairs, the serde fields of
+   rand_pcg's Lcg64Xsh32.  ss_get_rng_state writes 2 * noise_streams values
+   (count >= noise_streams, else SS_ERR_INVALID) and returns noise_streams;
+   ss_set_rng_state replaces the first min(count, noise_streams) generators,
+   as Simulation::resume's zip does (call it after ss_resume).  Other noise
+   modes have no generator state: get returns 0, set accepts count 0 only. */
+int ss_get_rng_state(ss_sim* sim, uint64_t* state_increment, int32_t count);
+int ss_set_rng_state(ss_sim* sim, const uint64_t* state_increment, int32_t count);
+
+/* The RandomVector array of the last step (N x 5 doubles: x, y, z,
+   axis_angle, rotate_angle) for SS_NOISE_EXTERNAL and SS_NOISE_REFERENCE
+   (mod.rs:73, 259-286); SS_ERR_STATE before the first step. */
+int ss_get_noise(ss_sim* sim, double* rv, uint64_t n);
 
 /* Per-stage device time of the last ss_profile_steps call, measured with HIP
    events on the simulation's stream.  Stage names: "bin", "fft_forward",
@@ -253,6 +277,16 @@ int ss_ensemble_step(ss_ensemble* ens, uint64_t nsteps, uint64_t* timestep_out);
  * ------------------------------------------------------------------------- */
 int ss_initial_condition(const ss_params* params, double amplitude, uint64_t seed,
                          double* particles, uint64_t n);
+
+/* Particle::create_isotropic (src/particle.rs:224-255) and
+   Particle::create_homogeneous (src/particle.rs:257-297), drawn on `device`
+   from the reference's stream: Pcg64Mcg::seed_from_u64(seed), five
+   Uniform(0, 1) values per particle in order.  particles: n x 5 doubles on
+   the host.  create_homogeneous returns SS_ERR_INVALID with the reference's
+   panic message for kappa == 0 or an angle that is NaN. */
+int ss_create_isotropic(uint64_t n, const ss_box* box, uint64_t seed, int32_t device, double* particles);
+int ss_create_homogeneous(uint64_t n, double kappa, const ss_box* box, uint64_t seed, int32_t device,
+                          double* particles);
 
 /* ---------------------------------------------------------------------------
  * FFT plan -- replaces FFTPlan (crates/fftw3/src/fft.rs:33-244): unnormalised,

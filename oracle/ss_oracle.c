@@ -18,14 +18,19 @@
  *   ndarray 0.12.1 dot           -> 8-way unrolled accumulation for contiguous
  *                                   1-D dots (numeric_util::unrolled_dot).
  *   num-complex 0.2.3 division   -> 1/z computed as re/(re*re) for real z.
- *   rand_pcg / rand_distr        -> NOT restated: noise is Philox4x32-10
- *                                   (Salmon et al., SC'11), identical in the
- *                                   HIP library, so both codes see the same
- *                                   increments.
+ *   rand_pcg 0.2.0 / rand_core 0.5.0 / rand 0.7.0 / rand_distr 0.2.1
+ *                                -> PCG32 and MCG128 streams, seed_from_u64,
+ *                                   the f64 uniform conversions and the
+ *                                   256-layer ziggurat normal, for the
+ *                                   reference's own noise (SS_NOISE_REFERENCE)
+ *                                   and initial conditions.  The default noise
+ *                                   is Philox4x32-10 (Salmon et al., SC'11),
+ *                                   identical in the HIP library.
  */
 #include "ss_oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -719,6 +724,252 @@ void o_noise(uint64_t seed, uint64_t step, uint64_t id, double dr, double out[5]
 }
 
 /* ------------------------------------------------------------------------ */
+/* The reference's own random streams, restated from the published           */
+/* algorithms of its pinned dependencies (Cargo.lock):                        */
+/*   rand_pcg 0.2.0   Lcg64Xsh32 (= Pcg32, PCG XSH-RR 64/32) and Mcg128Xsl64  */
+/*                    (= Pcg64Mcg, MCG XSL-RR 128/64), O'Neill 2014;          */
+/*   rand_core 0.5.0  seed_from_u64 (PCG32 output words fill the seed);       */
+/*   rand 0.7.0       Uniform<f64> in [0, 1), Standard and Open01 for f64;    */
+/*   rand_distr 0.2.1 StandardNormal: 256-layer ziggurat (Marsaglia & Tsang   */
+/*                    2000, Doornik 2005 layout) with its R, V constants;     */
+/* used by Simulation::new / do_timestep (src/bin/simulation/mod.rs:111-128,  */
+/* 259-286) and Particle::create_isotropic / create_homogeneous               */
+/* (src/particle.rs:224-297).                                                 */
+/* ------------------------------------------------------------------------ */
+static inline void qmul(const double a0, const double* a, const double b0, const double* b,
+                        double* r0, double* r);
+
+static const uint64_t PCG_MUL = 6364136223846793005ull;
+static const uint64_t SEED_INC = 11634580027462260723ull; /* rand_core seed_from_u64 */
+
+static inline uint32_t pcg_xsh_rr(uint64_t s) {
+    uint32_t x = (uint32_t)(((s >> 18) ^ s) >> 27);
+    uint32_t r = (uint32_t)(s >> 59);
+    return (x >> r) | (x << ((32u - r) & 31u));
+}
+
+/* Lcg64Xsh32::from_state_incr: state += increment, then one step. */
+static void pcg32_from_state_incr(uint64_t state, uint64_t inc, o_pcg32* r) {
+    r->inc = inc;
+    r->state = (state + inc) * PCG_MUL + inc;
+}
+
+void o_pcg32_new(uint64_t state, uint64_t stream, o_pcg32* r) {
+    pcg32_from_state_incr(state, (stream << 1) | 1u, r);
+}
+
+/* from_seed: two little-endian u64, the increment forced odd. */
+void o_pcg32_from_seed(const uint8_t seed[16], o_pcg32* r) {
+    uint64_t w[2] = {0, 0};
+    for (int k = 0; k < 16; ++k) w[k / 8] |= (uint64_t)seed[k] << (8 * (k % 8));
+    pcg32_from_state_incr(w[0], w[1] | 1u, r);
+}
+
+/* rand_core 0.5 SeedableRng::seed_from_u64: each 4-byte word of the seed is
+   a PCG32 output of a private LCG advanced before the output. */
+static void seed_bytes_from_u64(uint64_t state, uint8_t* seed, int nbytes) {
+    for (int k = 0; k < nbytes; k += 4) {
+        state = state * PCG_MUL + SEED_INC;
+        uint32_t x = pcg_xsh_rr(state);
+        for (int b = 0; b < 4 && k + b < nbytes; ++b) seed[k + b] = (uint8_t)(x >> (8 * b));
+    }
+}
+
+void o_pcg32_seed_from_u64(uint64_t s, o_pcg32* r) {
+    uint8_t seed[16];
+    seed_bytes_from_u64(s, seed, 16);
+    o_pcg32_from_seed(seed, r);
+}
+
+uint32_t o_pcg32_next_u32(o_pcg32* r) {
+    uint64_t old = r->state;
+    r->state = old * PCG_MUL + r->inc;
+    return pcg_xsh_rr(old);
+}
+
+/* rand_core impls::next_u64_via_u32: low word first. */
+uint64_t o_pcg32_next_u64(o_pcg32* r) {
+    uint64_t lo = o_pcg32_next_u32(r);
+    uint64_t hi = o_pcg32_next_u32(r);
+    return (hi << 32) | lo;
+}
+
+typedef unsigned __int128 u128;
+static const u128 MCG_MUL = ((u128)0x2360ED051FC65DA4ull << 64) | 0x4385DF649FCCF645ull;
+
+/* Mcg128Xsl64::new: the state forced odd. */
+void o_mcg128_new(uint64_t lo, uint64_t hi, o_mcg128* r) {
+    r->lo = lo | 1u;
+    r->hi = hi;
+}
+
+void o_mcg128_seed_from_u64(uint64_t s, o_mcg128* r) {
+    uint8_t seed[16];
+    seed_bytes_from_u64(s, seed, 16);
+    uint64_t w[2] = {0, 0};
+    for (int k = 0; k < 16; ++k) w[k / 8] |= (uint64_t)seed[k] << (8 * (k % 8));
+    o_mcg128_new(w[0], w[1], r);
+}
+
+/* One multiplication, then XSL-RR of the new state. */
+uint64_t o_mcg128_next_u64(o_mcg128* r) {
+    u128 s = (((u128)r->hi << 64) | r->lo) * MCG_MUL;
+    r->lo = (uint64_t)s;
+    r->hi = (uint64_t)(s >> 64);
+    uint64_t x = r->hi ^ r->lo;
+    unsigned rot = (unsigned)(r->hi >> 58);
+    return (x >> rot) | (x << ((64u - rot) & 63u));
+}
+
+/* Bits 12..63 as the mantissa of a double with exponent e: [2^e, 2^(e+1)). */
+static inline double mant_exp(uint64_t bits, int e) {
+    uint64_t v = (bits >> 12) | ((uint64_t)(1023 + e) << 52);
+    double d;
+    memcpy(&d, &v, 8);
+    return d;
+}
+
+/* rand 0.7 UniformFloat::sample for Uniform::new(0, 1): scale 1, low 0. */
+double o_uniform01(uint64_t bits) { return (mant_exp(bits, 0) - 1.0) * 1.0 + 0.0; }
+/* Standard for f64: the top 53 bits times 2^-53. */
+double o_gen_f64(uint64_t bits) { return (double)(bits >> 11) * (1.0 / 9007199254740992.0); }
+/* Open01 for f64: [1, 2) minus (1 - eps/2). */
+double o_open01(uint64_t bits) { return mant_exp(bits, 0) - (1.0 - 2.220446049250313e-16 / 2.0); }
+
+/* Ziggurat tables of rand_distr's normal: x_0 = V / f(R), x_1 = R,
+   x_i = f^-1(V / x_(i-1) + f(x_(i-1))), x_256 = 0, f_i = f(x_i), with
+   f(x) = exp(-x^2/2).  The crate's tables are these numbers printed with 18
+   decimals; printing and reading back reproduces them. */
+static double ZX[257], ZF[257];
+static int zig_ready = 0;
+void o_zig_tables(double* x, double* f) {
+    if (!zig_ready) {
+        const double R = 3.6541528853610088, V = 0.00492867323399;
+        double xv[257], fv[257];
+        xv[0] = V / exp(-R * R / 2.0);
+        xv[1] = R;
+        for (int i = 2; i < 256; ++i) {
+            double last = xv[i - 1];
+            xv[i] = sqrt(-2.0 * log(V / last + exp(-last * last / 2.0)));
+        }
+        xv[256] = 0.0;
+        for (int i = 0; i < 257; ++i) fv[i] = exp(-xv[i] * xv[i] / 2.0);
+        char buf[64];
+        for (int i = 0; i < 257; ++i) {
+            snprintf(buf, sizeof buf, "%.18f", xv[i]);
+            ZX[i] = strtod(buf, NULL);
+            snprintf(buf, sizeof buf, "%.18f", fv[i]);
+            ZF[i] = strtod(buf, NULL);
+        }
+        zig_ready = 1;
+    }
+    if (x) memcpy(x, ZX, sizeof ZX);
+    if (f) memcpy(f, ZF, sizeof ZF);
+}
+
+/* StandardNormal for f64 (rand_distr 0.2.1): symmetric ziggurat on one u64
+   per try (layer from the low 8 bits, u in [-1, 1) from the top 52), tail
+   from pairs of Open01 draws, wedge test against exp(-x^2/2). */
+double o_standard_normal(o_pcg32* r) {
+    o_zig_tables(NULL, NULL);
+    const double R = ZX[1];
+    for (;;) {
+        uint64_t bits = o_pcg32_next_u64(r);
+        int i = (int)(bits & 0xff);
+        double u = mant_exp(bits, 1) - 3.0;
+        double x = u * ZX[i];
+        if (fabs(x) < ZX[i + 1]) return x;
+        if (i == 0) {
+            double xx = 1.0, yy = 0.0;
+            while (-2.0 * yy < xx * xx) {
+                double x_ = o_open01(o_pcg32_next_u64(r));
+                double y_ = o_open01(o_pcg32_next_u64(r));
+                xx = log(x_) / R;
+                yy = log(y_);
+            }
+            return u < 0.0 ? xx - R : R - xx;
+        }
+        double g = o_gen_f64(o_pcg32_next_u64(r));
+        if (ZF[i + 1] + (ZF[i] - ZF[i + 1]) * g < exp(-x * x / 2.0)) return x;
+    }
+}
+
+/* The noise draw of do_timestep (mod.rs:259-286): chunks of N / T + 1
+   particles, chunk c drawn in order from stream c; per particle three
+   normals, then axis angle 2 pi U and rotate angle rayleigh_pdf(dr, U)
+   (mod.rs:383-385). */
+void o_ref_noise(o_pcg32* rngs, int t, uint64_t n, double dr, double* rv) {
+    uint64_t cs = n / (uint64_t)t + 1;
+#pragma omp parallel for schedule(static, 1)
+    for (int c = 0; c < t; ++c) {
+        uint64_t b = (uint64_t)c * cs;
+        if (b >= n) continue;
+        uint64_t e = b + cs < n ? b + cs : n;
+        o_pcg32* r = rngs + c;
+        for (uint64_t i = b; i < e; ++i) {
+            double* o = rv + 5 * i;
+            o[0] = o_standard_normal(r);
+            o[1] = o_standard_normal(r);
+            o[2] = o_standard_normal(r);
+            o[3] = TWOPI_ * o_uniform01(o_pcg32_next_u64(r));
+            o[4] = dr * sqrt(-2. * log(1. - o_uniform01(o_pcg32_next_u64(r))));
+        }
+    }
+}
+
+/* Particle::create_isotropic (particle.rs:224-255): Pcg64Mcg seeded from
+   the u64 seed, five uniforms per particle: box * U for the position,
+   2 pi U, pdf_sin(2 U) = acos(1 - 2 U); then Particle::new. */
+void o_create_isotropic(uint64_t n, const ss_box* bs, uint64_t seed, double* out) {
+    o_mcg128 r;
+    o_mcg128_seed_from_u64(seed, &r);
+    for (uint64_t i = 0; i < n; ++i) {
+        double in[5];
+        in[0] = bs->x * o_uniform01(o_mcg128_next_u64(&r));
+        in[1] = bs->y * o_uniform01(o_mcg128_next_u64(&r));
+        in[2] = bs->z * o_uniform01(o_mcg128_next_u64(&r));
+        in[3] = TWOPI_ * o_uniform01(o_mcg128_next_u64(&r));
+        in[4] = acos(1. - 2. * o_uniform01(o_mcg128_next_u64(&r)));
+        o_particle_new(in, bs, out + 5 * i);
+    }
+}
+
+/* Particle::create_homogeneous (particle.rs:257-297): polar angle from
+   pdf_homogeneous_fixpoint (particle.rs:347-360), Particle::new, then the
+   orientation vector rotated by the quaternion axis_angle([1, 0, 0], -pi/2)
+   (quaternion 0.3.1: (cos h, axis sin h), h = angle / 2; q (0, v) q*) and
+   written back by Orientation::from_vector_mut without a further wrap.
+   Returns -1 (the reference panics) for kappa == 0 or a NaN angle. */
+int o_create_homogeneous(uint64_t n, double kappa, const ss_box* bs, uint64_t seed, double* out) {
+    if (kappa == 0.0) return -1;
+    o_mcg128 r;
+    o_mcg128_seed_from_u64(seed, &r);
+    const double half = -(PI_ / 2.) / 2.;
+    const double sh = sin(half), q0 = cos(half);
+    const double qv[3] = {1. * sh, 0. * sh, 0. * sh};
+    const double qc[3] = {-qv[0], -qv[1], -qv[2]};
+    for (uint64_t i = 0; i < n; ++i) {
+        double in[5];
+        in[0] = bs->x * o_uniform01(o_mcg128_next_u64(&r));
+        in[1] = bs->y * o_uniform01(o_mcg128_next_u64(&r));
+        in[2] = bs->z * o_uniform01(o_mcg128_next_u64(&r));
+        in[3] = TWOPI_ * o_uniform01(o_mcg128_next_u64(&r));
+        double x = o_uniform01(o_mcg128_next_u64(&r));
+        in[4] = acos(log(exp(kappa) - 2. * x * sinh(kappa)) / kappa);
+        if (isnan(in[4])) return -1;
+        double* p = out + 5 * i;
+        o_particle_new(in, bs, p);
+        double cp = cos(p[3]), sp = sin(p[3]), ct = cos(p[4]), st = sin(p[4]);
+        double v[3] = {st * cp, st * sp, ct};
+        double t0, tv[3], r0, w[3];
+        qmul(q0, qv, 0.0, v, &t0, tv);
+        qmul(t0, tv, q0, qc, &r0, w);
+        o_orientation_from_vector(w, p + 3);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Particle update: LangevinBuilder chain of src/bin/simulation/mod.rs:323-363 */
 /* with the modifiers of langevin_builder/modifiers.rs                       */
 /* ------------------------------------------------------------------------ */
@@ -846,6 +1097,8 @@ struct osim {
     double* gb;        /* 9 complex */
     uint64_t timestep;
     int initialised;
+    o_pcg32* rngs; /* SS_NOISE_REFERENCE: one stream per chunk */
+    int nrng;
 };
 
 void osim_set_threads(int n) {
@@ -872,8 +1125,14 @@ osim* osim_create(const ss_params* p) {
     s->g = calloc(2 * 9 * s->s.ns, sizeof(double));
     s->b = calloc(2 * 3 * s->s.ns, sizeof(double));
     s->gb = calloc(2 * 9 * s->s.ns, sizeof(double));
+    if (p->noise_mode == SS_NOISE_REFERENCE) {
+        /* Simulation::new (mod.rs:111-128): T streams, all seeded alike */
+        s->nrng = p->noise_streams;
+        s->rngs = malloc(sizeof(o_pcg32) * (s->nrng > 0 ? s->nrng : 1));
+        for (int c = 0; c < s->nrng && s->rngs; ++c) o_pcg32_seed_from_u64(p->seed, s->rngs + c);
+    }
     if (!s->particles || !s->noise || !s->counts || !s->dist || !s->sigma || !s->u || !s->g ||
-        !s->b || !s->gb) {
+        !s->b || !s->gb || (p->noise_mode == SS_NOISE_REFERENCE && (s->nrng < 1 || !s->rngs))) {
         osim_destroy(s);
         return NULL;
     }
@@ -891,6 +1150,7 @@ void osim_destroy(osim* s) {
     free(s->g);
     free(s->b);
     free(s->gb);
+    free(s->rngs);
     solver_free(&s->s);
     free(s);
 }
@@ -929,6 +1189,8 @@ static void one_step(osim* s) {
 #pragma omp parallel for schedule(static)
         for (long long i = 0; i < (long long)s->n; ++i)
             o_noise(p->seed, s->timestep, (uint64_t)i, dr, s->noise + 5 * i);
+    } else if (p->noise_mode == SS_NOISE_REFERENCE) {
+        o_ref_noise(s->rngs, s->nrng, s->n, dr, s->noise);
     }
     if (magnetic_needed(p)) magnetic_core(sv, s->dist, s->b, s->gb);
     flow_core(sv, p->hydro_screening, s->dist, s->sigma, s->u, s->g);
@@ -991,6 +1253,30 @@ int osim_step(osim* s, uint64_t nsteps) {
 }
 
 uint64_t osim_get_timestep(const osim* s) { return s->timestep; }
+
+/* Snapshot::rng_state (mod.rs:80-86, 199-217): (state, increment) per
+   stream.  Setting copies min(count, T) streams, like the reference's zip. */
+int osim_get_rng_state(const osim* s, uint64_t* st, int count) {
+    if (count < s->nrng) return -1;
+    for (int c = 0; c < s->nrng; ++c) {
+        st[2 * c] = s->rngs[c].state;
+        st[2 * c + 1] = s->rngs[c].inc;
+    }
+    return s->nrng;
+}
+
+int osim_set_rng_state(osim* s, const uint64_t* st, int count) {
+    for (int c = 0; c < s->nrng && c < count; ++c) {
+        s->rngs[c].state = st[2 * c];
+        s->rngs[c].inc = st[2 * c + 1];
+    }
+    return 0;
+}
+
+int osim_set_timestep(osim* s, uint64_t t) {
+    s->timestep = t;
+    return 0;
+}
 
 int osim_get_particles(const osim* s, double* out, uint64_t n) {
     if (n > s->n) return -1;

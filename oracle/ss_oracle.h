@@ -32,6 +32,11 @@ int osim_get_flow_field(const osim* s, double* u, double* grad_u);
 int osim_get_magnetic_field(const osim* s, double* b, double* grad_b);
 /* Last step's noise (N x 5), whichever source produced it. */
 int osim_get_noise(const osim* s, double* rv);
+/* SS_NOISE_REFERENCE streams as (state, increment) pairs; get returns the
+   stream count (-1 if count is too small). */
+int osim_get_rng_state(const osim* s, uint64_t* st, int count);
+int osim_set_rng_state(osim* s, const uint64_t* st, int count);
+int osim_set_timestep(osim* s, uint64_t t);
 /* Number of OpenMP threads the parallel parts use (0: OpenMP default). */
 void osim_set_threads(int n);
 
@@ -58,6 +63,30 @@ void o_update_one(const ss_params* p, const double part[5], const double fields[
                   const double rv[5], double out[5]);
 void o_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void o_noise(uint64_t seed, uint64_t step, uint64_t id, double dr, double out[5]);
+/* The reference's random streams (rand_pcg 0.2.0, rand_core 0.5.0, rand 0.7.0,
+   rand_distr 0.2.1). */
+typedef struct o_pcg32 {
+    uint64_t state, inc;
+} o_pcg32;
+typedef struct o_mcg128 {
+    uint64_t lo, hi;
+} o_mcg128;
+void o_pcg32_new(uint64_t state, uint64_t stream, o_pcg32* r);
+void o_pcg32_from_seed(const uint8_t seed[16], o_pcg32* r);
+void o_pcg32_seed_from_u64(uint64_t s, o_pcg32* r);
+uint32_t o_pcg32_next_u32(o_pcg32* r);
+uint64_t o_pcg32_next_u64(o_pcg32* r);
+void o_mcg128_new(uint64_t lo, uint64_t hi, o_mcg128* r);
+void o_mcg128_seed_from_u64(uint64_t s, o_mcg128* r);
+uint64_t o_mcg128_next_u64(o_mcg128* r);
+double o_uniform01(uint64_t bits);
+double o_gen_f64(uint64_t bits);
+double o_open01(uint64_t bits);
+void o_zig_tables(double* x, double* f);
+double o_standard_normal(o_pcg32* r);
+void o_ref_noise(o_pcg32* rngs, int t, uint64_t n, double dr, double* rv);
+void o_create_isotropic(uint64_t n, const ss_box* bs, uint64_t seed, double* out);
+int o_create_homogeneous(uint64_t n, double kappa, const ss_box* bs, uint64_t seed, double* out);
 
 #ifdef __cplusplus
 }

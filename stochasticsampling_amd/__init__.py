@@ -16,6 +16,7 @@ from .params import (  # noqa: F401
     FP64,
     NOISE_EXTERNAL,
     NOISE_PHILOX,
+    NOISE_REFERENCE,
     Params,
     make_params,
 )
@@ -25,6 +26,8 @@ from .simulation import (  # noqa: F401
     FFTPlan,
     Simulation,
     Snapshot,
+    create_homogeneous,
+    create_isotropic,
     debug_coord_to_grid,
     debug_update,
     initial_condition,

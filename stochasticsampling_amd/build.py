@@ -10,7 +10,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "_lib")
 LIB = os.path.join(LIBDIR, "libss.so")
-SOURCES = ["ss_fft.hip", "ss_sim.hip"]
+SOURCES = ["ss_fft.hip", "ss_sim.hip", "ss_refrng.hip"]
 HEADERS = ["ss_device.h", "ss_internal.h", "ss_kernels.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

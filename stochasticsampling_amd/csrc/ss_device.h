@@ -383,4 +383,19 @@ __device__ inline void langevin(const UpdateParams<T>& P, T phi, T theta, const 
     dp[2] = p2;
 }
 
+// Particle::new (particle.rs:186-201): modulo and ang_pbc, twice, in FP64.
+__device__ inline void sanitize(const double* in, const double* box, double* out) {
+    double a0, a1, b0, b1;
+    out[0] = modulo(in[0], box[0]);
+    out[1] = modulo(in[1], box[1]);
+    out[2] = modulo(in[2], box[2]);
+    ang_pbc(in[3], in[4], a0, a1);
+    out[0] = modulo(out[0], box[0]);
+    out[1] = modulo(out[1], box[1]);
+    out[2] = modulo(out[2], box[2]);
+    ang_pbc(a0, a1, b0, b1);
+    out[3] = b0;
+    out[4] = b1;
+}
+
 }  // namespace ss

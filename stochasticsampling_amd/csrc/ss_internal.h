@@ -57,4 +57,17 @@ struct Fft3d {
 
 int factorize(int n, int* fac);
 
+// ---------------------------------------------------------------------------
+// SS_NOISE_REFERENCE: the reference's Pcg32 streams decoded on the GPU
+// (ss_refrng.hip).
+// ---------------------------------------------------------------------------
+struct RefNoise;
+int refnoise_create(RefNoise** out, uint64_t n, int streams, uint64_t seed);
+void refnoise_destroy(RefNoise* r);
+int refnoise_streams(const RefNoise* r);
+// Enqueue one step's RandomVectors into noise ([5][n], double or float).
+hipError_t refnoise_enqueue(RefNoise* r, void* noise, bool fp64, double dr, hipStream_t stream);
+int refnoise_get(RefNoise* r, uint64_t* state_increment, int count, hipStream_t stream);
+int refnoise_set(RefNoise* r, const uint64_t* state_increment, int count, hipStream_t stream);
+
 }  // namespace ss

@@ -556,21 +556,6 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // Import / export between caller AoS doubles and the device state.
 // ---------------------------------------------------------------------------
-// Particle::new (particle.rs:186-201): modulo and ang_pbc, twice, in FP64.
-__device__ inline void sanitize(const double* in, const double* box, double* out) {
-    double a0, a1, b0, b1;
-    out[0] = modulo(in[0], box[0]);
-    out[1] = modulo(in[1], box[1]);
-    out[2] = modulo(in[2], box[2]);
-    ang_pbc(in[3], in[4], a0, a1);
-    out[0] = modulo(out[0], box[0]);
-    out[1] = modulo(out[1], box[1]);
-    out[2] = modulo(out[2], box[2]);
-    ang_pbc(a0, a1, b0, b1);
-    out[3] = b0;
-    out[4] = b1;
-}
-
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_import(const double* __restrict__ aos, const double* __restrict__ comp, uint64_t n, State<T> s,
@@ -620,6 +605,14 @@ __global__ void __launch_bounds__(256)
     } else if (comp) {
         for (int k = 0; k < 3; ++k) comp[3 * j + k] = 0.0;
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_export_noise(const T* __restrict__ noise, uint64_t n,
+                                                      double* __restrict__ aos) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int k = 0; k < 5; ++k) aos[5 * i + k] = (double)noise[(size_t)k * n + i];
 }
 
 template <typename T>

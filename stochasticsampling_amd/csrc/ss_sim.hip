@@ -164,6 +164,8 @@ struct ss_sim {
     uint64_t timestep = 0;
     bool initialised = false;
     bool have_noise = false;
+    ss::RefNoise* refn = nullptr;  // SS_NOISE_REFERENCE generators and decoder buffers
+    double dr = 0.0;               // sqrt(2 D_r dt), the rotate-angle scale (mod.rs:266-269)
     bool have_fields = false;
     bool have_hist = false;
     hipStream_t stream = nullptr;
@@ -433,7 +435,7 @@ static void launch_tile_kernel(ss_sim* s, const TileArgs<T, NM>& a) {
 
 template <typename T, int NM, bool MOVE>
 static void launch_tile_nm(ss_sim* s, const TileArgs<T, NM>& a) {
-    const bool ext = s->p.noise_mode == SS_NOISE_EXTERNAL;
+    const bool ext = s->p.noise_mode != SS_NOISE_PHILOX;
     const bool keep = a.hist != nullptr;
     if (ext) {
         if (keep) launch_tile_kernel<T, NM, MOVE, true, true>(s, a);
@@ -500,7 +502,7 @@ static hipError_t launch_update(ss_sim* s) {
     UpdateParams<T> P = update_params<T>(s);
     State<T> st = state_a<T>(s);
     dim3 grid(blocks_for(s->n));
-    if (s->p.noise_mode == SS_NOISE_EXTERNAL)
+    if (s->p.noise_mode != SS_NOISE_PHILOX)
         hipLaunchKernelGGL((k_update_plain<T, true>), grid, dim3(256), 0, s->stream, st, s->n, g, P,
                            (const T*)s->rec, s->rs, s->p.seed, s->d_step, s->step_off, (const T*)s->noise);
     else
@@ -614,6 +616,12 @@ static int step_once(ss_sim* s, hipEvent_t* ev) {
     if (s->swap_hist) std::swap(s->hist, s->hist_next);
     int rc = spectral_stage<T>(s, ev);
     if (rc != SS_OK) return rc;
+    if (s->refn) {
+        // the RandomVectors of do_timestep (mod.rs:259-286) from the reference's streams
+        if ((e = refnoise_enqueue(s->refn, s->noise, s->prec == SS_FP64, s->dr, s->stream)) != hipSuccess)
+            return fail(SS_ERR_DEVICE, std::string("reference noise: ") + hipGetErrorString(e));
+        s->have_noise = true;
+    }
     if ((e = launch_update<T>(s)) != hipSuccess)
         return fail(SS_ERR_DEVICE, std::string("update: ") + hipGetErrorString(e));
     mark(ST_UPDATE);
@@ -797,6 +805,8 @@ static void free_all(ss_sim* s) {
     for (void* p : s->owned) (void)hipFree(p);
     s->owned.clear();
     s->fft.release();
+    refnoise_destroy(s->refn);
+    s->refn = nullptr;
     if (s->stream) (void)hipStreamDestroy(s->stream);
     s->stream = nullptr;
 }
@@ -889,7 +899,7 @@ static int create_impl(ss_sim* s) {
     A(s->rec, (size_t)s->ns * s->rs * sizeof(T));
     SS_HIP_CHECK(hipMemset(s->rec, 0, (size_t)s->ns * s->rs * sizeof(T)));
     SS_HIP_CHECK(hipMemset(s->spec_in, 0, cs * s->nm));
-    if (p.noise_mode == SS_NOISE_EXTERNAL) A(s->noise, 5 * n * sizeof(T));
+    if (p.noise_mode != SS_NOISE_PHILOX) A(s->noise, 5 * n * sizeof(T));
     const size_t stage_doubles = std::max<size_t>(5 * n, (size_t)24 * s->ns);
     A(s->staging, stage_doubles * sizeof(double));
     A(s->staging2, 3 * n * sizeof(double));
@@ -945,6 +955,10 @@ static int create_impl(ss_sim* s) {
         SS_HIP_CHECK(hipMemcpy(s->kfix, kf.data(), kf.size() * 8, hipMemcpyHostToDevice));
     }
     if ((rc = s->fft.init(s->nx, s->ny, s->nz, s->prec, p.fft_variant)) != SS_OK) return rc;
+    s->dr = std::sqrt(2. * p.diffusion_rotational * p.timestep);
+    if (p.noise_mode == SS_NOISE_REFERENCE &&
+        (rc = refnoise_create(&s->refn, s->n, p.noise_streams, p.seed)) != SS_OK)
+        return rc;
     return choose_spectral<T>(s);
 }
 
@@ -1006,8 +1020,11 @@ static int validate(const ss_params* p) {
     if (p->number_of_particles < 1) return fail(SS_ERR_INVALID, "number_of_particles must be >= 1");
     if (p->number_of_particles >= (1ull << 31)) return fail(SS_ERR_UNSUPPORTED, "at most 2^31-1 particles");
     if (p->precision != SS_FP64 && p->precision != SS_FP32) return fail(SS_ERR_INVALID, "bad precision");
-    if (p->noise_mode != SS_NOISE_PHILOX && p->noise_mode != SS_NOISE_EXTERNAL)
+    if (p->noise_mode != SS_NOISE_PHILOX && p->noise_mode != SS_NOISE_EXTERNAL &&
+        p->noise_mode != SS_NOISE_REFERENCE)
         return fail(SS_ERR_INVALID, "bad noise_mode");
+    if (p->noise_mode == SS_NOISE_REFERENCE && p->noise_streams < 1)  // mod.rs:113-116
+        return fail(SS_ERR_INVALID, "No environment variable 'RAYON_NUM_THREADS' set.");
     if (p->binning < SS_BIN_AUTO || p->binning > SS_BIN_SORTED) return fail(SS_ERR_INVALID, "bad binning");
     if (p->fft_variant != SS_FFT_AUTO && p->fft_variant != SS_FFT_RADIX && p->fft_variant != SS_FFT_DFT)
         return fail(SS_ERR_INVALID, "bad fft_variant");
@@ -1280,6 +1297,40 @@ int ss_resume(ss_sim* s, const double* particles, const double* compensation, ui
     SS_HIP_CHECK(hipMemcpy(s->d_step, &timestep, sizeof(timestep), hipMemcpyHostToDevice));
     s->initialised = true;
     s->have_fields = false;
+    return SS_OK;
+}
+
+int ss_get_rng_state(ss_sim* s, uint64_t* st, int32_t count) {
+    if (!s) return fail(SS_ERR_INVALID, "NULL handle");
+    if (!s->refn) return 0;
+    if (!st) return fail(SS_ERR_INVALID, "NULL argument");
+    SS_HIP_CHECK(hipSetDevice(s->p.device));
+    return refnoise_get(s->refn, st, count, s->stream);
+}
+
+int ss_set_rng_state(ss_sim* s, const uint64_t* st, int32_t count) {
+    if (!s) return fail(SS_ERR_INVALID, "NULL handle");
+    if (!s->refn) return count == 0 ? SS_OK : fail(SS_ERR_STATE, "noise_mode has no generator state");
+    if (count < 0 || (count > 0 && !st)) return fail(SS_ERR_INVALID, "bad rng state buffer");
+    SS_HIP_CHECK(hipSetDevice(s->p.device));
+    return refnoise_set(s->refn, st, count, s->stream);
+}
+
+int ss_get_noise(ss_sim* s, double* rv, uint64_t n) {
+    if (!s || !rv) return fail(SS_ERR_INVALID, "NULL argument");
+    if (s->p.noise_mode == SS_NOISE_PHILOX) return fail(SS_ERR_STATE, "Philox noise is drawn inside the update");
+    if (n != s->n) return fail(SS_ERR_INVALID, "noise length differs from number_of_particles");
+    if (!s->have_noise) return fail(SS_ERR_STATE, "no noise drawn yet");
+    SS_HIP_CHECK(hipSetDevice(s->p.device));
+    if (s->prec == SS_FP64)
+        hipLaunchKernelGGL(k_export_noise<double>, dim3(blocks_for(n)), dim3(256), 0, s->stream,
+                           (const double*)s->noise, n, s->staging);
+    else
+        hipLaunchKernelGGL(k_export_noise<float>, dim3(blocks_for(n)), dim3(256), 0, s->stream,
+                           (const float*)s->noise, n, s->staging);
+    SS_HIP_CHECK(hipGetLastError());
+    SS_HIP_CHECK(hipMemcpyAsync(rv, s->staging, sizeof(double) * 5 * n, hipMemcpyDeviceToHost, s->stream));
+    SS_HIP_CHECK(hipStreamSynchronize(s->stream));
     return SS_OK;
 }
 
@@ -1787,6 +1838,8 @@ int ss_ensemble_create(const ss_params* params, int nsys, ss_ensemble** out) {
         p.flags |= SS_FLAG_NO_GRAPH;
         if (p.precision != params[0].precision || p.noise_mode != params[0].noise_mode || p.device != params[0].device)
             return bail(fail(SS_ERR_INVALID, "ensemble members must share precision, noise mode and device"));
+        if (p.noise_mode == SS_NOISE_REFERENCE)
+            return bail(fail(SS_ERR_UNSUPPORTED, "ensembles draw Philox or external noise"));
         if (p.resort_interval != params[0].resort_interval)
             return bail(fail(SS_ERR_INVALID, "ensemble members must share the re-sort interval"));
         ss_sim* s = nullptr;

@@ -7,7 +7,7 @@ if the library is missing, every entry point raises.
 import ctypes as C
 import os
 
-from .params import ERRORS, SS_OK, Info, Params
+from .params import ERRORS, SS_OK, Box, Info, Params
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libss.so")
@@ -45,6 +45,9 @@ def _declare(L):
         "ss_get_magnetic_field": (C.c_int, [vp, _d, _d]),
         "ss_get_snapshot": (C.c_int, [vp, _d, _d, _P(C.c_uint64)]),
         "ss_resume": (C.c_int, [vp, _d, _d, C.c_uint64, C.c_uint64]),
+        "ss_get_rng_state": (C.c_int, [vp, _P(C.c_uint64), C.c_int32]),
+        "ss_set_rng_state": (C.c_int, [vp, _P(C.c_uint64), C.c_int32]),
+        "ss_get_noise": (C.c_int, [vp, _d, C.c_uint64]),
         "ss_profile_steps": (C.c_int, [vp, C.c_uint64, _d, C.c_char_p, C.c_int, C.c_int]),
         "ss_synchronize": (C.c_int, [vp]),
         "ss_get_diagnostics": (C.c_int, [vp, _d]),
@@ -54,6 +57,8 @@ def _declare(L):
         "ss_ensemble_member": (C.c_int, [vp, C.c_int, _P(vp)]),
         "ss_ensemble_step": (C.c_int, [vp, C.c_uint64, _P(C.c_uint64)]),
         "ss_initial_condition": (C.c_int, [pp, C.c_double, C.c_uint64, _d, C.c_uint64]),
+        "ss_create_isotropic": (C.c_int, [C.c_uint64, _P(Box), C.c_uint64, C.c_int32, _d]),
+        "ss_create_homogeneous": (C.c_int, [C.c_uint64, C.c_double, _P(Box), C.c_uint64, C.c_int32, _d]),
         "ss_fft_plan_create": (
             C.c_int,
             [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P(vp)],

@@ -10,6 +10,7 @@ FP64 = 0
 FP32 = 1
 NOISE_PHILOX = 0
 NOISE_EXTERNAL = 1
+NOISE_REFERENCE = 2
 BIN_AUTO = 0
 BIN_HISTOGRAM = 1
 BIN_SORTED = 2
@@ -76,6 +77,7 @@ class Params(C.Structure):
         ("device", C.c_int32),
         ("tile", C.c_int32 * 3),
         ("resort_interval", C.c_int32),
+        ("noise_streams", C.c_int32),
     ]
 
     @property
@@ -139,6 +141,7 @@ def make_params(
     device=0,
     tile=(0, 0, 0),
     resort_interval=0,
+    noise_streams=0,
 ):
     """Build a ``Params``; defaults are the small CPU-runnable configuration
     (L=10, 16^3 x 8 x 8, 2^16 particles, dt=1e-3, D_t=0.1, D_r=0.5, alpha=-1)."""
@@ -167,4 +170,5 @@ def make_params(
     p.device = device
     p.tile = (C.c_int32 * 3)(*tile)
     p.resort_interval = resort_interval
+    p.noise_streams = noise_streams
     return p

@@ -13,7 +13,7 @@ import ctypes as C
 import numpy as np
 
 from . import lib as _l
-from .params import Info, Params
+from .params import Box, Info, Params
 
 _P = C.POINTER
 
@@ -27,13 +27,16 @@ def _f64(a, shape=None):
 
 class Snapshot:
     """Snapshot (src/bin/simulation/mod.rs:80-86): particles, the low-order
-    position parts of the FP32 path, and the timestep (the counter-based noise
-    generator's whole state besides the seed)."""
+    position parts of the FP32 path, the timestep (with the seed, the whole
+    state of the counter-based Philox noise) and, for the reference's own
+    noise streams, ``rng_state``: (state, increment) per Pcg32 generator,
+    shape (T, 2) uint64 (None or empty otherwise)."""
 
-    def __init__(self, particles, compensation, timestep):
+    def __init__(self, particles, compensation, timestep, rng_state=None):
         self.particles = particles
         self.compensation = compensation
         self.timestep = timestep
+        self.rng_state = rng_state
 
 
 class Simulation:
@@ -79,18 +82,43 @@ class Simulation:
                 snapshot.timestep,
             )
         )
+        rs = getattr(snapshot, "rng_state", None)
+        if rs is not None and len(rs):
+            self.set_rng_state(rs)
 
     def get_snapshot(self):
         p = np.zeros((self.n, 5))
         comp = np.zeros((self.n, 3))
         t = C.c_uint64()
         _l.check(self.L.ss_get_snapshot(self._h, _l.dptr(p), _l.dptr(comp), C.byref(t)))
-        return Snapshot(p, comp, t.value)
+        return Snapshot(p, comp, t.value, self.get_rng_state())
+
+    def get_rng_state(self):
+        """(T, 2) uint64 (state, increment) of the reference-noise generators;
+        an empty array in the other noise modes."""
+        t = max(0, int(self.params.noise_streams))
+        buf = np.zeros((max(t, 1), 2), dtype=np.uint64)
+        rc = self.L.ss_get_rng_state(self._h, buf.ctypes.data_as(_P(C.c_uint64)), t)
+        _l.check(rc if rc < 0 else 0)
+        return buf[:rc].copy()
+
+    def set_rng_state(self, st):
+        """Replace the first min(len(st), T) generators (Simulation::resume's
+        zip, mod.rs:199-207)."""
+        a = np.ascontiguousarray(st, dtype=np.uint64).reshape(-1, 2)
+        _l.check(self.L.ss_set_rng_state(self._h, a.ctypes.data_as(_P(C.c_uint64)), a.shape[0]))
 
     # noise -------------------------------------------------------------------
     def set_noise(self, rv):
         r = _f64(rv)
         _l.check(self.L.ss_set_noise(self._h, _l.dptr(r), r.shape[0]))
+
+    def get_noise(self):
+        """RandomVectors of the last step (N x 5: x, y, z, axis_angle,
+        rotate_angle), external or reference noise."""
+        out = np.zeros((self.n, 5))
+        _l.check(self.L.ss_get_noise(self._h, _l.dptr(out), self.n))
+        return out
 
     # Simulation::do_timestep / Iterator ---------------------------------------
     def do_timestep(self):
@@ -254,6 +282,26 @@ def initial_condition(params, amplitude=0.05, seed=None, n=None):
     seed = params.seed if seed is None else seed
     out = np.zeros((n, 5))
     _l.check(L.ss_initial_condition(C.byref(params), amplitude, seed, _l.dptr(out), n))
+    return out
+
+
+def create_isotropic(n, box, seed, device=0):
+    """Particle::create_isotropic (src/particle.rs:224-255) from the
+    reference's Pcg64Mcg stream, drawn on the GPU: N x 5 [x, y, z, phi, theta]."""
+    L = _l.load()
+    out = np.zeros((n, 5))
+    b = Box(*box)
+    _l.check(L.ss_create_isotropic(n, C.byref(b), seed, device, _l.dptr(out)))
+    return out
+
+
+def create_homogeneous(n, kappa, box, seed, device=0):
+    """Particle::create_homogeneous (src/particle.rs:257-297): polar angle from
+    sin(theta) exp(kappa cos theta) about the y axis, same stream."""
+    L = _l.load()
+    out = np.zeros((n, 5))
+    b = Box(*box)
+    _l.check(L.ss_create_homogeneous(n, kappa, C.byref(b), seed, device, _l.dptr(out)))
     return out
 
 
