@@ -40,11 +40,11 @@ def make(prec, n, binning=None, **extra):
     )
 
 
-def time_gpu(prec, parts, steps, warmup, profile_steps, tune=False):
+def time_gpu(prec, parts, steps, warmup, profile_steps, tune=False, **extra):
     import stochasticsampling_amd as ss
     from stochasticsampling_amd import autotune
 
-    p = make(prec, parts.shape[0])
+    p = make(prec, parts.shape[0], **extra)
     if tune:  # tile shape and re-sort interval of the size class (shipped table)
         p = autotune.tuned(p)
     sim = ss.Simulation(p)
@@ -97,7 +97,7 @@ def fft_variant_study(parts, steps):
         sim.step(1)
         stages = sim.profile_steps(steps)
         sim.close()
-        spectral = sum(v for k, v in stages.items() if k not in ("update", "sort"))
+        spectral = sum(v for k, v in stages.items() if k not in ("noise", "update", "sort"))
         out[name] = {"ms_per_step": sum(stages.values()), "spectral_ms": spectral, "stages_ms": stages,
                      "u_rel_err_vs_fp64": eu, "grad_u_rel_err_vs_fp64": eg}
     del dist
@@ -218,6 +218,9 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--no-fft-study", action="store_true")
     ap.add_argument("--fft-study-steps", type=int, default=32)
+    ap.add_argument("--no-refnoise", action="store_true")
+    ap.add_argument("--refnoise-steps", type=int, default=200)
+    ap.add_argument("--refnoise-streams", type=int, default=64)
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -251,6 +254,17 @@ def main():
     if not args.no_fft_study:
         print("[bench] FFT variants", file=sys.stderr, flush=True)
         fftv = fft_variant_study(parts, args.fft_study_steps)
+
+    refn = None
+    if not args.no_refnoise:
+        # the reference's own noise stream (Pcg32 per rayon thread, ziggurat
+        # normals) decoded on the GPU, instead of in-kernel Philox
+        print(f"[bench] FP32 with the reference's noise stream ({args.refnoise_streams} streams)",
+              file=sys.stderr, flush=True)
+        msr, str_ = time_gpu(ss.FP32, parts, args.refnoise_steps, min(args.warmup, 10), min(args.profile_steps, 20),
+                             tune=True, noise_mode=ss.NOISE_REFERENCE, noise_streams=args.refnoise_streams)
+        refn = {"value": n / (msr * 1e-3), "ms_per_step": msr, "steps": args.refnoise_steps,
+                "streams": args.refnoise_streams, "noise_ms": str_.get("noise"), "stages_ms": str_}
 
     ens = None
     if not args.no_ensemble:
@@ -316,6 +330,7 @@ def main():
         "fp64_plain": fp64,
         "ensemble": ens,
         "fft_variants": fftv,
+        "reference_noise": refn,
         "stages_ms": st32,
     }
     if fp64:

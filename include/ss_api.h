@@ -210,7 +210,8 @@ int ss_get_noise(ss_sim* sim, double* rv, uint64_t n);
 
 /* Per-stage device time of the last ss_profile_steps call, measured with HIP
    events on the simulation's stream.  Stage names: "bin", "fft_forward",
-   "solve", "fft_inverse", "pack", "update", "sort".  Returns the number of
+   "solve", "fft_inverse", "pack", "noise" (SS_NOISE_REFERENCE draws),
+   "update", "sort".  Returns the number of
    stages written (at most max_stages); ms_per_step[i] is the mean per step. */
 int ss_profile_steps(ss_sim* sim, uint64_t nsteps, double* ms_per_step, char* names,
                      int name_stride, int max_stages);

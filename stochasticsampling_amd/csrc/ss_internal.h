@@ -65,8 +65,11 @@ struct RefNoise;
 int refnoise_create(RefNoise** out, uint64_t n, int streams, uint64_t seed);
 void refnoise_destroy(RefNoise* r);
 int refnoise_streams(const RefNoise* r);
-// Enqueue one step's RandomVectors into noise ([5][n], double or float).
-hipError_t refnoise_enqueue(RefNoise* r, void* noise, bool fp64, double dr, hipStream_t stream);
+// Enqueue one step's RandomVectors into noise: a record of 8 values (the
+// first five used; double or float) at slot inv[id] of each particle id
+// (inv null: at slot id).
+hipError_t refnoise_enqueue(RefNoise* r, void* noise, bool fp64, double dr, const uint32_t* inv,
+                            hipStream_t stream);
 int refnoise_get(RefNoise* r, uint64_t* state_increment, int count, hipStream_t stream);
 int refnoise_set(RefNoise* r, const uint64_t* state_increment, int count, hipStream_t stream);
 

@@ -413,20 +413,48 @@ __device__ inline void move_particle(const State<T>& s, uint64_t i, const Geo<T>
     move_loaded(s, i, q, P, rec + (size_t)c * rs, rv, nx, ny, nz, ophi, otheta);
 }
 
+// Noise sources of the update: Philox drawn in the kernel; caller-supplied
+// RandomVectors in five component arrays indexed by particle id; or records
+// of NOISE_REC values (the first five used) at each particle's storage slot,
+// which the reference-stream generator writes so that the update reads them
+// in order, 32 (FP32) or 64 (FP64) bytes per particle.
+enum { NZ_PHILOX = 0, NZ_EXT = 1, NZ_REC = 2 };
+constexpr int NOISE_REC = 8;
+
 template <typename T>
+struct NoiseRec {
+    T v[5];
+};
+
+__device__ inline NoiseRec<float> load_noise_rec(const float* __restrict__ noise, uint32_t id) {
+    const float4* p = reinterpret_cast<const float4*>(noise + (size_t)id * NOISE_REC);
+    const float4 a = p[0], b = p[1];
+    return NoiseRec<float>{{a.x, a.y, a.z, a.w, b.x}};
+}
+
+__device__ inline NoiseRec<double> load_noise_rec(const double* __restrict__ noise, uint32_t id) {
+    const double2* p = reinterpret_cast<const double2*>(noise + (size_t)id * NOISE_REC);
+    const double2 a = p[0], b = p[1], c = p[2];
+    return NoiseRec<double>{{a.x, a.y, b.x, b.y, c.x}};
+}
+
+template <typename T, int NZ>
 __device__ inline void draw_noise(const UpdateParams<T>& P, uint64_t seed, uint64_t step, uint32_t id,
-                                  uint64_t n, const T* __restrict__ noise, bool ext, Noise<T>& rv) {
-    if (ext) {
+                                  uint64_t n, const T* __restrict__ noise, Noise<T>& rv) {
+    if constexpr (NZ == NZ_EXT) {
         T v[5];
 #pragma unroll
         for (int k = 0; k < 5; ++k) v[k] = noise[(size_t)k * n + id];
         noise_from_vector<T>(v, rv);
+    } else if constexpr (NZ == NZ_REC) {
+        const NoiseRec<T> r = load_noise_rec(noise, id);
+        noise_from_vector<T>(r.v, rv);
     } else {
         philox_noise(seed, step, id, P.dr, rv);
     }
 }
 
-template <typename T, bool EXT_NOISE>
+template <typename T, int NZ>
 __global__ void __launch_bounds__(256)
     k_update_plain(State<T> s, uint64_t n, Geo<T> g, UpdateParams<T> P, const T* __restrict__ rec, int rs,
                    uint64_t seed, const uint64_t* __restrict__ step, uint32_t step_off,
@@ -434,7 +462,7 @@ __global__ void __launch_bounds__(256)
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Noise<T> rv;
-    draw_noise<T>(P, seed, *step + step_off, (uint32_t)i, n, noise, EXT_NOISE, rv);
+    draw_noise<T, NZ>(P, seed, *step + step_off, (uint32_t)i, n, noise, rv);
     T nx, ny, nz, ophi, otheta;
     move_particle<T>(s, i, g, P, rec, rs, rv, nx, ny, nz, ophi, otheta);
 }
@@ -607,12 +635,23 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// stride: distance between a particle's components (n: component arrays,
+// 1: records); rec: values per record (1 for component arrays); inv (may be
+// null): slot of each particle id.
 template <typename T>
-__global__ void __launch_bounds__(256) k_export_noise(const T* __restrict__ noise, uint64_t n,
+__global__ void __launch_bounds__(256) k_export_noise(const T* __restrict__ noise, uint64_t n, uint64_t stride,
+                                                      int rec, const uint32_t* __restrict__ inv,
                                                       double* __restrict__ aos) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    for (int k = 0; k < 5; ++k) aos[5 * i + k] = (double)noise[(size_t)k * n + i];
+    const uint64_t slot = inv ? inv[i] : i;
+    for (int k = 0; k < 5; ++k) aos[5 * i + k] = (double)noise[(size_t)k * stride + slot * (uint64_t)rec];
+}
+
+__global__ void __launch_bounds__(256) k_inverse_perm(const uint32_t* __restrict__ id, uint64_t n,
+                                                      uint32_t* __restrict__ inv) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) inv[id[i]] = (uint32_t)i;
 }
 
 template <typename T>

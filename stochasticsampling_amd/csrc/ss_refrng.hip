@@ -222,7 +222,14 @@ __device__ inline void rn_normal(const RnGen& G, const RnRing& R, const RnTab& T
             return;
         }
         const double g = rn_gen_f64(rn_draw(G, R, pos + k + 1));
-        if (T.f[i + 1] + (T.f[i] - T.f[i + 1]) * g < exp(-x * x / 2.0)) {
+        const double lhs = T.f[i + 1] + (T.f[i] - T.f[i + 1]) * g;
+        const double arg = -x * x / 2.0;  // in [-6.7, 0]: x is below x_1 here
+        // exp(arg) in single precision (relative error below 2e-6) decides
+        // unless lhs lies that close to it; then the double exp does
+        const float ef = __expf((float)arg);
+        const double dd = lhs - (double)ef;
+        const bool accept = fabs(dd) > 1e-5 * (double)ef ? dd < 0.0 : lhs < exp(arg);
+        if (accept) {
             len = k + 2;
             val = x;
             return;
@@ -263,7 +270,8 @@ __device__ inline void rn_len_at(const RnGen& G, const RnRing& R, const RnTab& T
 enum { RN_WALK = 0, RN_RESOLVE = 1, RN_WRITE = 2 };
 
 struct RnOut {
-    void* noise;  // [5][n] of T
+    void* noise;  // n records of 8 values of T
+    const uint32_t* inv;  // slot of each particle id (null: the id)
     uint64_t n;
     double dr;
     int fp64;
@@ -276,13 +284,18 @@ struct RnResult {
     uint64_t kend;   // WRITE: draw after the chunk's last particle (~0 if not reached)
 };
 
-__device__ inline void rn_store(const RnOut& o, uint64_t pid, const double v[5]) {
+// One particle's RandomVector as a record of 8 values (the update reads it
+// with 16-byte loads).
+__device__ inline void rn_store(const RnOut& o, uint64_t slot, const double v[5]) {
     if (o.fp64) {
-        double* p = (double*)o.noise;
-        for (int k = 0; k < 5; ++k) p[(size_t)k * o.n + pid] = v[k];
+        double2* p = reinterpret_cast<double2*>((double*)o.noise + slot * 8);
+        p[0] = make_double2(v[0], v[1]);
+        p[1] = make_double2(v[2], v[3]);
+        p[2] = make_double2(v[4], 0.0);
     } else {
-        float* p = (float*)o.noise;
-        for (int k = 0; k < 5; ++k) p[(size_t)k * o.n + pid] = (float)v[k];
+        float4* p = reinterpret_cast<float4*>((float*)o.noise + slot * 8);
+        p[0] = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+        p[1] = make_float4((float)v[4], 0.f, 0.f, 0.f);
     }
 }
 
@@ -290,14 +303,70 @@ __device__ inline void rn_store(const RnOut& o, uint64_t pid, const double v[5])
 // WALK: set their bits in `bitmap` (window at wbase).  RESOLVE: stop at the
 // first start whose bit is set.  WRITE: write particles base, base+1, ...
 // below m of the chunk whose first particle is pbegin.
+// Particle starts of one block from entry e (offset of the first start in
+// the block; 64 or more: none): lane k < 16 gets the offset of start k, or 64
+// once the chain has left the block.  rl: length of the particle that would
+// start at each lane's draw.  Pointer doubling over the 64 lanes: j_b jumps
+// 2^b particles, and lane k composes the jumps of the bits of k.
+__device__ inline uint32_t rn_chain(uint32_t rl, uint32_t e, int lane) {
+    const uint32_t j0 = min((uint32_t)lane + rl, 64u);
+    uint32_t t = (uint32_t)__shfl((int)j0, (int)(j0 & 63));
+    const uint32_t j1 = j0 >= 64 ? 64u : t;
+    t = (uint32_t)__shfl((int)j1, (int)(j1 & 63));
+    const uint32_t j2 = j1 >= 64 ? 64u : t;
+    t = (uint32_t)__shfl((int)j2, (int)(j2 & 63));
+    const uint32_t j3 = j2 >= 64 ? 64u : t;
+    uint32_t p = min(e, 64u);
+    t = (uint32_t)__shfl((int)j0, (int)(p & 63));
+    if (lane & 1) p = p >= 64 ? 64u : t;
+    t = (uint32_t)__shfl((int)j1, (int)(p & 63));
+    if (lane & 2) p = p >= 64 ? 64u : t;
+    t = (uint32_t)__shfl((int)j2, (int)(p & 63));
+    if (lane & 4) p = p >= 64 ? 64u : t;
+    t = (uint32_t)__shfl((int)j3, (int)(p & 63));
+    if (lane & 8) p = p >= 64 ? 64u : t;
+    return p;
+}
+
+// Write queue of a wavefront (WRITE): particles wait here until 64 are ready,
+// so that the rotate angle's log runs on full wavefronts and the records of
+// consecutive particles are stored together.
+constexpr uint32_t RN_QCAP = 80;  // 63 waiting + at most 13 starts per block
+struct RnQueue {
+    uint32_t* pid;  // [RN_QCAP]
+    double* v;      // [4][RN_QCAP]: three normals and the axis angle
+    double* u;      // [RN_QCAP]: uniform of the rotate angle
+};
+
+__device__ inline void rn_flush(const RnQueue& Q, uint32_t nq, int lane, const RnOut& out) {
+    if ((uint32_t)lane < nq) {
+        double v[5];
+        v[0] = Q.v[0 * RN_QCAP + lane];
+        v[1] = Q.v[1 * RN_QCAP + lane];
+        v[2] = Q.v[2 * RN_QCAP + lane];
+        v[3] = Q.v[3 * RN_QCAP + lane];
+        v[4] = out.dr * sqrt(-2. * log(1. - Q.u[lane]));  // rayleigh_pdf (mod.rs:383-385)
+        const uint32_t pid = Q.pid[lane];
+        rn_store(out, out.inv ? out.inv[pid] : pid, v);
+    }
+}
+
+// Decode particles from draw `start` while they start before `stop` (a
+// block boundary, or ~0).  WALK: record their starts in `bitmap` (window at
+// wbase).  RESOLVE: stop at the first start whose bit is set.  WRITE: write
+// particles base, base+1, ... below m of the chunk whose first particle is
+// pbegin.
 template <int MODE>
-__device__ RnResult rn_decode(RnGen& G, RnRing& R, const RnTab& T, uint64_t start, uint64_t stop, int lane,
-                              uint32_t* bitmap, uint64_t wbase, uint64_t base, uint64_t m, uint64_t pbegin,
+__device__ RnResult rn_decode(RnGen& G, RnRing& R, const RnQueue& Q, const RnTab& T, uint64_t start, uint64_t stop,
+                              int lane, uint32_t* bitmap, uint64_t wbase, uint64_t base, uint64_t m, uint64_t pbegin,
                               const RnOut& out) {
     constexpr bool VAL = MODE == RN_WRITE;
     RnResult res;
     res.merged = 0;
     res.kend = ~0ull;
+    res.s = start;
+    res.cnt = 0;
+    if (start >= stop) return res;
     uint64_t P = start & ~63ull;
     // lane generators at block P
     {
@@ -314,9 +383,11 @@ __device__ RnResult rn_decode(RnGen& G, RnRing& R, const RnTab& T, uint64_t star
     rn_class_block<VAL>(G, R, T, P + 64, lane);
     uint64_t cls_end = P + 128;
     __builtin_amdgcn_wave_barrier();
-    uint64_t s = start, cnt = 0;
-    bool done = s >= stop;
-    while (!done) {
+    uint32_t e = (uint32_t)(start - P);
+    uint64_t cnt = 0;
+    uint32_t qn = 0;
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;  // lanes below this one
+    for (;;) {
         // length of the particle that would start at this lane's draw
         const uint64_t q = P + (uint64_t)lane;
         uint32_t n1, n2, n3;
@@ -325,57 +396,74 @@ __device__ RnResult rn_decode(RnGen& G, RnRing& R, const RnTab& T, uint64_t star
         rn_len_at<VAL>(G, R, T, q + n1, cls_end, n2, v2);
         rn_len_at<VAL>(G, R, T, q + n1 + n2, cls_end, n3, v3);
         const uint32_t rl = n1 + n2 + n3 + 2;
-        uint64_t bm = 0;
-        if (MODE == RN_RESOLVE && P < stop) {
+        const uint32_t pk = rn_chain(rl, e, lane);
+        uint32_t count = (uint32_t)__popcll(__ballot(lane < 16 && pk < 64));
+        uint32_t enext;
+        if (count == 0) {
+            enext = e - 64;
+        } else {
+            const uint32_t last = (uint32_t)__builtin_amdgcn_readlane((int)pk, (int)count - 1);
+            enext = last + (uint32_t)__builtin_amdgcn_readlane((int)rl, (int)last) - 64;
+        }
+        bool done = false;
+        if (MODE == RN_RESOLVE) {
             const uint64_t o = (P - wbase) / 32;
-            bm = (uint64_t)bitmap[o] | ((uint64_t)bitmap[o + 1] << 32);
-        }
-        // scalar walk over the block
-        uint64_t mask = 0;
-        const uint64_t cnt0 = cnt;
-        while (s < P + 64 && s < stop) {
-            const uint32_t L = (uint32_t)__builtin_amdgcn_readlane((int)rl, (int)(s - P));
-            if (MODE == RN_RESOLVE && ((bm >> (s - P)) & 1ull)) {
+            const uint64_t bm = (uint64_t)bitmap[o] | ((uint64_t)bitmap[o + 1] << 32);
+            const uint64_t hit = __ballot((uint32_t)lane < count && ((bm >> (pk & 63)) & 1ull));
+            if (hit) {
+                const uint32_t k = (uint32_t)__ffsll((unsigned long long)hit) - 1;
                 res.merged = 1;
-                done = true;
-                break;
+                res.s = P + (uint32_t)__builtin_amdgcn_readlane((int)pk, (int)k);
+                res.cnt = cnt + k;
+                return res;
             }
-            if (MODE == RN_WRITE) {
-                if (base + cnt >= m) {
-                    done = true;
-                    break;
-                }
-                if (base + cnt == m - 1) res.kend = s + L;
-            }
-            mask |= 1ull << (s - P);
-            cnt += 1;
-            s += L;
         }
-        if (MODE == RN_WALK && P < stop && lane == 0) {
+        if (MODE == RN_WRITE && base + cnt + count >= m) {
+            const uint32_t kl = (uint32_t)(m - 1 - (base + cnt));
+            const uint32_t pl = (uint32_t)__builtin_amdgcn_readlane((int)pk, (int)kl);
+            res.kend = P + pl + (uint32_t)__builtin_amdgcn_readlane((int)rl, (int)pl);
+            count = kl + 1;
+            done = true;
+        }
+        // this block's starts as a bit mask
+        uint64_t bits = (uint32_t)lane < count ? (1ull << (pk & 63)) : 0ull;
+        for (int off = 8; off > 0; off >>= 1) bits |= __shfl_xor(bits, off);
+        const uint64_t mask = __shfl(bits, 0);
+        if (MODE == RN_WALK && lane == 0) {
             const uint64_t o = (P - wbase) / 32;
             bitmap[o] = (uint32_t)mask;
             bitmap[o + 1] = (uint32_t)(mask >> 32);
         }
-        if (MODE == RN_WALK && s >= stop) {
-            // blocks a particle longer than a block skipped: no starts there
-            for (uint64_t o = (P + 64 - wbase) / 32 + (uint64_t)lane; o < RN_WIN / 32; o += 64) bitmap[o] = 0u;
+        if (MODE == RN_WRITE) {
+            if ((mask >> lane) & 1ull) {
+                const uint32_t slot = qn + (uint32_t)__popcll(mask & lt);
+                const uint64_t up = q + n1 + n2 + n3;
+                Q.pid[slot] = (uint32_t)(pbegin + base + cnt + (slot - qn));
+                Q.v[0 * RN_QCAP + slot] = v1;
+                Q.v[1 * RN_QCAP + slot] = v2;
+                Q.v[2 * RN_QCAP + slot] = v3;
+                Q.v[3 * RN_QCAP + slot] = RN_TWOPI * rn_uniform01(rn_draw(G, R, up));
+                Q.u[slot] = rn_uniform01(rn_draw(G, R, up + 1));
+            }
+            qn += count;
+            __builtin_amdgcn_wave_barrier();
+            if (qn >= 64) {
+                rn_flush(Q, 64, lane, out);
+                if ((uint32_t)lane < qn - 64) {
+                    Q.pid[lane] = Q.pid[64 + lane];
+                    for (int k = 0; k < 4; ++k) Q.v[k * RN_QCAP + lane] = Q.v[k * RN_QCAP + 64 + lane];
+                    Q.u[lane] = Q.u[64 + lane];
+                }
+                qn -= 64;
+                __builtin_amdgcn_wave_barrier();
+            }
         }
-        if (MODE == RN_WRITE && ((mask >> lane) & 1ull)) {
-            const uint64_t below = lane ? (mask & (~0ull >> (64 - lane))) : 0ull;
-            const uint64_t idx = base + cnt0 + (uint64_t)__popcll(below);
-            const uint64_t up = q + n1 + n2 + n3;
-            double v[5];
-            v[0] = v1;
-            v[1] = v2;
-            v[2] = v3;
-            v[3] = RN_TWOPI * rn_uniform01(rn_draw(G, R, up));
-            v[4] = out.dr * sqrt(-2. * log(1. - rn_uniform01(rn_draw(G, R, up + 1))));
-            rn_store(out, pbegin + idx, v);
-        }
-        if (s >= stop) done = true;
-        if (done) break;
+        cnt += count;
+        res.s = P + 64 + enext;
+        if (done || P + 64 >= stop) break;
         // next block: generate three blocks ahead, classify two ahead
         P += 64;
+        e = enext;
         __builtin_amdgcn_wave_barrier();
         rn_gen_block(G, R, lane);
         __builtin_amdgcn_wave_barrier();
@@ -383,7 +471,7 @@ __device__ RnResult rn_decode(RnGen& G, RnRing& R, const RnTab& T, uint64_t star
         cls_end = P + 128;
         __builtin_amdgcn_wave_barrier();
     }
-    res.s = s;
+    if (MODE == RN_WRITE && qn) rn_flush(Q, qn, lane, out);
     res.cnt = cnt;
     return res;
 }
@@ -402,14 +490,19 @@ __device__ inline uint32_t rn_popcount_from(const uint32_t* bitmap, uint64_t o, 
     return c;
 }
 
+template <bool WRITE>
 struct RnShared {
     RnTab tab;
     uint64_t d[RN_WAVES][RN_RING];
     uint8_t nl[RN_WAVES][RN_RING];
-    double nv[RN_WAVES][RN_RING];
+    double nv[RN_WAVES][WRITE ? RN_RING : 1];
+    uint32_t qpid[RN_WAVES][WRITE ? RN_QCAP : 1];
+    double qv[RN_WAVES][WRITE ? 4 * RN_QCAP : 1];
+    double qu[RN_WAVES][WRITE ? RN_QCAP : 1];
 };
 
-__device__ inline void rn_setup(RnShared& sh, const RnTab* tab, RnRing& R, int wave) {
+template <bool WRITE>
+__device__ inline void rn_setup(RnShared<WRITE>& sh, const RnTab* tab, RnRing& R, RnQueue& Q, int wave) {
     for (int i = threadIdx.x; i < 257; i += blockDim.x) {
         sh.tab.x[i] = tab->x[i];
         sh.tab.f[i] = tab->f[i];
@@ -417,6 +510,9 @@ __device__ inline void rn_setup(RnShared& sh, const RnTab* tab, RnRing& R, int w
     R.d = sh.d[wave];
     R.nl = sh.nl[wave];
     R.nv = sh.nv[wave];
+    Q.pid = sh.qpid[wave];
+    Q.v = sh.qv[wave];
+    Q.u = sh.qu[wave];
     __syncthreads();
 }
 
@@ -427,13 +523,13 @@ __device__ inline void rn_gen_init(RnGen& G, const RnChunk& c) {
 }
 
 // Resolution of window j of chunk c from entry t; lane 0 stores the result.
-__device__ inline void rn_resolve_one(RnGen& G, RnRing& R, const RnTab& T, const RnChunk& c, RnWin* win,
-                                      const uint32_t* bitmap, uint64_t w, uint64_t t, int lane) {
+__device__ inline void rn_resolve_one(RnGen& G, RnRing& R, const RnQueue& Q, const RnTab& T, const RnChunk& c,
+                                      RnWin* win, const uint32_t* bitmap, uint64_t w, uint64_t t, int lane) {
     const uint64_t j = w - c.w0;
     const uint64_t wbase = j * RN_WIN, stop = wbase + RN_WIN;
     const uint32_t* bm = bitmap + w * (RN_WIN / 32);
-    RnOut none{nullptr, 0, 0.0, 0};
-    RnResult r = rn_decode<RN_RESOLVE>(G, R, T, t, stop, lane, const_cast<uint32_t*>(bm), wbase, 0, 0, 0, none);
+    RnOut none{nullptr, nullptr, 0, 0.0, 0};
+    RnResult r = rn_decode<RN_RESOLVE>(G, R, Q, T, t, stop, lane, const_cast<uint32_t*>(bm), wbase, 0, 0, 0, none);
     uint32_t tcnt = (uint32_t)r.cnt;
     if (r.merged) tcnt += rn_popcount_from(bm, r.s - wbase, lane);
     if (lane == 0) {
@@ -451,18 +547,19 @@ __device__ inline uint64_t rn_expected(const RnWin* win, uint64_t w) {
 __global__ void __launch_bounds__(64 * RN_WAVES)
     k_rn_walk(const RnChunk* __restrict__ ch, RnWin* __restrict__ win, const uint32_t* __restrict__ win_chunk,
               uint32_t* __restrict__ bitmap, const RnTab* __restrict__ tab, uint64_t nwin) {
-    __shared__ RnShared sh;
+    __shared__ RnShared<false> sh;
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     RnRing R;
-    rn_setup(sh, tab, R, wave);
+    RnQueue Q;
+    rn_setup(sh, tab, R, Q, wave);
     const uint64_t w = (uint64_t)blockIdx.x * RN_WAVES + wave;
     if (w >= nwin) return;
     const RnChunk c = ch[win_chunk[w]];
     RnGen G;
     rn_gen_init(G, c);
     const uint64_t j = w - c.w0, wbase = j * RN_WIN;
-    RnOut none{nullptr, 0, 0.0, 0};
-    RnResult r = rn_decode<RN_WALK>(G, R, sh.tab, wbase, wbase + RN_WIN, lane, bitmap + w * (RN_WIN / 32), wbase,
+    RnOut none{nullptr, nullptr, 0, 0.0, 0};
+    RnResult r = rn_decode<RN_WALK>(G, R, Q, sh.tab, wbase, wbase + RN_WIN, lane, bitmap + w * (RN_WIN / 32), wbase,
                                     0, 0, 0, none);
     if (lane == 0) {
         win[w].sexit = r.s;
@@ -478,10 +575,11 @@ __global__ void __launch_bounds__(64 * RN_WAVES)
 __global__ void __launch_bounds__(64 * RN_WAVES)
     k_rn_resolve(const RnChunk* __restrict__ ch, RnWin* __restrict__ win, const uint32_t* __restrict__ win_chunk,
                  const uint32_t* __restrict__ bitmap, const RnTab* __restrict__ tab, uint64_t nwin, int first) {
-    __shared__ RnShared sh;
+    __shared__ RnShared<false> sh;
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     RnRing R;
-    rn_setup(sh, tab, R, wave);
+    RnQueue Q;
+    rn_setup(sh, tab, R, Q, wave);
     const uint64_t w = (uint64_t)blockIdx.x * RN_WAVES + wave;
     if (w >= nwin) return;
     const RnChunk c = ch[win_chunk[w]];
@@ -494,7 +592,7 @@ __global__ void __launch_bounds__(64 * RN_WAVES)
     }
     RnGen G;
     rn_gen_init(G, c);
-    rn_resolve_one(G, R, sh.tab, c, win, bitmap, w, t, lane);
+    rn_resolve_one(G, R, Q, sh.tab, c, win, bitmap, w, t, lane);
 }
 
 __global__ void __launch_bounds__(256)
@@ -519,10 +617,11 @@ __global__ void __launch_bounds__(256)
 __global__ void __launch_bounds__(64 * RN_WAVES)
     k_rn_finish(RnChunk* __restrict__ ch, RnWin* __restrict__ win, const uint32_t* __restrict__ bitmap,
                 const RnTab* __restrict__ tab, int nchunks, RnOut out) {
-    __shared__ RnShared sh;
+    __shared__ RnShared<true> sh;
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     RnRing R;
-    rn_setup(sh, tab, R, wave);
+    RnQueue Q;
+    rn_setup(sh, tab, R, Q, wave);
     const int cidx = blockIdx.x * RN_WAVES + wave;
     if (cidx >= nchunks) return;
     const RnChunk c = ch[cidx];
@@ -548,7 +647,7 @@ __global__ void __launch_bounds__(64 * RN_WAVES)
         const uint64_t w = c.w0 + bad;
         const uint64_t t = vw[w - 1].merged ? vw[w - 1].sexit : vw[w - 1].texit;
         __builtin_amdgcn_wave_barrier();
-        rn_resolve_one(G, R, sh.tab, c, win, bitmap, w, t, lane);
+        rn_resolve_one(G, R, Q, sh.tab, c, win, bitmap, w, t, lane);
         __threadfence_block();
         jlo = bad + 1;
     }
@@ -569,7 +668,7 @@ __global__ void __launch_bounds__(64 * RN_WAVES)
     if (run < c.m) {  // more particles than the windows cover: decode on
         const uint64_t w = c.w0 + c.nw - 1;
         const uint64_t t = vw[w].merged ? vw[w].sexit : vw[w].texit;
-        RnResult r = rn_decode<RN_WRITE>(G, R, sh.tab, t, ~0ull, lane, nullptr, 0, run, c.m, c.begin, out);
+        RnResult r = rn_decode<RN_WRITE>(G, R, Q, sh.tab, t, ~0ull, lane, nullptr, 0, run, c.m, c.begin, out);
         kend = r.kend;
     }
     if (lane == 0) ch[cidx].kend = kend;
@@ -578,10 +677,11 @@ __global__ void __launch_bounds__(64 * RN_WAVES)
 __global__ void __launch_bounds__(64 * RN_WAVES)
     k_rn_write(RnChunk* __restrict__ ch, const RnWin* __restrict__ win, const uint32_t* __restrict__ win_chunk,
                const RnTab* __restrict__ tab, uint64_t nwin, RnOut out) {
-    __shared__ RnShared sh;
+    __shared__ RnShared<true> sh;
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     RnRing R;
-    rn_setup(sh, tab, R, wave);
+    RnQueue Q;
+    rn_setup(sh, tab, R, Q, wave);
     const uint64_t w = (uint64_t)blockIdx.x * RN_WAVES + wave;
     if (w >= nwin) return;
     const uint32_t ci = win_chunk[w];
@@ -590,7 +690,7 @@ __global__ void __launch_bounds__(64 * RN_WAVES)
     RnGen G;
     rn_gen_init(G, c);
     const uint64_t j = w - c.w0, stop = (j + 1) * RN_WIN;
-    RnResult r = rn_decode<RN_WRITE>(G, R, sh.tab, win[w].tentry, stop, lane, nullptr, 0, win[w].base, c.m,
+    RnResult r = rn_decode<RN_WRITE>(G, R, Q, sh.tab, win[w].tentry, stop, lane, nullptr, 0, win[w].base, c.m,
                                      c.begin, out);
     if (r.kend != ~0ull && lane == 0) ch[ci].kend = r.kend;
 }
@@ -697,11 +797,12 @@ void refnoise_destroy(RefNoise* r) {
 
 int refnoise_streams(const RefNoise* r) { return r ? r->streams : 0; }
 
-hipError_t refnoise_enqueue(RefNoise* r, void* noise, bool fp64, double dr, hipStream_t stream) {
+hipError_t refnoise_enqueue(RefNoise* r, void* noise, bool fp64, double dr, const uint32_t* inv,
+                            hipStream_t stream) {
     if (r->nchunks == 0) return hipSuccess;
     const dim3 wblk(64 * RN_WAVES);
     const dim3 wgrid((unsigned)((r->nwin + RN_WAVES - 1) / RN_WAVES));
-    RnOut out{noise, r->n, dr, fp64 ? 1 : 0};
+    RnOut out{noise, inv, r->n, dr, fp64 ? 1 : 0};
     hipLaunchKernelGGL(k_rn_walk, wgrid, wblk, 0, stream, r->ch, r->win, r->win_chunk, r->bitmap, r->tab, r->nwin);
     hipLaunchKernelGGL(k_rn_resolve, wgrid, wblk, 0, stream, r->ch, r->win, r->win_chunk, r->bitmap, r->tab,
                        r->nwin, 1);

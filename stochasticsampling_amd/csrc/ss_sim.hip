@@ -175,6 +175,10 @@ struct ss_sim {
     void *x2 = nullptr, *y2 = nullptr, *z2 = nullptr, *phi2 = nullptr, *theta2 = nullptr;
     float *lx = nullptr, *ly = nullptr, *lz = nullptr, *lx2 = nullptr, *ly2 = nullptr, *lz2 = nullptr;
     uint32_t *id = nullptr, *id2 = nullptr;
+    // SS_NOISE_REFERENCE on the tiled path: slot of each particle id in the
+    // current state, so that the noise records follow the storage order
+    uint32_t* inv = nullptr;
+    bool last_resort = false;  // the last step's update changed the storage order
     // tiled path
     TileGeo tg;
     int resort = 16;             // steps between tile sorts
@@ -422,27 +426,29 @@ static hipError_t launch_spec_planes_inv(ss_sim* s) {
     return hipGetLastError();
 }
 
-template <typename T, int NM, bool MOVE, bool EXT, bool KEEP>
+template <typename T, int NM, bool MOVE, int NZ, bool KEEP>
 static void launch_tile_kernel(ss_sim* s, const TileArgs<T, NM>& a) {
     const int per = (s->tg.ntiles + s->tg.nxcd - 1) / s->tg.nxcd;
     const dim3 grid((unsigned)(per * s->tg.nxcd));
     const size_t lds = tile_lds_bytes(s->tg.ws, NM, s->na);
     if (MOVE && a.dtoff)
-        hipLaunchKernelGGL((k_tile_update<T, NM, MOVE, EXT, KEEP, MOVE>), grid, dim3(256), lds, s->stream, a);
+        hipLaunchKernelGGL((k_tile_update<T, NM, MOVE, NZ, KEEP, MOVE>), grid, dim3(256), lds, s->stream, a);
     else
-        hipLaunchKernelGGL((k_tile_update<T, NM, MOVE, EXT, KEEP, false>), grid, dim3(256), lds, s->stream, a);
+        hipLaunchKernelGGL((k_tile_update<T, NM, MOVE, NZ, KEEP, false>), grid, dim3(256), lds, s->stream, a);
 }
 
 template <typename T, int NM, bool MOVE>
 static void launch_tile_nm(ss_sim* s, const TileArgs<T, NM>& a) {
-    const bool ext = s->p.noise_mode != SS_NOISE_PHILOX;
     const bool keep = a.hist != nullptr;
-    if (ext) {
-        if (keep) launch_tile_kernel<T, NM, MOVE, true, true>(s, a);
-        else launch_tile_kernel<T, NM, MOVE, true, false>(s, a);
+    if (s->p.noise_mode == SS_NOISE_EXTERNAL) {
+        if (keep) launch_tile_kernel<T, NM, MOVE, NZ_EXT, true>(s, a);
+        else launch_tile_kernel<T, NM, MOVE, NZ_EXT, false>(s, a);
+    } else if (s->p.noise_mode == SS_NOISE_REFERENCE) {
+        if (keep) launch_tile_kernel<T, NM, MOVE, NZ_REC, true>(s, a);
+        else launch_tile_kernel<T, NM, MOVE, NZ_REC, false>(s, a);
     } else {
-        if (keep) launch_tile_kernel<T, NM, MOVE, false, true>(s, a);
-        else launch_tile_kernel<T, NM, MOVE, false, false>(s, a);
+        if (keep) launch_tile_kernel<T, NM, MOVE, NZ_PHILOX, true>(s, a);
+        else launch_tile_kernel<T, NM, MOVE, NZ_PHILOX, false>(s, a);
     }
 }
 
@@ -478,6 +484,13 @@ static void swap_sorted(ss_sim* s) {
     s->pending = false;
 }
 
+// inv[id[slot]] = slot for the current state (after every change of order).
+static void launch_inverse_perm(ss_sim* s) {
+    if (!s->inv) return;
+    hipLaunchKernelGGL(k_inverse_perm, dim3(blocks_for(s->n)), dim3(256), 0, s->stream, (const uint32_t*)s->id, s->n,
+                       s->inv);
+}
+
 // Tiled path: move every particle and bin the new state (MOVE), or only bin
 // the current state (initial sampling).  A pending re-sort makes the move
 // write state B in the new order, which then becomes current.
@@ -491,7 +504,11 @@ static hipError_t launch_tile_update(ss_sim* s) {
         launch_tile_nm<T, 9, MOVE>(s, tile_args<T, 9>(s));
     else
         launch_tile_nm<T, 12, MOVE>(s, tile_args<T, 12>(s));
-    if (MOVE && s->pending) swap_sorted(s);
+    if (MOVE) s->last_resort = s->pending;
+    if (MOVE && s->pending) {
+        swap_sorted(s);
+        launch_inverse_perm(s);
+    }
     return hipGetLastError();
 }
 
@@ -502,11 +519,14 @@ static hipError_t launch_update(ss_sim* s) {
     UpdateParams<T> P = update_params<T>(s);
     State<T> st = state_a<T>(s);
     dim3 grid(blocks_for(s->n));
-    if (s->p.noise_mode != SS_NOISE_PHILOX)
-        hipLaunchKernelGGL((k_update_plain<T, true>), grid, dim3(256), 0, s->stream, st, s->n, g, P,
+    if (s->p.noise_mode == SS_NOISE_EXTERNAL)
+        hipLaunchKernelGGL((k_update_plain<T, NZ_EXT>), grid, dim3(256), 0, s->stream, st, s->n, g, P,
+                           (const T*)s->rec, s->rs, s->p.seed, s->d_step, s->step_off, (const T*)s->noise);
+    else if (s->p.noise_mode == SS_NOISE_REFERENCE)
+        hipLaunchKernelGGL((k_update_plain<T, NZ_REC>), grid, dim3(256), 0, s->stream, st, s->n, g, P,
                            (const T*)s->rec, s->rs, s->p.seed, s->d_step, s->step_off, (const T*)s->noise);
     else
-        hipLaunchKernelGGL((k_update_plain<T, false>), grid, dim3(256), 0, s->stream, st, s->n, g, P,
+        hipLaunchKernelGGL((k_update_plain<T, NZ_PHILOX>), grid, dim3(256), 0, s->stream, st, s->n, g, P,
                            (const T*)s->rec, s->rs, s->p.seed, s->d_step, s->step_off, (const T*)s->noise);
     return hipGetLastError();
 }
@@ -553,14 +573,15 @@ static hipError_t launch_initial_sort(ss_sim* s) {
                        state_b<T>(s), s->n, geo<T>(s), s->tg, s->toff, s->cursor);
     hipError_t e = hipGetLastError();
     swap_ab(s);
+    launch_inverse_perm(s);
     s->steps_since_sort = 0;
     s->pending = false;
     return e;
 }
 
-enum Stage { ST_BIN = 0, ST_FWD, ST_SOLVE, ST_INV, ST_PACK, ST_UPDATE, ST_SORT, ST_COUNT };
+enum Stage { ST_BIN = 0, ST_FWD, ST_SOLVE, ST_INV, ST_PACK, ST_NOISE, ST_UPDATE, ST_SORT, ST_COUNT };
 static const char* STAGE_NAMES[ST_COUNT] = {"bin", "fft_forward", "solve", "fft_inverse",
-                                            "pack", "update", "sort"};
+                                            "pack", "noise", "update", "sort"};
 
 // Moments -> cell records (bin/convert, forward FFT, solve, inverse FFT,
 // pack).  ev (may be null): events recorded after each stage.
@@ -618,10 +639,11 @@ static int step_once(ss_sim* s, hipEvent_t* ev) {
     if (rc != SS_OK) return rc;
     if (s->refn) {
         // the RandomVectors of do_timestep (mod.rs:259-286) from the reference's streams
-        if ((e = refnoise_enqueue(s->refn, s->noise, s->prec == SS_FP64, s->dr, s->stream)) != hipSuccess)
+        if ((e = refnoise_enqueue(s->refn, s->noise, s->prec == SS_FP64, s->dr, s->inv, s->stream)) != hipSuccess)
             return fail(SS_ERR_DEVICE, std::string("reference noise: ") + hipGetErrorString(e));
         s->have_noise = true;
     }
+    mark(ST_NOISE);
     if ((e = launch_update<T>(s)) != hipSuccess)
         return fail(SS_ERR_DEVICE, std::string("update: ") + hipGetErrorString(e));
     mark(ST_UPDATE);
@@ -767,6 +789,7 @@ static int run_steps(ss_sim* s, uint64_t nsteps) {
                 // advance the bookkeeping the captured period performs
                 for (int k = 0; k < K; ++k) s->timestep += 1;
                 if (s->sorted) {
+                    s->last_resort = K == 1 && s->pending;  // only a period's first update re-sorts
                     if (s->pending) swap_sorted(s);
                     s->pending = true;
                     s->steps_since_sort = 0;
@@ -899,7 +922,9 @@ static int create_impl(ss_sim* s) {
     A(s->rec, (size_t)s->ns * s->rs * sizeof(T));
     SS_HIP_CHECK(hipMemset(s->rec, 0, (size_t)s->ns * s->rs * sizeof(T)));
     SS_HIP_CHECK(hipMemset(s->spec_in, 0, cs * s->nm));
-    if (p.noise_mode != SS_NOISE_PHILOX) A(s->noise, 5 * n * sizeof(T));
+    if (p.noise_mode == SS_NOISE_EXTERNAL) A(s->noise, 5 * n * sizeof(T));
+    if (p.noise_mode == SS_NOISE_REFERENCE) A(s->noise, NOISE_REC * n * sizeof(T));
+    if (p.noise_mode == SS_NOISE_REFERENCE && s->sorted) A(s->inv, n * sizeof(uint32_t));
     const size_t stage_doubles = std::max<size_t>(5 * n, (size_t)24 * s->ns);
     A(s->staging, stage_doubles * sizeof(double));
     A(s->staging2, 3 * n * sizeof(double));
@@ -1322,12 +1347,22 @@ int ss_get_noise(ss_sim* s, double* rv, uint64_t n) {
     if (n != s->n) return fail(SS_ERR_INVALID, "noise length differs from number_of_particles");
     if (!s->have_noise) return fail(SS_ERR_STATE, "no noise drawn yet");
     SS_HIP_CHECK(hipSetDevice(s->p.device));
+    const bool recs = s->p.noise_mode == SS_NOISE_REFERENCE;
+    const uint64_t stride = recs ? 1 : n;
+    const int rec = recs ? NOISE_REC : 1;
+    // the records sit at the slots of the order the last step's update read
+    uint32_t* slots = nullptr;
+    if (s->inv) {
+        slots = reinterpret_cast<uint32_t*>(s->staging2);
+        hipLaunchKernelGGL(k_inverse_perm, dim3(blocks_for(n)), dim3(256), 0, s->stream,
+                           (const uint32_t*)(s->last_resort ? s->id2 : s->id), n, slots);
+    }
     if (s->prec == SS_FP64)
         hipLaunchKernelGGL(k_export_noise<double>, dim3(blocks_for(n)), dim3(256), 0, s->stream,
-                           (const double*)s->noise, n, s->staging);
+                           (const double*)s->noise, n, stride, rec, (const uint32_t*)slots, s->staging);
     else
         hipLaunchKernelGGL(k_export_noise<float>, dim3(blocks_for(n)), dim3(256), 0, s->stream,
-                           (const float*)s->noise, n, s->staging);
+                           (const float*)s->noise, n, stride, rec, (const uint32_t*)slots, s->staging);
     SS_HIP_CHECK(hipGetLastError());
     SS_HIP_CHECK(hipMemcpyAsync(rv, s->staging, sizeof(double) * 5 * n, hipMemcpyDeviceToHost, s->stream));
     SS_HIP_CHECK(hipStreamSynchronize(s->stream));

@@ -93,7 +93,7 @@ inline size_t tile_lds_bytes(int ws, int nm, int na) {
 // also runs for a tile without particles); a pass that does not own the tile
 // adds its own cells to the inflow accumulators instead.  kf must hold the
 // fixed-point moment kernels; acc is the window.  Ends with a barrier.
-template <typename T, int NM, bool MOVE, bool EXT_NOISE, bool KEEP, bool SCATTER>
+template <typename T, int NM, bool MOVE, int NZ, bool KEEP, bool SCATTER>
 __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0, uint32_t q1, bool owner,
                                  long long* acc, const long long* kf) {
     constexpr int NC = NM + 1;
@@ -114,9 +114,11 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
     uint32_t i = p0 + threadIdx.x;
     Loaded<T> cur{};
     uint32_t cid = 0;
+    NoiseRec<T> ncur{};  // NZ_REC: the record is prefetched with the state
     if (MOVE && i < p1) {
         cid = a.s.id[i];
         load_particle(a.s, i, cur);
+        if constexpr (NZ == NZ_REC) ncur = load_noise_rec(a.noise, i);
     }
     for (; i < p1; i += blockDim.x) {
         T nx, ny, nz, ophi, otheta;
@@ -134,8 +136,13 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
             Loaded<T> nxt;
             const uint32_t nid = a.s.id[j];
             load_particle(a.s, j, nxt);
+            NoiseRec<T> nnxt{};
+            if constexpr (NZ == NZ_REC) nnxt = load_noise_rec(a.noise, j);
             Noise<T> rv;
-            draw_noise<T>(a.P, a.seed, step, cid, a.n, a.noise, EXT_NOISE, rv);
+            if constexpr (NZ == NZ_REC)
+                noise_from_vector<T>(ncur.v, rv);
+            else
+                draw_noise<T, NZ>(a.P, a.seed, step, cid, a.n, a.noise, rv);
             State<T> out = a.s;
             uint32_t o = i;
             if constexpr (scatter) {
@@ -163,6 +170,7 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
                 move_loaded<T>(out, o, cur, a.P, recp, rv, nx, ny, nz, ophi, otheta);
             cur = nxt;
             cid = nid;
+            if constexpr (NZ == NZ_REC) ncur = nnxt;
         } else {
             nx = a.s.x[i];
             ny = a.s.y[i];
@@ -233,7 +241,7 @@ __device__ inline void load_kf(const long long* __restrict__ kfix, int na, long 
 
 // One system: workgroup `blockIdx.x` moves and bins the particles of one
 // tile (tile_for_block).
-template <typename T, int NM, bool MOVE, bool EXT_NOISE, bool KEEP, bool SCATTER>
+template <typename T, int NM, bool MOVE, int NZ, bool KEEP, bool SCATTER>
 __global__ void __launch_bounds__(256) k_tile_update(TileArgs<T, NM> a) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     const int tile = tile_for_block(a.tg, blockIdx.x);
@@ -241,7 +249,7 @@ __global__ void __launch_bounds__(256) k_tile_update(TileArgs<T, NM> a) {
     long long* acc = reinterpret_cast<long long*>(smem_raw);  // [ws][NM+1]
     long long* kf = acc + (size_t)a.tg.ws * (NM + 1);         // [NM][na]
     load_kf<NM>(a.kfix, a.g.na, kf);
-    tile_pass<T, NM, MOVE, EXT_NOISE, KEEP, SCATTER>(a, tile, a.toff[tile], a.toff[tile + 1], true, acc, kf);
+    tile_pass<T, NM, MOVE, NZ, KEEP, SCATTER>(a, tile, a.toff[tile], a.toff[tile + 1], true, acc, kf);
 }
 
 // Many systems in one persistent launch: work item w of the concatenated
@@ -252,7 +260,7 @@ constexpr int MAX_BATCH = 256;
 // Dynamic LDS: [lds_tile bytes for the tile body][njobs + 1 ints of start]; no
 // static LDS, so the 64-bit accumulators at the region's start stay 8-byte
 // aligned.
-template <typename T, int NM, bool MOVE, bool EXT_NOISE, bool SCATTER>
+template <typename T, int NM, bool MOVE, int NZ, bool SCATTER>
 __global__ void __launch_bounds__(256)
     k_tile_update_batched(const TileArgs<T, NM>* __restrict__ jobs, const int* __restrict__ start, int njobs,
                           int lds_tile, uint32_t step_off) {
@@ -278,7 +286,7 @@ __global__ void __launch_bounds__(256)
             long long* acc = reinterpret_cast<long long*>(smem_raw);
             long long* kf = acc + (size_t)a.tg.ws * (NM + 1);
             load_kf<NM>(a.kfix, a.g.na, kf);
-            tile_pass<T, NM, MOVE, EXT_NOISE, false, SCATTER>(a, tile, a.toff[tile], a.toff[tile + 1], true, acc,
+            tile_pass<T, NM, MOVE, NZ, false, SCATTER>(a, tile, a.toff[tile], a.toff[tile + 1], true, acc,
                                                                kf);
         }
     }

@@ -11,64 +11,42 @@ Behaviour kept from the reference:
 * ``particles_head`` limits the particles written; the initial entry holds
   the distribution and, when particle output is on, the particles;
 * a snapshot every ``snapshot`` steps and after the last step when
-  ``final_snapshot``.
+  ``final_snapshot``;
+* the isotropic and homogeneous initial conditions are the reference's own
+  streams (Pcg64Mcg::seed_from_u64(seed), particle.rs:224-297), drawn on
+  the GPU.
 
-Differences: the noise is counter-based (Philox keyed by seed, step and
-particle), so snapshots carry no generator state and a resumed run repeats
-an uninterrupted one exactly; the isotropic and homogeneous initial
-conditions follow the reference's placement rules (particle.rs:224-297)
-with numpy's generator instead of Pcg64Mcg.
+Noise: ``noise="philox"`` (default) is the counter-based stream keyed by
+(seed, step, particle), whose snapshots carry no generator state;
+``noise="reference"`` is the reference's own Pcg32 streams (one per rayon
+thread, ``noise_streams`` or the environment's RAYON_NUM_THREADS like
+mod.rs:113-116), whose generator states go into every snapshot.
 """
-import math
 import os
 import time
 
 import numpy as np
 
 from . import output as out
-from .params import FLAG_KEEP_DISTRIBUTION, FP32, FP64
+from .params import FLAG_KEEP_DISTRIBUTION, FP32, NOISE_REFERENCE
 from .simulation import Simulation, Snapshot
+from .simulation import create_homogeneous as _gpu_homogeneous
+from .simulation import create_isotropic as _gpu_isotropic
 
 VERSION = "0.1.0"
 
 
-def create_isotropic(n, box, seed):
-    """Particle::create_isotropic (particle.rs:224-255): uniform positions,
-    phi = 2 pi U, theta = acos(1 - 2U)."""
-    rng = np.random.default_rng(seed)
-    u = rng.random((n, 5))
-    p = np.empty((n, 5))
-    p[:, 0] = box[0] * u[:, 0]
-    p[:, 1] = box[1] * u[:, 1]
-    p[:, 2] = box[2] * u[:, 2]
-    p[:, 3] = 2 * math.pi * u[:, 3]
-    p[:, 4] = np.arccos(1.0 - 2.0 * u[:, 4])
-    return p
+def create_isotropic(n, box, seed, device=0):
+    """Particle::create_isotropic (particle.rs:224-255)."""
+    return _gpu_isotropic(n, box, seed, device)
 
 
-def create_homogeneous(n, kappa, box, seed):
-    """Particle::create_homogeneous (particle.rs:257-297): polar angle from
-    sin(theta) exp(kappa cos theta) (pdf_homogeneous_fixpoint, particle.rs:
-    347-360), then the orientation rotated by -pi/2 about x, so that the
-    alignment axis is y."""
+def create_homogeneous(n, kappa, box, seed, device=0):
+    """Particle::create_homogeneous (particle.rs:257-297); the reference
+    panics for kappa == 0 and for an angle that is NaN, this raises."""
     if kappa == 0.0:
         raise ValueError("Alignment of zero is the isotropic state. Please use that instead.")
-    rng = np.random.default_rng(seed)
-    u = rng.random((n, 5))
-    p = np.empty((n, 5))
-    p[:, 0] = box[0] * u[:, 0]
-    p[:, 1] = box[1] * u[:, 1]
-    p[:, 2] = box[2] * u[:, 2]
-    phi = 2 * math.pi * u[:, 3]
-    theta = np.arccos(np.log(math.exp(kappa) - 2.0 * u[:, 4] * math.sinh(kappa)) / kappa)
-    if np.any(np.isnan(theta)):
-        raise ValueError("Caution, the alignment parameter is too high for the given precision.")
-    n0, n1, n2 = np.sin(theta) * np.cos(phi), np.sin(theta) * np.sin(phi), np.cos(theta)
-    # rotation by -pi/2 about x: (x, y, z) -> (x, z, -y)
-    v0, v1, v2 = n0, n2, -n1
-    p[:, 3] = np.mod(np.arctan2(v1, v0), 2 * math.pi)
-    p[:, 4] = math.pi / 2 - np.arctan2(v2, np.sqrt(v0 * v0 + v1 * v1))
-    return p
+    return _gpu_homogeneous(n, kappa, box, seed, device)
 
 
 def output_base(out_dir, prefix, when=None):
@@ -132,11 +110,11 @@ def init_simulation(settings, params, init_type="distribution", particles=None):
         sim.init(particles)
     elif init_type == "distribution":
         if s.init_distribution == "Isotropic":
-            sim.init(create_isotropic(s.number_of_particles, box, s.seed))
+            sim.init(create_isotropic(s.number_of_particles, box, s.seed, params.device))
         else:
             q = settings.parameters
             sim.init(create_homogeneous(s.number_of_particles, q.magnetic_reorientation / q.diffusion.rotational,
-                                        box, s.seed))
+                                        box, s.seed, params.device))
     elif init_type in ("file", "resume"):
         fname = settings.environment.init_file
         if fname is None:
@@ -146,7 +124,8 @@ def init_simulation(settings, params, init_type="distribution", particles=None):
             sim.init(out.read_particles(fname))
         else:
             snap = out.read_snapshot(fname)
-            sim.resume(Snapshot(snap["particles"], snap["compensation"], snap["timestep"]))
+            rng = np.array(snap["rng_state"], dtype=np.uint64).reshape(-1, 2)
+            sim.resume(Snapshot(snap["particles"], snap["compensation"], snap["timestep"], rng))
     else:
         sim.close()
         raise ValueError(f"unknown init_type {init_type!r}")
@@ -196,14 +175,16 @@ def run_simulation(settings, sim, writer):
             writer.append(out.entry_value(ts, distribution=dist, flowfield=ff, magneticfield=mf, particles=parts))
         if due(o.snapshot):
             snap = sim.get_snapshot()
-            writer.write_snapshot(out.snapshot_value(snap.particles, snap.timestep, snap.compensation))
+            writer.write_snapshot(out.snapshot_value(snap.particles, snap.timestep, snap.compensation,
+                                                     snap.rng_state))
         t += 1
     if o.final_snapshot:
         snap = sim.get_snapshot()
-        writer.write_snapshot(out.snapshot_value(snap.particles, snap.timestep, snap.compensation))
+        writer.write_snapshot(out.snapshot_value(snap.particles, snap.timestep, snap.compensation, snap.rng_state))
 
 
-def run(settings, out_dir, precision=FP32, init_type="distribution", **overrides):
+def run(settings, out_dir, precision=FP32, init_type="distribution", noise="philox", noise_streams=None,
+        **overrides):
     """A whole run like the reference binary's ``run`` (main.rs:66-132):
     output directory, the settings saved as TOML, the header, the loop.
     Returns the output file path."""
@@ -211,6 +192,15 @@ def run(settings, out_dir, precision=FP32, init_type="distribution", **overrides
     flags = overrides.pop("flags", 0)
     if o.distribution is not None or o.initial_condition:
         flags |= FLAG_KEEP_DISTRIBUTION
+    if noise == "reference":
+        if noise_streams is None:
+            env = os.environ.get("RAYON_NUM_THREADS")
+            if not env or not env.isdigit():
+                raise ValueError("No environment variable 'RAYON_NUM_THREADS' set.")
+            noise_streams = int(env)
+        overrides.update(noise_mode=NOISE_REFERENCE, noise_streams=noise_streams)
+    elif noise != "philox":
+        raise ValueError(f"unknown noise {noise!r}")
     params = settings.to_params(precision=precision, flags=flags, **overrides)
     base = output_base(out_dir, settings.environment.prefix)
     save_toml(settings, base + ".toml")

@@ -381,7 +381,7 @@ def bincode_decode_snapshot(b):
     r = _BR(b)
     parts = r.particles()
     nrng = r.u64()
-    rng = [r.take("<QQ") for _ in range(nrng)]
+    rng = [tuple(r.take("<QQ")) for _ in range(nrng)]
     ts = r.u64()
     comp = None
     if r.pos < len(r.b):
@@ -444,11 +444,15 @@ def entry_value(timestep, distribution=None, flowfield=None, magneticfield=None,
     }
 
 
-def snapshot_value(particles, timestep, compensation=None):
-    """Snapshot (src/bin/simulation/mod.rs:80-86).  The counter-based noise
-    has no generator state, so ``rng_state`` is empty; the FP32 path's
-    low-order position parts follow as an extra ``compensation`` field."""
-    v = {"particles": np.ascontiguousarray(particles, dtype=np.float64), "rng_state": [], "timestep": int(timestep)}
+def snapshot_value(particles, timestep, compensation=None, rng_state=None):
+    """Snapshot (src/bin/simulation/mod.rs:80-86).  ``rng_state``: the
+    reference-noise generators as (state, increment) rows, each serialised
+    like rand_pcg's Lcg64Xsh32 {state, increment}; empty for the
+    counter-based Philox noise, which has no generator state.  The FP32
+    path's low-order position parts follow as an extra ``compensation``
+    field."""
+    rng = [] if rng_state is None else [{"state": int(s), "increment": int(i)} for s, i in rng_state]
+    v = {"particles": np.ascontiguousarray(particles, dtype=np.float64), "rng_state": rng, "timestep": int(timestep)}
     if compensation is not None:
         v["compensation"] = np.ascontiguousarray(compensation, dtype=np.float64)
     return v
@@ -588,7 +592,8 @@ def read_snapshot(path):
         return bincode_decode_snapshot(b)
     t = msgpack_decode(b) if fmt == "MsgPack" else cbor_decode(b)
     comp = t.get("compensation")
-    return {"particles": _plain_particles(t["particles"]), "rng_state": t["rng_state"], "timestep": t["timestep"],
+    rng = [(g["state"], g["increment"]) if isinstance(g, dict) else tuple(g) for g in t["rng_state"]]
+    return {"particles": _plain_particles(t["particles"]), "rng_state": rng, "timestep": t["timestep"],
             "compensation": None if comp is None else np.asarray(comp, dtype=np.float64).reshape(-1, 3)}
 
 

@@ -53,19 +53,23 @@ def _oracle_states(oracle, seed, t):
 
 
 @pytest.mark.parametrize(
-    "n,t,binning,precision",
+    "n,t,binning,precision,resort",
     [
-        (1000, 3, ss.BIN_HISTOGRAM, ss.FP64),
-        (5, 8, ss.BIN_HISTOGRAM, ss.FP64),
-        (50_000, 1, ss.BIN_HISTOGRAM, ss.FP64),
-        (20_000, 7, ss.BIN_SORTED, ss.FP64),
-        (1 << 20, 16, ss.BIN_SORTED, ss.FP32),
+        (1000, 3, ss.BIN_HISTOGRAM, ss.FP64, 0),
+        (5, 8, ss.BIN_HISTOGRAM, ss.FP64, 0),
+        (50_000, 1, ss.BIN_HISTOGRAM, ss.FP64, 0),
+        (20_000, 7, ss.BIN_SORTED, ss.FP64, 0),
+        (20_000, 7, ss.BIN_SORTED, ss.FP32, 1),
+        (1 << 20, 16, ss.BIN_SORTED, ss.FP32, 2),
     ],
 )
-def test_reference_noise_matches_oracle(oracle, n, t, binning, precision):
-    print(f"reference noise n={n} T={t} binning={binning} precision={precision}", flush=True)
+def test_reference_noise_matches_oracle(oracle, n, t, binning, precision, resort):
+    """Every step's RandomVectors and generator states; on the tiled path
+    the noise records follow the storage order, which re-sorts change (every
+    step with resort 1)."""
+    print(f"reference noise n={n} T={t} binning={binning} precision={precision} resort={resort}", flush=True)
     p = small_params(n=n, binning=binning, precision=precision, noise_mode=ss.NOISE_REFERENCE, noise_streams=t,
-                     seed=2024)
+                     seed=2024, resort_interval=resort)
     dr = math.sqrt(2.0 * p.diffusion_rotational * p.timestep)
     R = oracle.zig_tables()[0][1]
     states = _oracle_states(oracle, p.seed, t)
@@ -74,7 +78,7 @@ def test_reference_noise_matches_oracle(oracle, n, t, binning, precision):
     st0 = sim.get_rng_state()
     assert st0.shape == (t, 2)
     assert all((int(a), int(b)) == (s.state, s.inc) for (a, b), s in zip(st0, states))
-    for step in range(3):
+    for step in range(4):
         sim.step(1)
         want = oracle.ref_noise(states, n, dr)
         got = sim.get_noise()

@@ -197,19 +197,48 @@ def test_msgpack_output_reads_like_datastreamer(tmp_path):
     assert ff.shape == (3, 11, 12, 13)
 
 
-def test_particle_files_and_initial_conditions(tmp_path):
+def test_particle_files_round_trip(tmp_path):
+    rng = np.random.default_rng(4)
     for fmt, ext in O.EXTENSIONS.items():
         path = str(tmp_path / f"init.{ext}")
-        p = driver.create_isotropic(50, (1.0, 2.0, 3.0), 4)
+        p = rng.random((50, 5))
         O.write_particles(path, p)
         np.testing.assert_array_equal(O.read_particles(path), p)
+
+
+@pytest.mark.parametrize("fmt", ["MsgPack", "CBOR", "Bincode"])
+def test_snapshot_rng_state_round_trip(tmp_path, fmt):
+    """rng_state rows serialise like rand_pcg's Lcg64Xsh32 {state, increment}
+    (full u64 range) and read back as (state, increment) pairs."""
+    rng = np.random.default_rng(2)
+    parts = rng.random((4, 5))
+    st = [(2**64 - 1, 2**63 + 1), (12345, 6789)]
+    with O.OutputWriter(str(tmp_path / "r"), fmt) as w:
+        snap = w.write_snapshot(O.snapshot_value(parts, 9, rng_state=st))
+    s = O.read_snapshot(snap)
+    assert s["rng_state"] == st and s["timestep"] == 9
+    if fmt == "Bincode":
+        raw = lzma.decompress(open(snap, "rb").read())
+        # Vec<Particle>: u64 length + 4 x 5 f64, then Vec<Pcg32>: u64 length + (state, increment) pairs
+        off = 8 + 4 * 5 * 8
+        assert struct.unpack_from("<5Q", raw, off) == (2, 2**64 - 1, 2**63 + 1, 12345, 6789)
+    elif fmt == "MsgPack":
+        assert O.msgpack_decode(lzma.decompress(open(snap, "rb").read()))["rng_state"][0] == {
+            "state": 2**64 - 1, "increment": 2**63 + 1}
+
+
+@pytest.mark.gpu
+def test_driver_initial_conditions_are_the_reference_streams(oracle):
     p = driver.create_isotropic(200000, (1.0, 2.0, 3.0), 5)
+    np.testing.assert_array_equal(p[:, :4], oracle.create_isotropic(200000, (1.0, 2.0, 3.0), 5)[:, :4])
     assert abs(np.cos(p[:, 4]).mean()) < 0.01 and p[:, 1].max() < 2.0
     h = driver.create_homogeneous(200000, 2.0, (1.0, 1.0, 1.0), 6)
     ny = np.sin(h[:, 4]) * np.sin(h[:, 3])
     # <n . y> for p(c) ~ exp(kappa c): coth(kappa) - 1/kappa
     assert abs(ny.mean() - (1.0 / np.tanh(2.0) - 0.5)) < 0.01
     assert abs((np.sin(h[:, 4]) * np.cos(h[:, 3])).mean()) < 0.01 and abs(np.cos(h[:, 4]).mean()) < 0.01
+    with pytest.raises(ValueError, match="Alignment of zero"):
+        driver.create_homogeneous(10, 0.0, (1.0, 1.0, 1.0), 6)
 
 
 @pytest.mark.gpu
@@ -261,3 +290,36 @@ def test_driver_run_and_resume(tmp_path):
     res.resume(ss.Snapshot(mid["particles"], mid["compensation"], mid["timestep"]))
     res.step(6)
     np.testing.assert_array_equal(res.get_particles(), sim.get_particles())
+
+
+@pytest.mark.gpu
+def test_driver_reference_noise_snapshot_resume(tmp_path):
+    """Reference noise through the driver: the snapshot files carry the
+    generators, and a run resumed from the step-4 snapshot file continues the
+    uninterrupted run exactly (init_simulation 'resume', init.rs:51-115)."""
+    text = open(os.path.join(SET, "parameter_no_defaults.toml")).read()
+    text = (text.replace("number_of_particles = 100", "number_of_particles = 2000")
+            .replace("number_of_timesteps = 500", "number_of_timesteps = 6")
+            .replace("timestep = 0.1", "timestep = 0.001")
+            .replace("x = 11", "x = 8").replace("y = 12", "y = 8").replace("z = 13", "z = 8")
+            .replace("phi = 6", "phi = 4").replace("theta = 7", "theta = 4")
+            .replace("x = 1.0", "x = 4.0").replace("y = 2.0", "y = 4.0").replace("z = 3.0", "z = 4.0")
+            .replace("[simulation.output_at_timestep]",
+                     "[simulation.output_at_timestep]\n        particles = 7\n        snapshot = 4"))
+    st = S.read_parameter_file(text=text)
+    path = driver.run(st, str(tmp_path), precision=ss.FP64, noise="reference", noise_streams=3)
+    r = O.OutputReader(path)
+    last = [r[i] for i in range(len(r)) if r[i]["timestep"] == 7][0]["particles"]
+    d = os.path.dirname(path)
+    base = os.path.basename(path)[: -len(".msgpack-lzma")]
+    mid = os.path.join(d, f"{base}.snapshot.1.msgpack-lzma")
+    snap = O.read_snapshot(mid)
+    assert snap["timestep"] == 4 and len(snap["rng_state"]) == 3
+    st.environment.init_file = mid
+    p = st.to_params(precision=ss.FP64, noise_mode=ss.NOISE_REFERENCE, noise_streams=3)
+    sim = driver.init_simulation(st, p, "resume")
+    sim.step(3)
+    np.testing.assert_array_equal(sim.get_particles(), last)
+    with pytest.raises(ValueError, match="RAYON_NUM_THREADS"):
+        os.environ.pop("RAYON_NUM_THREADS", None)
+        driver.run(st, str(tmp_path), precision=ss.FP64, noise="reference")
