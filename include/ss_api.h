@@ -289,6 +289,21 @@ int ss_create_isotropic(uint64_t n, const ss_box* box, uint64_t seed, int32_t de
 int ss_create_homogeneous(uint64_t n, double kappa, const ss_box* box, uint64_t seed, int32_t device,
                           double* particles);
 
+/* DensityGradient::get_gradient + get_real_gradient
+   (src/distribution/density_gradient.rs:22-110): the density of a
+   normalised distribution ([x][y][z][phi][theta] doubles, like ss_flow_solve)
+   summed over the angles times dtheta dphi, its gradient by FFT; grad: 3 x
+   Ns real parts.  Not used by the step (the reference comments it out). */
+int ss_density_gradient(const ss_params* params, const double* dist, double* grad);
+
+/* interpolate_vector_field (src/mesh/interpolate.rs:11-123): trilinear
+   interpolation of a field [3][nx][ny][nz] (cell-centred, periodic) at n
+   positions (n x 3) into out (n x 3), in FP64.  The reference takes the z
+   offset from position.y; this entry point takes it from position.z.  Not
+   used by the step. */
+int ss_interpolate_vector_field(const ss_params* params, const double* field, const double* positions, uint64_t n,
+                                double* out);
+
 /* ---------------------------------------------------------------------------
  * FFT plan -- replaces FFTPlan (crates/fftw3/src/fft.rs:33-244): unnormalised,
  * in place, forward sign -1, backward +1, batched over `batch` contiguous

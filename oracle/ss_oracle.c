@@ -667,6 +667,127 @@ int o_mean_magnetic_field(const ss_params* p, const double* dist, double* b, dou
 }
 
 /* ------------------------------------------------------------------------ */
+/* Physics the live step leaves unused: DensityGradient                      */
+/* (distribution/density_gradient.rs:22-110) and the trilinear interpolation  */
+/* of mesh/interpolate.rs:11-123                                             */
+/* ------------------------------------------------------------------------ */
+/* ndarray numeric_util::unrolled_sum for contiguous slices (ArrayBase::sum). */
+static double unrolled_sum(const double* x, size_t len) {
+    double sum = 0.0;
+    double p0 = 0, p1 = 0, p2 = 0, p3 = 0, p4 = 0, p5 = 0, p6 = 0, p7 = 0;
+    while (len >= 8) {
+        p0 = p0 + x[0];
+        p1 = p1 + x[1];
+        p2 = p2 + x[2];
+        p3 = p3 + x[3];
+        p4 = p4 + x[4];
+        p5 = p5 + x[5];
+        p6 = p6 + x[6];
+        p7 = p7 + x[7];
+        x += 8;
+        len -= 8;
+    }
+    sum = sum + (p0 + p4);
+    sum = sum + (p1 + p5);
+    sum = sum + (p2 + p6);
+    sum = sum + (p3 + p7);
+    for (size_t i = 0; i < len; ++i) sum = sum + x[i];
+    return sum;
+}
+
+/* DensityGradient::get_real_gradient after update_gradient: density per
+   cell (sum over the angles times dtheta dphi), forward FFT, times k (i / N),
+   three inverse FFTs, real parts.  grad: 3 x Ns. */
+void o_density_gradient(const ss_params* p, const double* dist, double* grad) {
+    const int nx = p->grid_size.x, ny = p->grid_size.y, nz = p->grid_size.z;
+    const size_t ns = (size_t)nx * ny * nz, na = (size_t)p->grid_size.phi * p->grid_size.theta;
+    ogw gw = grid_width(p);
+    const double dthph = gw.theta * gw.phi;
+    double* rho = calloc(2 * ns, sizeof(double));
+    double* g = calloc(2 * 3 * ns, sizeof(double));
+    double *kx = malloc(sizeof(double) * nx), *ky = malloc(sizeof(double) * ny), *kz = malloc(sizeof(double) * nz);
+    o_k_sampling(nx, p->box_size.x, kx);
+    o_k_sampling(ny, p->box_size.y, ky);
+    o_k_sampling(nz, p->box_size.z, kz);
+    for (size_t c = 0; c < ns; ++c) {
+        /* Complex::from(sum) * dthph: (re * dthph, 0 * dthph) */
+        rho[2 * c] = unrolled_sum(dist + c * na, na) * dthph;
+        rho[2 * c + 1] = 0.0 * dthph;
+    }
+    o_fft3d(nx, ny, nz, rho, -1);
+    /* norm = Complex(0, 1) / n; e = (k * d) * norm in num-complex arithmetic,
+       k = (k_a, 0) */
+    const double nre = 0.0 / (double)ns, nim = 1.0 / (double)ns;
+    for (int ix = 0; ix < nx; ++ix)
+        for (int iy = 0; iy < ny; ++iy)
+            for (int iz = 0; iz < nz; ++iz) {
+                const size_t c = ((size_t)ix * ny + iy) * nz + iz;
+                const double k[3] = {kx[ix], ky[iy], kz[iz]};
+                const double dre = rho[2 * c], dim = rho[2 * c + 1];
+                for (int a = 0; a < 3; ++a) {
+                    const double kre = k[a], kim = 0.0;
+                    const double e1re = kre * dre - kim * dim, e1im = kre * dim + kim * dre;
+                    g[2 * ((size_t)a * ns + c)] = e1re * nre - e1im * nim;
+                    g[2 * ((size_t)a * ns + c) + 1] = e1re * nim + e1im * nre;
+                }
+            }
+    for (int a = 0; a < 3; ++a) o_fft3d(nx, ny, nz, g + 2 * (size_t)a * ns, +1);
+    for (size_t q = 0; q < 3 * ns; ++q) grad[q] = g[2 * q];
+    free(rho);
+    free(g);
+    free(kx);
+    free(ky);
+    free(kz);
+}
+
+/* lerp 0.2.0: a (1 - t) + b t. */
+static inline double o_lerp(double a, double b, double t) { return a * (1.0 - t) + b * t; }
+
+/* trilinear_interpolation (interpolate.rs:105-123): v[i][j][k] at i, j, k in
+   {0, 1} along x, y, z; x first, then y, then z. */
+double o_trilinear(const double rel[3], const double v[8]) {
+    const double c00 = o_lerp(v[0], v[4], rel[0]);
+    const double c01 = o_lerp(v[1], v[5], rel[0]);
+    const double c10 = o_lerp(v[2], v[6], rel[0]);
+    const double c11 = o_lerp(v[3], v[7], rel[0]);
+    const double c0 = o_lerp(c00, c10, rel[1]);
+    const double c1 = o_lerp(c01, c11, rel[1]);
+    return o_lerp(c0, c1, rel[2]);
+}
+
+/* get_neighbouring_cell_values (interpolate.rs:61-103): the 2x2x2 cells
+   around the position, from floor(q / g - 1/2), wrapped periodically. */
+void o_neighbour_values(const double* f, int nx, int ny, int nz, const double pos[3], const double gw[3],
+                        double v[8]) {
+    const int n[3] = {nx, ny, nz};
+    int idx[3];
+    for (int a = 0; a < 3; ++a) idx[a] = (int)floor(pos[a] / gw[a] - 0.5);
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int k = 0; k < 2; ++k) {
+                const int r[3] = {idx[0] + i, idx[1] + j, idx[2] + k};
+                int w[3];
+                for (int a = 0; a < 3; ++a) w[a] = (r[a] + n[a]) % n[a];
+                v[4 * i + 2 * j + k] = f[((size_t)w[0] * ny + w[1]) * nz + w[2]];
+            }
+}
+
+/* interpolate_vector_field (interpolate.rs:11-39) for a field [3][nx][ny][nz]:
+   the fractional offsets (q / g + 1/2) mod 1 -- the reference takes z's from
+   position.y, restated here from position.z. */
+void o_interpolate_vector_field(const double* field, int nx, int ny, int nz, const double pos[3],
+                                const double gw[3], double out[3]) {
+    double rel[3];
+    for (int a = 0; a < 3; ++a) rel[a] = fmod(pos[a] / gw[a] + 0.5, 1.0);
+    const size_t ns = (size_t)nx * ny * nz;
+    for (int c = 0; c < 3; ++c) {
+        double v[8];
+        o_neighbour_values(field + (size_t)c * ns, nx, ny, nz, pos, gw, v);
+        out[c] = o_trilinear(rel, v);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* Noise: Philox4x32-10 (Salmon et al. 2011) -> the RandomVector of          */
 /* src/bin/simulation/mod.rs:51-57,259-286,383-385                           */
 /* ------------------------------------------------------------------------ */

@@ -62,6 +62,13 @@ int o_mean_magnetic_field(const ss_params* p, const double* dist, double* b, dou
 void o_update_one(const ss_params* p, const double part[5], const double fields[34],
                   const double rv[5], double out[5]);
 void o_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* DensityGradient (real parts, 3 x Ns) and trilinear interpolation. */
+void o_density_gradient(const ss_params* p, const double* dist, double* grad);
+double o_trilinear(const double rel[3], const double v[8]);
+void o_neighbour_values(const double* f, int nx, int ny, int nz, const double pos[3], const double gw[3],
+                        double v[8]);
+void o_interpolate_vector_field(const double* field, int nx, int ny, int nz, const double pos[3],
+                                const double gw[3], double out[3]);
 void o_noise(uint64_t seed, uint64_t step, uint64_t id, double dr, double out[5]);
 /* The reference's random streams (rand_pcg 0.2.0, rand_core 0.5.0, rand 0.7.0,
    rand_distr 0.2.1). */

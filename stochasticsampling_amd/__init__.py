@@ -30,7 +30,9 @@ from .simulation import (  # noqa: F401
     create_isotropic,
     debug_coord_to_grid,
     debug_update,
+    density_gradient,
     initial_condition,
+    interpolate_vector_field,
     mean_flow_field,
     mean_magnetic_field,
 )

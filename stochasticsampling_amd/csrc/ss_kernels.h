@@ -669,6 +669,96 @@ __global__ void __launch_bounds__(256) k_real_parts(const cplx<T>* __restrict__ 
     if (i < cnt) out[i] = (double)c[i].x;
 }
 
+// DensityGradient (distribution/density_gradient.rs:73-110), not part of the
+// step: the density of each cell as ndarray sums a contiguous row (eight
+// partial sums, numeric_util::unrolled_sum) times dtheta dphi.
+__global__ void __launch_bounds__(256) k_density_rows(const double* __restrict__ dist, uint64_t ns, int na,
+                                                      double dthph, double* __restrict__ rho) {
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ns) return;
+    const double* x = dist + c * (uint64_t)na;
+    double p[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int len = na, o = 0;
+    while (len >= 8) {
+        for (int k = 0; k < 8; ++k) p[k] = p[k] + x[o + k];
+        o += 8;
+        len -= 8;
+    }
+    double sum = 0.0;
+    sum = sum + (p[0] + p[4]);
+    sum = sum + (p[1] + p[5]);
+    sum = sum + (p[2] + p[6]);
+    sum = sum + (p[3] + p[7]);
+    for (int i = 0; i < len; ++i) sum = sum + x[o + i];
+    rho[c] = sum * dthph;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_density_spec(const double* __restrict__ rho, uint64_t ns, T dthph,
+                                                      cplx<T>* __restrict__ spec) {
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < ns) spec[c] = cplx<T>{(T)rho[c], T(0) * dthph};
+}
+
+// g_a = (k_a d) (i / N) in complex arithmetic with k = (k_a, 0), as the
+// reference evaluates it (no fused multiply-adds).
+template <typename T>
+__global__ void __launch_bounds__(256) k_density_grad(const cplx<T>* __restrict__ d, Geo<T> g,
+                                                      const T* __restrict__ kx, const T* __restrict__ ky,
+                                                      const T* __restrict__ kz, T nre, T nim,
+                                                      cplx<T>* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.ns) return;
+    const int iz = c % g.nz, iy = (c / g.nz) % g.ny, ix = c / (g.nz * g.ny);
+    const T k[3] = {kx[ix], ky[iy], kz[iz]};
+    const T dre = d[c].x, dim = d[c].y;
+    for (int a = 0; a < 3; ++a) {
+        const T kre = k[a], kim = T(0);
+        const T e1re = kre * dre - kim * dim, e1im = kre * dim + kim * dre;
+        out[(size_t)a * g.ns + c] = cplx<T>{e1re * nre - e1im * nim, e1re * nim + e1im * nre};
+    }
+}
+
+// interpolate_vector_field (mesh/interpolate.rs:11-123) of a field
+// [3][nx][ny][nz] at n positions: the 2x2x2 cells around floor(q / g - 1/2)
+// (periodic), trilinear with lerp(a, b, t) = a (1 - t) + b t along x, y, z.
+// The reference takes z's fractional offset from position.y; here from z.
+__global__ void __launch_bounds__(256) k_interpolate(const double* __restrict__ f, int nx, int ny, int nz,
+                                                     double gx, double gy, double gz,
+                                                     const double* __restrict__ pos, uint64_t n,
+                                                     double* __restrict__ out) {
+#pragma clang fp contract(off)
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double q[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    const double gw[3] = {gx, gy, gz};
+    const int dims[3] = {nx, ny, nz};
+    double rel[3];
+    int w0[3], w1[3];
+    for (int a = 0; a < 3; ++a) {
+        rel[a] = fmod(q[a] / gw[a] + 0.5, 1.0);
+        const int idx = (int)floor(q[a] / gw[a] - 0.5);
+        w0[a] = (idx + dims[a]) % dims[a];
+        w1[a] = (idx + 1 + dims[a]) % dims[a];
+    }
+    const size_t ns = (size_t)nx * ny * nz;
+    auto lerp = [](double a, double b, double t) { return a * (1.0 - t) + b * t; };
+    for (int comp = 0; comp < 3; ++comp) {
+        const double* fc = f + (size_t)comp * ns;
+        double v[8];
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                for (int c = 0; c < 2; ++c)
+                    v[4 * a + 2 * b + c] =
+                        fc[((size_t)(a ? w1[0] : w0[0]) * ny + (b ? w1[1] : w0[1])) * nz + (c ? w1[2] : w0[2])];
+        const double c00 = lerp(v[0], v[4], rel[0]), c01 = lerp(v[1], v[5], rel[0]);
+        const double c10 = lerp(v[2], v[6], rel[0]), c11 = lerp(v[3], v[7], rel[0]);
+        const double c0 = lerp(c00, c10, rel[1]), c1 = lerp(c01, c11, rel[1]);
+        out[3 * i + comp] = lerp(c0, c1, rel[2]);
+    }
+}
+
 // Contraction of a caller-given distribution (values, not counts) for the
 // solver entry points ss_flow_solve / ss_magnetic_solve.
 template <typename T, int NM>

@@ -1524,6 +1524,87 @@ int ss_magnetic_solve(const ss_params* params, const double* dist, double* b, do
                                         : solve_dense<float>(params, dist, true, b, grad_b);
 }
 
+// DensityGradient::get_real_gradient (distribution/density_gradient.rs).
+template <typename T>
+static int density_gradient(const ss_params* params, const double* dist, double* grad) {
+    ss_params p = *params;
+    p.number_of_particles = 1;
+    p.noise_mode = SS_NOISE_PHILOX;
+    p.binning = SS_BIN_HISTOGRAM;
+    p.flags &= ~SS_FLAG_QUASI2D;
+    ss_sim* s = nullptr;
+    int rc = ss_create(&p, &s);
+    if (rc != SS_OK) return rc;
+    const size_t nb = (size_t)s->ns * s->na;
+    double *d = nullptr, *rho = nullptr;
+    if (hipMalloc(&d, nb * sizeof(double)) != hipSuccess || hipMalloc(&rho, (size_t)s->ns * sizeof(double)) != hipSuccess)
+        rc = fail(SS_ERR_ALLOC, "hipMalloc");
+    if (rc == SS_OK && hipMemcpy(d, dist, nb * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(SS_ERR_DEVICE, "copy in");
+    if (rc == SS_OK) {
+        GW gw = grid_width(s->p);
+        const double dthph = gw.theta * gw.phi;
+        const dim3 grid(blocks_for((uint64_t)s->ns));
+        hipLaunchKernelGGL(k_density_rows, grid, dim3(256), 0, s->stream, d, (uint64_t)s->ns, s->na, dthph, rho);
+        hipLaunchKernelGGL(k_density_spec<T>, grid, dim3(256), 0, s->stream, rho, (uint64_t)s->ns, (T)dthph,
+                           (cplx<T>*)s->spec_in);
+        if (hipGetLastError() != hipSuccess) rc = fail(SS_ERR_DEVICE, "density launch");
+    }
+    if (rc == SS_OK) rc = s->fft.run(s->spec_in, 1, -1, s->stream);
+    if (rc == SS_OK) {
+        const T nre = T(0) / (T)s->ns, nim = T(1) / (T)s->ns;  // Complex(0, 1) / n
+        hipLaunchKernelGGL(k_density_grad<T>, dim3(blocks_for((uint64_t)s->ns)), dim3(256), 0, s->stream,
+                           (const cplx<T>*)s->spec_in, geo<T>(s), (const T*)s->kx, (const T*)s->ky,
+                           (const T*)s->kz, nre, nim, (cplx<T>*)s->spec_out);
+        if (hipGetLastError() != hipSuccess) rc = fail(SS_ERR_DEVICE, "gradient launch");
+    }
+    if (rc == SS_OK) rc = s->fft.run(s->spec_out, 3, +1, s->stream);
+    if (rc == SS_OK) rc = copy_real<T>(s, s->spec_out, 0, 3, grad);
+    (void)hipFree(d);
+    (void)hipFree(rho);
+    std::string msg = ss_last_error();
+    ss_destroy(s);
+    if (rc != SS_OK) set_error(msg);
+    return rc;
+}
+
+int ss_density_gradient(const ss_params* params, const double* dist, double* grad) {
+    if (!params || !dist || !grad) return fail(SS_ERR_INVALID, "NULL argument");
+    return params->precision == SS_FP64 ? density_gradient<double>(params, dist, grad)
+                                        : density_gradient<float>(params, dist, grad);
+}
+
+int ss_interpolate_vector_field(const ss_params* params, const double* field, const double* positions, uint64_t n,
+                                double* out) {
+    if (!params || !field || (n && (!positions || !out))) return fail(SS_ERR_INVALID, "NULL argument");
+    const ss_grid& g = params->grid_size;
+    if (g.x < 1 || g.y < 1 || g.z < 1) return fail(SS_ERR_INVALID, "grid sizes must be >= 1");
+    if (n == 0) return SS_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SS_ERR_DEVICE, "no GPU visible");
+    if (params->device < 0 || params->device >= ndev) return fail(SS_ERR_INVALID, "bad device ordinal");
+    SS_HIP_CHECK(hipSetDevice(params->device));
+    const size_t nf = 3 * (size_t)g.x * g.y * g.z;
+    double *df = nullptr, *dp = nullptr, *dout = nullptr;
+    hipError_t e = hipMalloc(&df, nf * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&dp, 3 * n * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&dout, 3 * n * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpy(df, field, nf * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dp, positions, 3 * n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        GW gw = grid_width(*params);
+        hipLaunchKernelGGL(k_interpolate, dim3(blocks_for(n)), dim3(256), 0, 0, df, g.x, g.y, g.z, gw.x, gw.y, gw.z,
+                           dp, n, dout);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out, dout, 3 * n * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(df);
+    (void)hipFree(dp);
+    (void)hipFree(dout);
+    if (e != hipSuccess) return fail(SS_ERR_DEVICE, std::string("interpolation: ") + hipGetErrorString(e));
+    return SS_OK;
+}
+
 template <typename T>
 static int debug_update(const ss_params* p, uint64_t n, const double* parts, const double* fields,
                         const double* rv, double* out) {

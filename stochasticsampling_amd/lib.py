@@ -68,6 +68,8 @@ def _declare(L):
         "ss_fft_plan_destroy": (C.c_int, [vp]),
         "ss_flow_solve": (C.c_int, [pp, _d, _d, _d]),
         "ss_magnetic_solve": (C.c_int, [pp, _d, _d, _d]),
+        "ss_density_gradient": (C.c_int, [pp, _d, _d]),
+        "ss_interpolate_vector_field": (C.c_int, [pp, _d, _d, C.c_uint64, _d]),
         "ss_debug_update": (C.c_int, [pp, C.c_uint64, _d, _d, _d, _d]),
         "ss_debug_coord_to_grid": (C.c_int, [pp, C.c_uint64, _d, _P(C.c_int32)]),
     }

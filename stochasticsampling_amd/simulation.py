@@ -305,6 +305,30 @@ def create_homogeneous(n, kappa, box, seed, device=0):
     return out
 
 
+def density_gradient(params, dist):
+    """DensityGradient::get_gradient + get_real_gradient
+    (src/distribution/density_gradient.rs): gradient of the density of a
+    normalised distribution, real parts, shape (3, nx, ny, nz)."""
+    L = _l.load()
+    d = _f64(dist)
+    out = np.zeros((3,) + params.grid_shape)
+    _l.check(L.ss_density_gradient(C.byref(params), _l.dptr(d), _l.dptr(out)))
+    return out
+
+
+def interpolate_vector_field(params, field, positions):
+    """interpolate_vector_field (src/mesh/interpolate.rs:11-39): trilinear
+    interpolation of a cell-centred periodic field (3, nx, ny, nz) at
+    positions (n, 3); the z offset comes from position.z (the reference
+    takes position.y)."""
+    L = _l.load()
+    f = _f64(field)
+    q = _f64(positions).reshape(-1, 3)
+    out = np.zeros((q.shape[0], 3))
+    _l.check(L.ss_interpolate_vector_field(C.byref(params), _l.dptr(f), _l.dptr(q), q.shape[0], _l.dptr(out)))
+    return out
+
+
 def mean_flow_field(params, dist):
     """SpectralSolver::new + mean_flow_field (spectral_solver.rs:33-68,233-254)
     on a given distribution; returns (u [3,...], grad u [3,3,...]) real parts."""

@@ -102,6 +102,11 @@ class Oracle:
         L.osim_get_rng_state.argtypes = [C.c_void_p, u64p, C.c_int]
         L.osim_set_rng_state.argtypes = [C.c_void_p, u64p, C.c_int]
         L.osim_set_timestep.argtypes = [C.c_void_p, C.c_uint64]
+        L.o_density_gradient.argtypes = [pp, _d, _d]
+        L.o_trilinear.argtypes = [_d, _d]
+        L.o_trilinear.restype = C.c_double
+        L.o_neighbour_values.argtypes = [_d, C.c_int, C.c_int, C.c_int, _d, _d, _d]
+        L.o_interpolate_vector_field.argtypes = [_d, C.c_int, C.c_int, C.c_int, _d, _d, _d]
 
     # pure functions -------------------------------------------------------
     def modulo(self, f, m):
@@ -230,6 +235,34 @@ class Oracle:
 
     def set_threads(self, n):
         self.L.osim_set_threads(n)
+
+    # physics the live step leaves unused -----------------------------------
+    def density_gradient(self, params, dist):
+        d = np.ascontiguousarray(dist, dtype=np.float64)
+        out = np.zeros(3 * params.ns)
+        self.L.o_density_gradient(C.byref(params), _dp(d), _dp(out))
+        return out.reshape((3,) + params.grid_shape)
+
+    def trilinear(self, rel, v):
+        r = np.ascontiguousarray(rel, dtype=np.float64)
+        a = np.ascontiguousarray(v, dtype=np.float64).ravel()
+        return self.L.o_trilinear(_dp(r), _dp(a))
+
+    def neighbour_values(self, f, pos, gw):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        p = np.ascontiguousarray(pos, dtype=np.float64)
+        g = np.ascontiguousarray(gw, dtype=np.float64)
+        out = np.zeros(8)
+        self.L.o_neighbour_values(_dp(f), f.shape[0], f.shape[1], f.shape[2], _dp(p), _dp(g), _dp(out))
+        return out.reshape(2, 2, 2)
+
+    def interpolate_vector_field(self, field, pos, gw):
+        f = np.ascontiguousarray(field, dtype=np.float64)
+        p = np.ascontiguousarray(pos, dtype=np.float64)
+        g = np.ascontiguousarray(gw, dtype=np.float64)
+        out = np.zeros(3)
+        self.L.o_interpolate_vector_field(_dp(f), f.shape[1], f.shape[2], f.shape[3], _dp(p), _dp(g), _dp(out))
+        return out
 
     # the reference's random streams ----------------------------------------
     def pcg32_new(self, state, stream):
