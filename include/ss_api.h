@@ -63,6 +63,10 @@ extern "C" {
 #define SS_FLAG_AXIS_PASSES 16u       /* tiled path: run the spectral stage as separate kernels
                                          (convert, one pass per FFT axis, solve, pack) instead of
                                          the three fused plane/pencil kernels (same results)      */
+#define SS_FLAG_UNPAIRED_SPECTRAL 32u /* fused spectral stage with one complex transform per real
+                                         component instead of two real fields per transform
+                                         (bit-identical to SS_FLAG_AXIS_PASSES; the paired default
+                                         agrees to rounding)                                      */
 
 /* Size of the simulation box (lib.rs BoxSize). */
 typedef struct ss_box {
@@ -234,7 +238,8 @@ int ss_synchronize(ss_sim* sim);
    reports). */
 typedef struct ss_info {
     int32_t sorted;          /* tiled binning path (particles grouped by tile) */
-    int32_t fused_spectral;  /* spectral stage as the three fused kernels      */
+    int32_t fused_spectral;  /* spectral stage as the three fused kernels: 1 one
+                                transform per component, 2 two fields per transform */
     int32_t tile[3];         /* tile extent in cells (tiled path)              */
     int32_t ntiles;
     int32_t resort_interval; /* steps between re-sorts (tiled path)            */

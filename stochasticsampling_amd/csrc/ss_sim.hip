@@ -150,6 +150,12 @@ struct ss_sim {
     // radices 2-5); spec_in then holds moments transformed along z and y only
     bool fused = false;
     bool spec_partial = false;
+    // paired fused stage: two real fields per complex transform; spec_in then
+    // holds the moments packed in pairs (spec_packed)
+    bool paired = false;
+    bool spec_packed = false;
+    uint2* pairs = nullptr;  // (pencil, mirror pencil) with pencil <= mirror
+    int npairs = 0;
     int pencils = 8;  // (y, z) pencils per workgroup of the pencil kernel
     uint32_t* home_ctr = nullptr;  // re-sort: particles placed from a segment's front
     bool mag = false;  // magnetic solve runs
@@ -416,6 +422,54 @@ static hipError_t launch_spec_pencils(ss_sim* s) {
 }
 
 template <typename T>
+static hipError_t launch_spec_planes_fwd2(ss_sim* s) {
+    double ms, ds;
+    scales(s, ms, ds);
+    const size_t lds = spec_plane_lds(s->ny, s->nz, sizeof(T));
+    const int npk = (s->nm + 1) / 2;
+    const dim3 grid((unsigned)(s->nx * npk));
+    T* rho = s->use_rho ? (T*)s->rho : nullptr;
+    const SpecAxis ay = spec_axis(s->fft.ax[1]), az = spec_axis(s->fft.ax[2]);
+    const cplx<T>* twy = (const cplx<T>*)s->fft.ax[1].tw;
+    const cplx<T>* twz = (const cplx<T>*)s->fft.ax[2].tw;
+    if (s->nm == 6)
+        hipLaunchKernelGGL((k_spec_planes_fwd2<T, 6>), grid, dim3(SPEC_PLANE_THREADS), lds, s->stream, s->mom_main,
+                           s->mom_inflow, s->nx, s->fixed_scale * ms, ds, ay, az, twy, twz, (cplx<T>*)s->spec_in, rho,
+                           (T*)s->rec, s->rs);
+    else
+        hipLaunchKernelGGL((k_spec_planes_fwd2<T, 9>), grid, dim3(SPEC_PLANE_THREADS), lds, s->stream, s->mom_main,
+                           s->mom_inflow, s->nx, s->fixed_scale * ms, ds, ay, az, twy, twz, (cplx<T>*)s->spec_in, rho,
+                           (T*)s->rec, s->rs);
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_spec_pencils2(ss_sim* s) {
+    const int np = s->ny * s->nz;
+    const int nout = s->use_e ? 12 : 6;
+    const int npk = (s->nsig + 1) / 2;
+    const size_t lds = spec_pencil_lds(s->nx, 2 * std::max(npk, nout / 2) * s->pencils, sizeof(T));
+    const dim3 grid((unsigned)((s->npairs + s->pencils - 1) / s->pencils));
+    hipLaunchKernelGGL(k_spec_pencils2<T>, grid, dim3(SPEC_PENCIL_THREADS), lds, s->stream,
+                       (const cplx<T>*)s->spec_in, (cplx<T>*)s->spec_out, (const uint2*)s->pairs, s->npairs, np, s->nz,
+                       s->ny, s->pencils, s->nsig, nout, spec_axis(s->fft.ax[0]), (const cplx<T>*)s->fft.ax[0].tw,
+                       (const T*)s->kx, (const T*)s->ky, (const T*)s->kz, (const T*)s->ik2, (const T*)s->kinv,
+                       (T)s->p.hydro_screening, (T)(double)s->ns);
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_spec_planes_inv2(ss_sim* s) {
+    const int nout = s->use_e ? 12 : 6;
+    const size_t lds = spec_plane_lds(s->ny, s->nz, sizeof(T));
+    const dim3 grid((unsigned)(s->nx * (nout / 2)));
+    hipLaunchKernelGGL(k_spec_planes_inv2<T>, grid, dim3(SPEC_PLANE_THREADS), lds, s->stream,
+                       (const cplx<T>*)s->spec_out, s->nx, spec_axis(s->fft.ax[1]), spec_axis(s->fft.ax[2]),
+                       (const cplx<T>*)s->fft.ax[1].tw, (const cplx<T>*)s->fft.ax[2].tw, (T*)s->rec, s->rs);
+    return hipGetLastError();
+}
+
+template <typename T>
 static hipError_t launch_spec_planes_inv(ss_sim* s) {
     const int nout = s->use_e ? 12 : 6;
     const size_t lds = spec_plane_lds(s->ny, s->nz, sizeof(T));
@@ -594,18 +648,19 @@ static int spectral_stage(ss_sim* s, hipEvent_t* ev) {
     if (s->fused) {
         // stages: bin = moments -> z, y transforms; solve = x transforms and
         // solve; fft_inverse = y, z inverse transforms into the records
-        if ((e = launch_spec_planes_fwd<T>(s)) != hipSuccess)
+        if ((e = s->paired ? launch_spec_planes_fwd2<T>(s) : launch_spec_planes_fwd<T>(s)) != hipSuccess)
             return fail(SS_ERR_DEVICE, std::string("spectral planes: ") + hipGetErrorString(e));
         mark(ST_BIN);
         mark(ST_FWD);
-        if ((e = launch_spec_pencils<T>(s)) != hipSuccess)
+        if ((e = s->paired ? launch_spec_pencils2<T>(s) : launch_spec_pencils<T>(s)) != hipSuccess)
             return fail(SS_ERR_DEVICE, std::string("spectral pencils: ") + hipGetErrorString(e));
         mark(ST_SOLVE);
-        if ((e = launch_spec_planes_inv<T>(s)) != hipSuccess)
+        if ((e = s->paired ? launch_spec_planes_inv2<T>(s) : launch_spec_planes_inv<T>(s)) != hipSuccess)
             return fail(SS_ERR_DEVICE, std::string("spectral planes: ") + hipGetErrorString(e));
         mark(ST_INV);
         mark(ST_PACK);
         s->spec_partial = true;
+        s->spec_packed = s->paired;
     } else {
         if ((e = launch_bin<T>(s)) != hipSuccess)
             return fail(SS_ERR_DEVICE, std::string("bin: ") + hipGetErrorString(e));
@@ -714,6 +769,15 @@ static int export_particles(ss_sim* s, double* out, double* comp, uint64_t n) {
 // spec_full: u(3), G(9), [b(3), gradB(9)], inverse transformed.
 template <typename T>
 static int full_fields(ss_sim* s) {
+    if (s->spec_packed) {  // the paired stage left the moments packed in pairs
+        const size_t ns = (size_t)s->ns;
+        hipLaunchKernelGGL(k_spec_unpack<T>, dim3(blocks_for(ns)), dim3(256), 0, s->stream,
+                           (const cplx<T>*)s->spec_in, s->nsig, s->nx, s->ny, s->nz, (cplx<T>*)s->spec_full);
+        SS_HIP_CHECK(hipGetLastError());
+        SS_HIP_CHECK(hipMemcpyAsync(s->spec_in, s->spec_full, sizeof(cplx<T>) * ns * s->nsig, hipMemcpyDeviceToDevice,
+                                    s->stream));
+        s->spec_packed = false;
+    }
     if (s->spec_partial) {  // the fused step left the x transform of the moments undone
         int rc = s->fft.run_axis(s->spec_in, s->nm, -1, 0, s->stream);
         if (rc != SS_OK) return rc;
@@ -796,6 +860,7 @@ static int run_steps(ss_sim* s, uint64_t nsteps) {
                 }
                 s->have_fields = true;
                 s->spec_partial = s->fused;
+                s->spec_packed = s->fused && s->paired;
             }
             SS_HIP_CHECK(hipGraphLaunch(s->graph[key], s->stream));
             nsteps -= (uint64_t)K;
@@ -863,6 +928,30 @@ static int choose_spectral(ss_sim* s) {
                                      (int)pencil));
     s->pencils = pb;
     s->fused = true;
+    if (!(s->p.flags & SS_FLAG_UNPAIRED_SPECTRAL)) {
+        // pencil pairs (p, mirror p); a pair holds two pencils of half the
+        // components, so the same pencil-group size fits the same LDS
+        const int ny = s->ny, nz = s->nz, np = ny * nz;
+        std::vector<uint2> pr;
+        for (int p = 0; p < np; ++p) {
+            const int y = p / nz, z = p % nz;
+            const int m = ((ny - y) % ny) * nz + (nz - z) % nz;
+            if (p <= m) pr.push_back(make_uint2((unsigned)p, (unsigned)m));
+        }
+        SS_HIP_CHECK(hipFuncSetAttribute((const void*)k_spec_planes_fwd2<T, 6>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane));
+        SS_HIP_CHECK(hipFuncSetAttribute((const void*)k_spec_planes_fwd2<T, 9>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane));
+        SS_HIP_CHECK(hipFuncSetAttribute((const void*)k_spec_planes_inv2<T>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane));
+        SS_HIP_CHECK(hipFuncSetAttribute((const void*)k_spec_pencils2<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)pencil));
+        int rc = alloc(s, (void**)&s->pairs, pr.size() * sizeof(uint2));
+        if (rc != SS_OK) return rc;
+        SS_HIP_CHECK(hipMemcpy(s->pairs, pr.data(), pr.size() * sizeof(uint2), hipMemcpyHostToDevice));
+        s->npairs = (int)pr.size();
+        s->paired = true;
+    }
     return SS_OK;
 }
 
@@ -1395,7 +1484,7 @@ int ss_get_info(const ss_sim* s, ss_info* out) {
     if (!s || !out) return fail(SS_ERR_INVALID, "NULL argument");
     std::memset(out, 0, sizeof(*out));
     out->sorted = s->sorted ? 1 : 0;
-    out->fused_spectral = s->fused ? 1 : 0;
+    out->fused_spectral = s->fused ? (s->paired ? 2 : 1) : 0;
     if (s->sorted) {
         for (int k = 0; k < 3; ++k) out->tile[k] = s->tg.t[k];
         out->ntiles = s->tg.ntiles;
@@ -1919,6 +2008,7 @@ static int ens_run(ss_ensemble* e, uint64_t nsteps) {
                     s->steps_since_sort = 0;
                     s->have_fields = true;
                     s->spec_partial = s->fused;
+                    s->spec_packed = s->fused && s->paired;
                 }
                 e->timestep += K;
             }

@@ -168,4 +168,236 @@ __global__ void __launch_bounds__(SPEC_PLANE_THREADS)
     for (int c = threadIdx.x; c < np; c += blockDim.x) rec[(base + c) * rs + slot] = r2[c].x;
 }
 
+// ---------------------------------------------------------------------------
+// Paired variant: two real fields per complex transform.
+//
+// The moments and the fields are real, so two of them share one complex
+// transform (a + i b).  The pencil kernel takes a (y, z) pencil together with
+// its mirror (-y, -z), where the transforms of a and b separate:
+// A(k) = (Z(k) + conj Z(-k)) / 2, B(k) = (Z(k) - conj Z(-k)) / 2i.  The solve
+// runs at k and at -k (with the reference's k table, whose Nyquist entries
+// are their own mirror), and the Hermitian part H(k) = (X(k) + conj X(-k)) / 2
+// of each output goes into the inverse transform: its inverse is real and
+// equals the real part of the inverse of X, which is what the reference
+// keeps.  Off the Nyquist planes X(-k) = conj X(k) exactly, so H = X there.
+// Half the transforms of the per-component path, the same results up to
+// rounding.
+// ---------------------------------------------------------------------------
+
+// Forward: packed component c = moments 2c + i 2c+1 of plane x (2c+1 may be
+// absent); blocks with c == 0 also convert the count component.
+template <typename T, int NM>
+__global__ void __launch_bounds__(SPEC_PLANE_THREADS)
+    k_spec_planes_fwd2(const long long* __restrict__ main, long long* __restrict__ inflow, int nx, double scale,
+                       double rho_scale, SpecAxis ay, SpecAxis az, const cplx<T>* __restrict__ twy,
+                       const cplx<T>* __restrict__ twz, cplx<T>* __restrict__ spec, T* __restrict__ rho,
+                       T* __restrict__ rec, int rs) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    const int ny = ay.n, nz = az.n, np = ny * nz;
+    cplx<T>* ty = reinterpret_cast<cplx<T>*>(smem_raw);
+    cplx<T>* tz = ty + ny;
+    cplx<T>* b0 = tz + nz;
+    cplx<T>* b1 = b0 + np;
+    const int x = blockIdx.x % nx, c = blockIdx.x / nx;
+    const int q0 = 2 * c, q1 = 2 * c + 1;
+    const size_t ns = (size_t)nx * np, base = (size_t)x * np;
+    for (int t = threadIdx.x; t < ny; t += blockDim.x) ty[t] = twy[t];
+    for (int t = threadIdx.x; t < nz; t += blockDim.x) tz[t] = twz[t];
+    const long long* m0 = main + (size_t)q0 * ns + base;
+    long long* i0 = inflow + (size_t)q0 * ns + base;
+    const long long* m1 = main + (size_t)q1 * ns + base;
+    long long* i1 = inflow + (size_t)q1 * ns + base;
+    const bool two = q1 < NM;
+    for (int t = threadIdx.x; t < np; t += blockDim.x) {
+        const long long v0 = m0[t] + i0[t];
+        i0[t] = 0;
+        long long v1 = 0;
+        if (two) {
+            v1 = m1[t] + i1[t];
+            i1[t] = 0;
+        }
+        b0[t] = cplx<T>{(T)((double)v0 * scale), (T)((double)v1 * scale)};
+    }
+    if (c == 0) {
+        const long long* mc = main + (size_t)NM * ns + base;
+        long long* ic = inflow + (size_t)NM * ns + base;
+        for (int t = threadIdx.x; t < np; t += blockDim.x) {
+            const long long v = mc[t] + ic[t];
+            ic[t] = 0;
+            if (rho) {
+                const T r = (T)((double)v * rho_scale);
+                rho[base + t] = r;
+                rec[(base + t) * rs + REC_RHO] = r;
+            }
+        }
+    }
+    __syncthreads();
+    cplx<T>* r1 = fft_lines<T, false>(b0, b1, tz, az, ny, nz, 1, -1);
+    cplx<T>* r2 = fft_lines<T, true>(r1, r1 == b0 ? b1 : b0, ty, ay, nz, 1, nz, -1);
+    cplx<T>* out = spec + (size_t)c * ns + base;
+    for (int t = threadIdx.x; t < np; t += blockDim.x) out[t] = r2[t];
+}
+
+// (a - conj b) / 2i
+template <typename T>
+__device__ inline cplx<T> half_diff_over_i(cplx<T> a, cplx<T> b) {
+    return cplx<T>{(a.y + b.y) * T(0.5), (b.x - a.x) * T(0.5)};
+}
+
+// Pencil pairs g*pb .. g*pb+pb-1 of `pairs` (pencil p, mirror m; p == m for
+// pencils that are their own mirror).  npk packed stress components in, nout
+// outputs (u, W [, E]) out as nout/2 packed fields.  Line (c, s, w) of pair s,
+// pencil w (0: p, 1: m) sits at A[((c pb + s) 2 + w) (nx + 1)].
+template <typename T>
+__global__ void __launch_bounds__(SPEC_PENCIL_THREADS)
+    k_spec_pencils2(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, const uint2* __restrict__ pairs,
+                    int npairs, int np, int nz, int ny, int pb_max, int nsig, int nout, SpecAxis ax,
+                    const cplx<T>* __restrict__ twx, const T* __restrict__ kx, const T* __restrict__ ky,
+                    const T* __restrict__ kz, const T* __restrict__ ik2, const T* __restrict__ kinv, T screening,
+                    T norm) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    const int nx = ax.n, ls = nx + 1;
+    const int npk = (nsig + 1) / 2, nop = nout / 2;
+    const int nl = 2 * (npk > nop ? npk : nop) * pb_max;
+    cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+    cplx<T>* A = tw + nx;
+    cplx<T>* B = A + (size_t)nl * ls;
+    const size_t ns = (size_t)nx * np;
+    // XCD-aware order, as in k_spec_pencils
+    const int groups = gridDim.x;
+    const int g = groups % 8 == 0 ? (blockIdx.x % 8) * (groups / 8) + blockIdx.x / 8 : blockIdx.x;
+    const int s0 = g * pb_max;
+    const int pb = min(pb_max, npairs - s0);
+    for (int t = threadIdx.x; t < nx; t += blockDim.x) tw[t] = twx[t];
+    // lanes over (w, s) for loads of neighbouring pencils
+    for (int f = threadIdx.x; f < npk * nx * 2 * pb_max; f += blockDim.x) {
+        const int ws = f % (2 * pb_max), xx = (f / (2 * pb_max)) % nx, c = f / (2 * pb_max * nx);
+        const int s = ws >> 1, w = ws & 1;
+        cplx<T> v = cplx<T>{T(0), T(0)};
+        if (s < pb) {
+            const uint2 pr = pairs[s0 + s];
+            v = in[(size_t)c * ns + (size_t)xx * np + (w ? pr.y : pr.x)];
+        }
+        A[(size_t)((c * pb_max + s) * 2 + w) * ls + xx] = v;
+    }
+    __syncthreads();
+    cplx<T>* S = fft_lines<T, false>(A, B, tw, ax, npk * pb_max * 2, ls, 1, -1);
+    cplx<T>* O = S == A ? B : A;
+    // one thread per mirror pair of bins: (p, i) and (m, -i)
+    for (int f = threadIdx.x; f < nx * pb_max; f += blockDim.x) {
+        const int s = f % pb_max, i = f / pb_max;
+        const int im = i == 0 ? 0 : nx - i;
+        if (s >= pb) {
+            for (int d = 0; d < nop; ++d) {
+                O[(size_t)((d * pb_max + s) * 2 + 0) * ls + i] = cplx<T>{T(0), T(0)};
+                O[(size_t)((d * pb_max + s) * 2 + 1) * ls + i] = cplx<T>{T(0), T(0)};
+            }
+            continue;
+        }
+        const uint2 pr = pairs[s0 + s];
+        const int p = (int)pr.x, m = (int)pr.y;
+        if (p == m && i > im) continue;  // its own mirror: the pair (i, -i) once
+        // unpacked stress spectra at (p, i); at (m, -i) they are the conjugates
+        cplx<T> sg[9];
+        for (int c = 0; c < npk; ++c) {
+            const cplx<T> zp = S[(size_t)((c * pb_max + s) * 2 + 0) * ls + i];
+            const cplx<T> zm = S[(size_t)((c * pb_max + s) * 2 + 1) * ls + im];
+            sg[2 * c] = cplx<T>{(zp.x + zm.x) * T(0.5), (zp.y - zm.y) * T(0.5)};
+            if (2 * c + 1 < nsig) sg[2 * c + 1] = half_diff_over_i(zp, zm);
+        }
+        const size_t cp = (size_t)i * np + p, cm = (size_t)im * np + m;
+        const T k1[3] = {kx[i], ky[p / nz], kz[p % nz]};
+        const T k2[3] = {kx[im], ky[m / nz], kz[m % nz]};
+        T u1r[3], u1i[3], u2r[3], u2i[3];
+        flow_point<T>([&](int q) { return sg[q]; }, nsig, k1, kinv[cp], ik2[cp], screening, norm, u1r, u1i);
+        flow_point<T>([&](int q) { return cplx<T>{sg[q].x, -sg[q].y}; }, nsig, k2, kinv[cm], ik2[cm], screening,
+                      norm, u2r, u2i);
+        // outputs at k and at -k, then their Hermitian part
+        cplx<T> h[12];
+        for (int o = 0; o < nout; ++o) {
+            cplx<T> a, b;
+            if (o < 3) {
+                a = cplx<T>{u1r[o], u1i[o]};
+                b = cplx<T>{u2r[o], u2i[o]};
+            } else if (o < 6) {
+                a = vort_part(k1, u1r, u1i, o - 3);
+                b = vort_part(k2, u2r, u2i, o - 3);
+            } else {
+                a = strain_part(k1, u1r, u1i, o - 6);
+                b = strain_part(k2, u2r, u2i, o - 6);
+            }
+            h[o] = cplx<T>{(a.x + b.x) * T(0.5), (a.y - b.y) * T(0.5)};
+        }
+        // packed outputs 2d + i (2d+1) at (p, i) and, with H(-k) = conj H(k), at (m, -i)
+        for (int d = 0; d < nop; ++d) {
+            const cplx<T> h0 = h[2 * d], h1 = h[2 * d + 1];
+            O[(size_t)((d * pb_max + s) * 2 + 0) * ls + i] = cplx<T>{h0.x - h1.y, h0.y + h1.x};
+            O[(size_t)((d * pb_max + s) * 2 + 1) * ls + im] = cplx<T>{h0.x + h1.y, h1.x - h0.y};
+            if (p == m) {  // the same pencil: (p, -i) too
+                O[(size_t)((d * pb_max + s) * 2 + 0) * ls + im] = cplx<T>{h0.x + h1.y, h1.x - h0.y};
+                O[(size_t)((d * pb_max + s) * 2 + 1) * ls + i] = cplx<T>{h0.x - h1.y, h0.y + h1.x};
+            }
+        }
+    }
+    __syncthreads();
+    cplx<T>* R = fft_lines<T, false>(O, S, tw, ax, nop * pb_max * 2, ls, 1, +1);
+    for (int f = threadIdx.x; f < nop * nx * 2 * pb_max; f += blockDim.x) {
+        const int ws = f % (2 * pb_max), xx = (f / (2 * pb_max)) % nx, d = f / (2 * pb_max * nx);
+        const int s = ws >> 1, w = ws & 1;
+        if (s >= pb) continue;
+        const uint2 pr = pairs[s0 + s];
+        if (w == 1 && pr.x == pr.y) continue;
+        out[(size_t)d * ns + (size_t)xx * np + (w ? pr.y : pr.x)] = R[(size_t)((d * pb_max + s) * 2 + w) * ls + xx];
+    }
+}
+
+// Inverse: packed field d (outputs 2d + i 2d+1) of plane x; the real part is
+// output 2d, the imaginary part output 2d+1.
+template <typename T>
+__global__ void __launch_bounds__(SPEC_PLANE_THREADS)
+    k_spec_planes_inv2(const cplx<T>* __restrict__ spec, int nx, SpecAxis ay, SpecAxis az,
+                       const cplx<T>* __restrict__ twy, const cplx<T>* __restrict__ twz, T* __restrict__ rec,
+                       int rs) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    const int ny = ay.n, nz = az.n, np = ny * nz;
+    cplx<T>* ty = reinterpret_cast<cplx<T>*>(smem_raw);
+    cplx<T>* tz = ty + ny;
+    cplx<T>* b0 = tz + nz;
+    cplx<T>* b1 = b0 + np;
+    const int x = blockIdx.x % nx, d = blockIdx.x / nx;
+    const size_t ns = (size_t)nx * np, base = (size_t)x * np;
+    for (int t = threadIdx.x; t < ny; t += blockDim.x) ty[t] = twy[t];
+    for (int t = threadIdx.x; t < nz; t += blockDim.x) tz[t] = twz[t];
+    const cplx<T>* src = spec + (size_t)d * ns + base;
+    for (int t = threadIdx.x; t < np; t += blockDim.x) b0[t] = src[t];
+    __syncthreads();
+    cplx<T>* r1 = fft_lines<T, true>(b0, b1, ty, ay, nz, 1, nz, +1);
+    cplx<T>* r2 = fft_lines<T, false>(r1, r1 == b0 ? b1 : b0, tz, az, ny, nz, 1, +1);
+    const int o0 = 2 * d, o1 = 2 * d + 1;
+    const int sl0 = o0 < 6 ? o0 : REC_E + o0 - 6, sl1 = o1 < 6 ? o1 : REC_E + o1 - 6;
+    for (int t = threadIdx.x; t < np; t += blockDim.x) {
+        rec[(base + t) * rs + sl0] = r2[t].x;
+        rec[(base + t) * rs + sl1] = r2[t].y;
+    }
+}
+
+// Getter path: packed z,y-transformed moments -> the per-component layout
+// (plane x keeps its own Hermitian symmetry in (y, z)).
+template <typename T>
+__global__ void __launch_bounds__(256) k_spec_unpack(const cplx<T>* __restrict__ packed, int nsig, int nx, int ny,
+                                                     int nz, cplx<T>* __restrict__ out) {
+    const size_t np = (size_t)ny * nz, ns = (size_t)nx * np;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ns) return;
+    const size_t x = t / np, p = t % np;
+    const int y = (int)(p / nz), z = (int)(p % nz);
+    const size_t pm = (size_t)((ny - y) % ny) * nz + (size_t)((nz - z) % nz);
+    for (int c = 0; 2 * c < nsig; ++c) {
+        const cplx<T> zp = packed[(size_t)c * ns + x * np + p];
+        const cplx<T> zm = packed[(size_t)c * ns + x * np + pm];
+        out[(size_t)(2 * c) * ns + t] = cplx<T>{(zp.x + zm.x) * T(0.5), (zp.y - zm.y) * T(0.5)};
+        if (2 * c + 1 < nsig) out[(size_t)(2 * c + 1) * ns + t] = half_diff_over_i(zp, zm);
+    }
+}
+
 }  // namespace ss

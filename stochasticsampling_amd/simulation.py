@@ -198,6 +198,7 @@ class Simulation:
         return {
             "sorted": bool(i.sorted),
             "fused_spectral": bool(i.fused_spectral),
+            "paired_spectral": i.fused_spectral == 2,
             "tile": tuple(i.tile),
             "ntiles": i.ntiles,
             "resort_interval": i.resort_interval,

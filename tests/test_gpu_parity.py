@@ -361,8 +361,10 @@ def test_fused_spectral_stage_equals_axis_passes(precision, grid, extra):
     box = tuple(0.8 * g for g in grid[:3])
     kw = dict(grid=grid, box=box, n=6000, precision=precision, binning=ss.BIN_SORTED, resort_interval=3, **extra)
     parts = ss.initial_condition(small_params(**kw), amplitude=0.1, seed=12)
-    sims = [ss.Simulation(small_params(flags=flags | f, **kw)) for f in (0, ss.FLAG_AXIS_PASSES)]
+    sims = [ss.Simulation(small_params(flags=flags | f, **kw))
+            for f in (ss.FLAG_UNPAIRED_SPECTRAL, ss.FLAG_AXIS_PASSES)]
     assert sims[0].get_info()["fused_spectral"] and not sims[1].get_info()["fused_spectral"]
+    assert not sims[0].get_info()["paired_spectral"]
     out = []
     for sim in sims:
         sim.init(parts)
@@ -374,6 +376,44 @@ def test_fused_spectral_stage_equals_axis_passes(precision, grid, extra):
         np.testing.assert_array_equal(sim.get_flow_field(), u)
     for a, b in zip(out[0], out[1]):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision,tol", [(ss.FP64, 1e-13), (ss.FP32, 2e-6)])
+@pytest.mark.parametrize("grid,extra", [
+    ((16, 16, 16, 8, 8), {}),
+    ((12, 10, 15, 6, 6), {"shape": 0.6, "volume_exclusion": 0.3}),
+    ((15, 9, 5, 4, 6), {"shape": -0.4}),
+    ((8, 12, 16, 6, 4), {"hydro_screening": 0.5}),
+])
+def test_paired_spectral_stage_matches_unpaired(precision, tol, grid, extra):
+    """Two real fields per complex transform, a pencil solved with its mirror
+    and the Hermitian part of each output: the same fields as one transform
+    per component up to rounding -- on even and odd grids (Nyquist planes
+    or none), with the strain (12 outputs), the density and screening."""
+    box = tuple(0.8 * g for g in grid[:3])
+    kw = dict(grid=grid, box=box, n=6000, precision=precision, binning=ss.BIN_SORTED, resort_interval=3,
+              flags=ss.FLAG_KEEP_DISTRIBUTION, **extra)
+    parts = ss.initial_condition(small_params(**kw), amplitude=0.1, seed=13)
+    a = ss.Simulation(small_params(**kw))
+    kw["flags"] |= ss.FLAG_UNPAIRED_SPECTRAL
+    b = ss.Simulation(small_params(**kw))
+    assert a.get_info()["paired_spectral"] and not b.get_info()["paired_spectral"]
+    for sim in (a, b):
+        sim.init(parts)
+        sim.step(1)
+    ua, ga = a.get_flow_field(with_gradient=True)
+    ub, gb = b.get_flow_field(with_gradient=True)
+    assert rel_err(ua, ub) < tol and rel_err(ga, gb) < tol
+    np.testing.assert_array_equal(a.get_counts(), b.get_counts())
+    # one more step through the records the paired stage wrote
+    for sim in (a, b):
+        sim.step(1)
+    pa, pb = a.get_particles(), b.get_particles()
+    L = np.array(box)
+    d = np.abs(pa[:, :3] - pb[:, :3])
+    d = np.minimum(d, L - d)
+    assert np.max(d) < 1e3 * tol * max(L)
+    assert np.max(np.abs(pa[:, 4] - pb[:, 4])) < 1e3 * tol
 
 
 @pytest.mark.parametrize("precision", [ss.FP32, ss.FP64])
