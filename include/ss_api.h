@@ -63,10 +63,10 @@ extern "C" {
 #define SS_FLAG_AXIS_PASSES 16u       /* tiled path: run the spectral stage as separate kernels
                                          (convert, one pass per FFT axis, solve, pack) instead of
                                          the three fused plane/pencil kernels (same results)      */
-#define SS_FLAG_UNPAIRED_SPECTRAL 32u /* fused spectral stage with one complex transform per real
-                                         component instead of two real fields per transform
-                                         (bit-identical to SS_FLAG_AXIS_PASSES; the paired default
-                                         agrees to rounding)                                      */
+#define SS_FLAG_PAIRED_SPECTRAL 32u   /* fused spectral stage with two real fields per complex
+                                         transform (pencils solved with their mirrors) instead of
+                                         one transform per component: half the transforms, the same
+                                         fields to rounding; slower at 64^3 on MI355X             */
 
 /* Size of the simulation box (lib.rs BoxSize). */
 typedef struct ss_box {

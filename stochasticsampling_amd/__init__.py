@@ -12,7 +12,7 @@ from .params import (  # noqa: F401
     FLAG_MAGNETIC_ALWAYS,
     FLAG_NO_GRAPH,
     FLAG_QUASI2D,
-    FLAG_UNPAIRED_SPECTRAL,
+    FLAG_PAIRED_SPECTRAL,
     FP32,
     FP64,
     NOISE_EXTERNAL,

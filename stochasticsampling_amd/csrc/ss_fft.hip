@@ -121,10 +121,10 @@ __global__ void __launch_bounds__(256) k_fft_axis(cplx<T>* __restrict__ data, Ax
     for (int s = 0; s < d.nf; ++s) {
         const int R = d.fac[s];
         switch (R) {
-            case 4: stage<T, 4, false>(src, dst, tw, d.n, Ns, nl, d.n, 1, sign); break;
-            case 2: stage<T, 2, false>(src, dst, tw, d.n, Ns, nl, d.n, 1, sign); break;
-            case 3: stage<T, 3, false>(src, dst, tw, d.n, Ns, nl, d.n, 1, sign); break;
-            case 5: stage<T, 5, false>(src, dst, tw, d.n, Ns, nl, d.n, 1, sign); break;
+            case 4: stage<T, 4, false, 256>(src, dst, tw, d.n, Ns, nl, d.n, 1, sign); break;
+            case 2: stage<T, 2, false, 256>(src, dst, tw, d.n, Ns, nl, d.n, 1, sign); break;
+            case 3: stage<T, 3, false, 256>(src, dst, tw, d.n, Ns, nl, d.n, 1, sign); break;
+            case 5: stage<T, 5, false, 256>(src, dst, tw, d.n, Ns, nl, d.n, 1, sign); break;
             default:
                 if constexpr (GENERIC) stage_generic<T>(src, dst, tw, d.n, Ns, R, nl, sign);
                 break;

@@ -53,15 +53,20 @@ __device__ inline void dft_small(cplx<T>* v, const cplx<T>* tw, int n, int sign)
 }
 
 // One radix-R Stockham stage over nlines lines of length n (Ns = product of
-// the radices of the earlier stages).  LINES_FAST: consecutive threads take
-// consecutive lines (lines interleaved in memory, es > 1); otherwise
-// consecutive butterflies of one line (contiguous lines, es = 1).
-template <typename T, int R, bool LINES_FAST>
-__device__ inline void stage(const cplx<T>* src, cplx<T>* dst, const cplx<T>* tw, int n, int Ns, int nlines,
+// the radices of the earlier stages) by a workgroup of NT threads.
+// LINES_FAST: consecutive threads take consecutive lines (lines interleaved in
+// memory, es > 1); otherwise consecutive butterflies of one line (contiguous
+// lines, es = 1).  NC > 0 fixes n = NC and Ns = NSC at compile time, NLC > 0
+// the number of lines (the same arithmetic, with the index divisions folded).
+template <typename T, int R, bool LINES_FAST, int NT, int NC = 0, int NSC = 0, int NLC = 0>
+__device__ inline void stage(const cplx<T>* src, cplx<T>* dst, const cplx<T>* tw, int n_, int Ns_, int nlines_,
                              int ls, int es, int sign) {
+    const int n = NC ? NC : n_;
+    const int Ns = NC ? NSC : Ns_;
+    const int nlines = NLC ? NLC : nlines_;
     const int m = n / R;
     const int tstep = n / (Ns * R);
-    for (int b = threadIdx.x; b < nlines * m; b += blockDim.x) {
+    for (int b = threadIdx.x; b < nlines * m; b += NT) {
         int l, j;
         if (LINES_FAST) {
             l = b % nlines;
@@ -99,7 +104,7 @@ struct SpecAxis {
 
 // All stages of one axis, ping-ponging between a and b; returns the buffer
 // holding the result.  Ends with a barrier.
-template <typename T, bool LINES_FAST>
+template <typename T, bool LINES_FAST, int NT>
 __device__ inline cplx<T>* fft_lines(cplx<T>* a, cplx<T>* b, const cplx<T>* tw, const SpecAxis& ax, int nlines,
                                      int ls, int es, int sign) {
     cplx<T>* src = a;
@@ -108,10 +113,10 @@ __device__ inline cplx<T>* fft_lines(cplx<T>* a, cplx<T>* b, const cplx<T>* tw, 
     for (int s = 0; s < ax.nf; ++s) {
         const int R = ax.fac[s];
         switch (R) {
-            case 4: stage<T, 4, LINES_FAST>(src, dst, tw, ax.n, Ns, nlines, ls, es, sign); break;
-            case 2: stage<T, 2, LINES_FAST>(src, dst, tw, ax.n, Ns, nlines, ls, es, sign); break;
-            case 3: stage<T, 3, LINES_FAST>(src, dst, tw, ax.n, Ns, nlines, ls, es, sign); break;
-            default: stage<T, 5, LINES_FAST>(src, dst, tw, ax.n, Ns, nlines, ls, es, sign); break;
+            case 4: stage<T, 4, LINES_FAST, NT>(src, dst, tw, ax.n, Ns, nlines, ls, es, sign); break;
+            case 2: stage<T, 2, LINES_FAST, NT>(src, dst, tw, ax.n, Ns, nlines, ls, es, sign); break;
+            case 3: stage<T, 3, LINES_FAST, NT>(src, dst, tw, ax.n, Ns, nlines, ls, es, sign); break;
+            default: stage<T, 5, LINES_FAST, NT>(src, dst, tw, ax.n, Ns, nlines, ls, es, sign); break;
         }
         __syncthreads();
         cplx<T>* t = src;
@@ -120,6 +125,36 @@ __device__ inline cplx<T>* fft_lines(cplx<T>* a, cplx<T>* b, const cplx<T>* tw, 
         Ns *= R;
     }
     return src;
+}
+
+// fft_lines for a power-of-two length N known at compile time (radix-4
+// stages, then a radix 2 when log2 N is odd: the plan factorize() makes), over
+// NL lines when NL > 0.
+template <typename T, bool LINES_FAST, int NT, int N, int NL, int NS = 1>
+__device__ inline cplx<T>* fft_lines_pow2(cplx<T>* src, cplx<T>* dst, const cplx<T>* tw, int nlines, int ls,
+                                          int es, int sign) {
+    static_assert(N > 0 && (N & (N - 1)) == 0, "power of two");
+    if constexpr (NS >= N) {
+        return src;
+    } else {
+        constexpr int R = (N / NS) % 4 == 0 ? 4 : 2;
+        stage<T, R, LINES_FAST, NT, N, NS, NL>(src, dst, tw, N, NS, nlines, ls, es, sign);
+        __syncthreads();
+        return fft_lines_pow2<T, LINES_FAST, NT, N, NL, NS * R>(dst, src, tw, nlines, ls, es, sign);
+    }
+}
+
+// Whether the runtime plan of an axis is the one fft_lines_pow2<N> runs.
+inline bool pow2_plan(const SpecAxis& a, int N) {
+    if (a.n != N || N <= 1 || (N & (N - 1))) return false;
+    int m = N, k = 0;
+    while (m > 1) {
+        const int r = m % 4 == 0 ? 4 : 2;
+        if (k >= a.nf || a.fac[k] != r) return false;
+        m /= r;
+        ++k;
+    }
+    return k == a.nf;
 }
 
 }  // namespace ss

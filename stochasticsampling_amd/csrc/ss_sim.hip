@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -157,6 +158,7 @@ struct ss_sim {
     uint2* pairs = nullptr;  // (pencil, mirror pencil) with pencil <= mirror
     int npairs = 0;
     int pencils = 8;  // (y, z) pencils per workgroup of the pencil kernel
+    int spec_p = 0;   // 16, 32 or 64 when all three axes have that length (stages fixed at compile time)
     uint32_t* home_ctr = nullptr;  // re-sort: particles placed from a segment's front
     bool mag = false;  // magnetic solve runs
     bool use_e = false, use_rho = false, use_b = false, use_gb = false;
@@ -387,6 +389,18 @@ static SpecAxis spec_axis(const FftAxis& a) {
     return r;
 }
 
+// Calls f with std::integral_constant<int, P> for the compile-time axis length
+// of the fused spectral kernels (0: lengths at run time).
+template <typename F>
+static auto with_spec_p(int p, F f) {
+    switch (p) {
+        case 16: return f(std::integral_constant<int, 16>{});
+        case 32: return f(std::integral_constant<int, 32>{});
+        case 64: return f(std::integral_constant<int, 64>{});
+        default: return f(std::integral_constant<int, 0>{});
+    }
+}
+
 template <typename T>
 static hipError_t launch_spec_planes_fwd(ss_sim* s) {
     double ms, ds;
@@ -397,15 +411,18 @@ static hipError_t launch_spec_planes_fwd(ss_sim* s) {
     const SpecAxis ay = spec_axis(s->fft.ax[1]), az = spec_axis(s->fft.ax[2]);
     const cplx<T>* twy = (const cplx<T>*)s->fft.ax[1].tw;
     const cplx<T>* twz = (const cplx<T>*)s->fft.ax[2].tw;
-    if (s->nm == 6)
-        hipLaunchKernelGGL((k_spec_planes_fwd<T, 6>), grid, dim3(SPEC_PLANE_THREADS), lds, s->stream, s->mom_main, s->mom_inflow,
-                           s->nx, s->fixed_scale * ms, ds, ay, az, twy, twz, (cplx<T>*)s->spec_in, rho, (T*)s->rec,
-                           s->rs);
-    else
-        hipLaunchKernelGGL((k_spec_planes_fwd<T, 9>), grid, dim3(SPEC_PLANE_THREADS), lds, s->stream, s->mom_main, s->mom_inflow,
-                           s->nx, s->fixed_scale * ms, ds, ay, az, twy, twz, (cplx<T>*)s->spec_in, rho, (T*)s->rec,
-                           s->rs);
-    return hipGetLastError();
+    return with_spec_p(s->spec_p, [&](auto pc) {
+        constexpr int P = decltype(pc)::value;
+        if (s->nm == 6)
+            hipLaunchKernelGGL((k_spec_planes_fwd<T, 6, P>), grid, dim3(SPEC_PLANE_THREADS), lds, s->stream,
+                               s->mom_main, s->mom_inflow, s->nx, s->fixed_scale * ms, ds, ay, az, twy, twz,
+                               (cplx<T>*)s->spec_in, rho, (T*)s->rec, s->rs);
+        else
+            hipLaunchKernelGGL((k_spec_planes_fwd<T, 9, P>), grid, dim3(SPEC_PLANE_THREADS), lds, s->stream,
+                               s->mom_main, s->mom_inflow, s->nx, s->fixed_scale * ms, ds, ay, az, twy, twz,
+                               (cplx<T>*)s->spec_in, rho, (T*)s->rec, s->rs);
+        return hipGetLastError();
+    });
 }
 
 template <typename T>
@@ -414,11 +431,15 @@ static hipError_t launch_spec_pencils(ss_sim* s) {
     const int nout = s->use_e ? 12 : 6;
     const size_t lds = spec_pencil_lds(s->nx, std::max(s->nsig, nout) * s->pencils, sizeof(T));
     const dim3 grid((unsigned)((np + s->pencils - 1) / s->pencils));
-    hipLaunchKernelGGL(k_spec_pencils<T>, grid, dim3(SPEC_PENCIL_THREADS), lds, s->stream, (const cplx<T>*)s->spec_in,
-                       (cplx<T>*)s->spec_out, np, s->nz, s->pencils, s->nsig, nout, spec_axis(s->fft.ax[0]),
-                       (const cplx<T>*)s->fft.ax[0].tw, (const T*)s->kx, (const T*)s->ky, (const T*)s->kz,
-                       (const T*)s->ik2, (const T*)s->kinv, (T)s->p.hydro_screening, (T)(double)s->ns);
-    return hipGetLastError();
+    return with_spec_p(s->spec_p, [&](auto pc) {
+        constexpr int P = decltype(pc)::value;
+        hipLaunchKernelGGL((k_spec_pencils<T, P>), grid, dim3(SPEC_PENCIL_THREADS), lds, s->stream,
+                           (const cplx<T>*)s->spec_in, (cplx<T>*)s->spec_out, np, s->nz, s->pencils, s->nsig, nout,
+                           spec_axis(s->fft.ax[0]), (const cplx<T>*)s->fft.ax[0].tw, (const T*)s->kx,
+                           (const T*)s->ky, (const T*)s->kz, (const T*)s->ik2, (const T*)s->kinv,
+                           (T)s->p.hydro_screening, (T)(double)s->ns);
+        return hipGetLastError();
+    });
 }
 
 template <typename T>
@@ -432,15 +453,18 @@ static hipError_t launch_spec_planes_fwd2(ss_sim* s) {
     const SpecAxis ay = spec_axis(s->fft.ax[1]), az = spec_axis(s->fft.ax[2]);
     const cplx<T>* twy = (const cplx<T>*)s->fft.ax[1].tw;
     const cplx<T>* twz = (const cplx<T>*)s->fft.ax[2].tw;
-    if (s->nm == 6)
-        hipLaunchKernelGGL((k_spec_planes_fwd2<T, 6>), grid, dim3(SPEC_PLANE_THREADS), lds, s->stream, s->mom_main,
-                           s->mom_inflow, s->nx, s->fixed_scale * ms, ds, ay, az, twy, twz, (cplx<T>*)s->spec_in, rho,
-                           (T*)s->rec, s->rs);
-    else
-        hipLaunchKernelGGL((k_spec_planes_fwd2<T, 9>), grid, dim3(SPEC_PLANE_THREADS), lds, s->stream, s->mom_main,
-                           s->mom_inflow, s->nx, s->fixed_scale * ms, ds, ay, az, twy, twz, (cplx<T>*)s->spec_in, rho,
-                           (T*)s->rec, s->rs);
-    return hipGetLastError();
+    return with_spec_p(s->spec_p, [&](auto pc) {
+        constexpr int P = decltype(pc)::value;
+        if (s->nm == 6)
+            hipLaunchKernelGGL((k_spec_planes_fwd2<T, 6, P>), grid, dim3(SPEC_PLANE_THREADS), lds, s->stream,
+                               s->mom_main, s->mom_inflow, s->nx, s->fixed_scale * ms, ds, ay, az, twy, twz,
+                               (cplx<T>*)s->spec_in, rho, (T*)s->rec, s->rs);
+        else
+            hipLaunchKernelGGL((k_spec_planes_fwd2<T, 9, P>), grid, dim3(SPEC_PLANE_THREADS), lds, s->stream,
+                               s->mom_main, s->mom_inflow, s->nx, s->fixed_scale * ms, ds, ay, az, twy, twz,
+                               (cplx<T>*)s->spec_in, rho, (T*)s->rec, s->rs);
+        return hipGetLastError();
+    });
 }
 
 template <typename T>
@@ -450,12 +474,15 @@ static hipError_t launch_spec_pencils2(ss_sim* s) {
     const int npk = (s->nsig + 1) / 2;
     const size_t lds = spec_pencil_lds(s->nx, 2 * std::max(npk, nout / 2) * s->pencils, sizeof(T));
     const dim3 grid((unsigned)((s->npairs + s->pencils - 1) / s->pencils));
-    hipLaunchKernelGGL(k_spec_pencils2<T>, grid, dim3(SPEC_PENCIL_THREADS), lds, s->stream,
-                       (const cplx<T>*)s->spec_in, (cplx<T>*)s->spec_out, (const uint2*)s->pairs, s->npairs, np, s->nz,
-                       s->ny, s->pencils, s->nsig, nout, spec_axis(s->fft.ax[0]), (const cplx<T>*)s->fft.ax[0].tw,
-                       (const T*)s->kx, (const T*)s->ky, (const T*)s->kz, (const T*)s->ik2, (const T*)s->kinv,
-                       (T)s->p.hydro_screening, (T)(double)s->ns);
-    return hipGetLastError();
+    return with_spec_p(s->spec_p, [&](auto pc) {
+        constexpr int P = decltype(pc)::value;
+        hipLaunchKernelGGL((k_spec_pencils2<T, P>), grid, dim3(SPEC_PENCIL_THREADS), lds, s->stream,
+                           (const cplx<T>*)s->spec_in, (cplx<T>*)s->spec_out, (const uint2*)s->pairs, s->npairs, np,
+                           s->nz, s->ny, s->pencils, s->nsig, nout, spec_axis(s->fft.ax[0]),
+                           (const cplx<T>*)s->fft.ax[0].tw, (const T*)s->kx, (const T*)s->ky, (const T*)s->kz,
+                           (const T*)s->ik2, (const T*)s->kinv, (T)s->p.hydro_screening, (T)(double)s->ns);
+        return hipGetLastError();
+    });
 }
 
 template <typename T>
@@ -463,10 +490,13 @@ static hipError_t launch_spec_planes_inv2(ss_sim* s) {
     const int nout = s->use_e ? 12 : 6;
     const size_t lds = spec_plane_lds(s->ny, s->nz, sizeof(T));
     const dim3 grid((unsigned)(s->nx * (nout / 2)));
-    hipLaunchKernelGGL(k_spec_planes_inv2<T>, grid, dim3(SPEC_PLANE_THREADS), lds, s->stream,
-                       (const cplx<T>*)s->spec_out, s->nx, spec_axis(s->fft.ax[1]), spec_axis(s->fft.ax[2]),
-                       (const cplx<T>*)s->fft.ax[1].tw, (const cplx<T>*)s->fft.ax[2].tw, (T*)s->rec, s->rs);
-    return hipGetLastError();
+    return with_spec_p(s->spec_p, [&](auto pc) {
+        constexpr int P = decltype(pc)::value;
+        hipLaunchKernelGGL((k_spec_planes_inv2<T, P>), grid, dim3(SPEC_PLANE_THREADS), lds, s->stream,
+                           (const cplx<T>*)s->spec_out, s->nx, spec_axis(s->fft.ax[1]), spec_axis(s->fft.ax[2]),
+                           (const cplx<T>*)s->fft.ax[1].tw, (const cplx<T>*)s->fft.ax[2].tw, (T*)s->rec, s->rs);
+        return hipGetLastError();
+    });
 }
 
 template <typename T>
@@ -474,10 +504,13 @@ static hipError_t launch_spec_planes_inv(ss_sim* s) {
     const int nout = s->use_e ? 12 : 6;
     const size_t lds = spec_plane_lds(s->ny, s->nz, sizeof(T));
     const dim3 grid((unsigned)(s->nx * nout));
-    hipLaunchKernelGGL(k_spec_planes_inv<T>, grid, dim3(SPEC_PLANE_THREADS), lds, s->stream, (const cplx<T>*)s->spec_out, s->nx,
-                       spec_axis(s->fft.ax[1]), spec_axis(s->fft.ax[2]), (const cplx<T>*)s->fft.ax[1].tw,
-                       (const cplx<T>*)s->fft.ax[2].tw, (T*)s->rec, s->rs);
-    return hipGetLastError();
+    return with_spec_p(s->spec_p, [&](auto pc) {
+        constexpr int P = decltype(pc)::value;
+        hipLaunchKernelGGL((k_spec_planes_inv<T, P>), grid, dim3(SPEC_PLANE_THREADS), lds, s->stream,
+                           (const cplx<T>*)s->spec_out, s->nx, spec_axis(s->fft.ax[1]), spec_axis(s->fft.ax[2]),
+                           (const cplx<T>*)s->fft.ax[1].tw, (const cplx<T>*)s->fft.ax[2].tw, (T*)s->rec, s->rs);
+        return hipGetLastError();
+    });
 }
 
 template <typename T, int NM, bool MOVE, int NZ, bool KEEP>
@@ -914,21 +947,44 @@ static int choose_spectral(ss_sim* s) {
     const size_t cap = std::min<size_t>((size_t)lim, 144 * 1024);
     const size_t plane = spec_plane_lds(s->ny, s->nz, sizeof(T));
     const int nmax = std::max(s->nsig, s->nout);
-    int pb = 16;
+    // four pencils (pencil pairs) per workgroup: at 64^3 more workgroups of
+    // fewer lines finish sooner than fewer larger ones, down to this size
+    int pb = 4;
     while (pb > 1 && spec_pencil_lds(s->nx, nmax * pb, sizeof(T)) > 64 * 1024) pb /= 2;
     const size_t pencil = spec_pencil_lds(s->nx, nmax * pb, sizeof(T));
     if (plane > cap || pencil > cap) return SS_OK;
-    SS_HIP_CHECK(hipFuncSetAttribute((const void*)k_spec_planes_fwd<T, 6>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane));
-    SS_HIP_CHECK(hipFuncSetAttribute((const void*)k_spec_planes_fwd<T, 9>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane));
-    SS_HIP_CHECK(hipFuncSetAttribute((const void*)k_spec_planes_inv<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)plane));
-    SS_HIP_CHECK(hipFuncSetAttribute((const void*)k_spec_pencils<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)pencil));
+    const SpecAxis a0 = spec_axis(s->fft.ax[0]), a1 = spec_axis(s->fft.ax[1]), a2 = spec_axis(s->fft.ax[2]);
+    s->spec_p = 0;
+    for (int P : {16, 32, 64})
+        if (pow2_plan(a0, P) && pow2_plan(a1, P) && pow2_plan(a2, P)) s->spec_p = P;
+    const bool pair = (s->p.flags & SS_FLAG_PAIRED_SPECTRAL) != 0;
+    hipError_t ae = with_spec_p(s->spec_p, [&](auto pc) {
+        constexpr int P = decltype(pc)::value;
+        // the cap, not this simulation's size: the attribute belongs to the
+        // kernel, which simulations of other sizes launch as well
+        const int pl = (int)cap, pe = (int)cap;
+        const hipFuncAttribute at = hipFuncAttributeMaxDynamicSharedMemorySize;
+        hipError_t e = hipSuccess;
+        auto set = [&](const void* f, int v) {
+            if (e == hipSuccess) e = hipFuncSetAttribute(f, at, v);
+        };
+        if (pair) {
+            set((const void*)k_spec_planes_fwd2<T, 6, P>, pl);
+            set((const void*)k_spec_planes_fwd2<T, 9, P>, pl);
+            set((const void*)k_spec_planes_inv2<T, P>, pl);
+            set((const void*)k_spec_pencils2<T, P>, pe);
+        } else {
+            set((const void*)k_spec_planes_fwd<T, 6, P>, pl);
+            set((const void*)k_spec_planes_fwd<T, 9, P>, pl);
+            set((const void*)k_spec_planes_inv<T, P>, pl);
+            set((const void*)k_spec_pencils<T, P>, pe);
+        }
+        return e;
+    });
+    SS_HIP_CHECK(ae);
     s->pencils = pb;
     s->fused = true;
-    if (!(s->p.flags & SS_FLAG_UNPAIRED_SPECTRAL)) {
+    if (pair) {
         // pencil pairs (p, mirror p); a pair holds two pencils of half the
         // components, so the same pencil-group size fits the same LDS
         const int ny = s->ny, nz = s->nz, np = ny * nz;
@@ -938,14 +994,6 @@ static int choose_spectral(ss_sim* s) {
             const int m = ((ny - y) % ny) * nz + (nz - z) % nz;
             if (p <= m) pr.push_back(make_uint2((unsigned)p, (unsigned)m));
         }
-        SS_HIP_CHECK(hipFuncSetAttribute((const void*)k_spec_planes_fwd2<T, 6>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane));
-        SS_HIP_CHECK(hipFuncSetAttribute((const void*)k_spec_planes_fwd2<T, 9>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane));
-        SS_HIP_CHECK(hipFuncSetAttribute((const void*)k_spec_planes_inv2<T>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane));
-        SS_HIP_CHECK(hipFuncSetAttribute((const void*)k_spec_pencils2<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)pencil));
         int rc = alloc(s, (void**)&s->pairs, pr.size() * sizeof(uint2));
         if (rc != SS_OK) return rc;
         SS_HIP_CHECK(hipMemcpy(s->pairs, pr.data(), pr.size() * sizeof(uint2), hipMemcpyHostToDevice));

@@ -24,55 +24,106 @@ namespace ss {
 // workgroup's LDS-bound stages more waves to overlap.
 constexpr int SPEC_PLANE_THREADS = 512;
 constexpr int SPEC_PENCIL_THREADS = 512;
+constexpr int SPEC_MAX_PENCILS = 16;  // pencils (or pencil pairs) per workgroup
+
+// A strided loop over t < n whose global loads are issued U iterations at a
+// time, the stores after them: a loop that consumes each load before the next
+// one waits for the memory latency once per iteration.
+template <int NT, int U = 8, typename Load, typename Store>
+__device__ inline void batched_loop(int n, Load load, Store store) {
+    constexpr int st = NT;
+    for (int b = threadIdx.x; b < n; b += U * st) {
+        decltype(load(0)) v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (b + u * st < n) v[u] = load(b + u * st);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (b + u * st < n) store(b + u * st, v[u]);
+    }
+}
+
+// The transform of nlines lines of one axis: with P > 0 every axis has the
+// power-of-two length P and the stages run with their sizes fixed at compile
+// time (NL > 0: nlines = NL as well); the arithmetic is the same.
+template <typename T, bool LINES_FAST, int NT, int P, int NL>
+__device__ inline cplx<T>* spec_lines(cplx<T>* a, cplx<T>* b, const cplx<T>* tw, const SpecAxis& ax, int nlines,
+                                      int ls, int es, int sign) {
+    if constexpr (P > 0)
+        return fft_lines_pow2<T, LINES_FAST, NT, P, NL>(a, b, tw, nlines, ls, es, sign);
+    else
+        return fft_lines<T, LINES_FAST, NT>(a, b, tw, ax, nlines, ls, es, sign);
+}
 
 // LDS of one plane kernel: twiddles of y and z, two plane buffers.
 inline size_t spec_plane_lds(int ny, int nz, size_t esz) { return esz * 2 * ((size_t)ny + nz + 2 * (size_t)ny * nz); }
 
 // Forward: component q of plane x.  Blocks with q == 0 also convert the count
 // component (density, when wanted).  Clears the inflow entries it reads.
-template <typename T, int NM>
+template <typename T, int NM, int P>
 __global__ void __launch_bounds__(SPEC_PLANE_THREADS)
     k_spec_planes_fwd(const long long* __restrict__ main, long long* __restrict__ inflow, int nx, double scale,
                       double rho_scale, SpecAxis ay, SpecAxis az, const cplx<T>* __restrict__ twy,
                       const cplx<T>* __restrict__ twz, cplx<T>* __restrict__ spec, T* __restrict__ rho,
                       T* __restrict__ rec, int rs) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-    const int ny = ay.n, nz = az.n, np = ny * nz;
+    const int ny = P ? P : ay.n, nz = P ? P : az.n, np = ny * nz;
     cplx<T>* ty = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* tz = ty + ny;
     cplx<T>* b0 = tz + nz;
     cplx<T>* b1 = b0 + np;
     const int x = blockIdx.x % nx, q = blockIdx.x / nx;
     const size_t ns = (size_t)nx * np, base = (size_t)x * np;
-    for (int t = threadIdx.x; t < ny; t += blockDim.x) ty[t] = twy[t];
-    for (int t = threadIdx.x; t < nz; t += blockDim.x) tz[t] = twz[t];
+    for (int t = threadIdx.x; t < ny; t += SPEC_PLANE_THREADS) ty[t] = twy[t];
+    for (int t = threadIdx.x; t < nz; t += SPEC_PLANE_THREADS) tz[t] = twz[t];
     const long long* mq = main + (size_t)q * ns + base;
     long long* iq = inflow + (size_t)q * ns + base;
-    for (int c = threadIdx.x; c < np; c += blockDim.x) {
-        const long long v = mq[c] + iq[c];
-        iq[c] = 0;
-        b0[c] = cplx<T>{(T)((double)v * scale), T(0)};
-    }
+    batched_loop<SPEC_PLANE_THREADS>(
+        np, [&](int c) { return mq[c] + iq[c]; },
+        [&](int c, long long v) {
+            iq[c] = 0;
+            b0[c] = cplx<T>{(T)((double)v * scale), T(0)};
+        });
     if (q == 0) {
         const long long* mc = main + (size_t)NM * ns + base;
         long long* ic = inflow + (size_t)NM * ns + base;
-        for (int c = threadIdx.x; c < np; c += blockDim.x) {
-            const long long v = mc[c] + ic[c];
-            ic[c] = 0;
-            if (rho) {
-                const T r = (T)((double)v * rho_scale);
-                rho[base + c] = r;
-                rec[(base + c) * rs + REC_RHO] = r;
-            }
-        }
+        batched_loop<SPEC_PLANE_THREADS>(
+            np, [&](int c) { return mc[c] + ic[c]; },
+            [&](int c, long long v) {
+                ic[c] = 0;
+                if (rho) {
+                    const T r = (T)((double)v * rho_scale);
+                    rho[base + c] = r;
+                    rec[(base + c) * rs + REC_RHO] = r;
+                }
+            });
     }
     __syncthreads();
     // z lines: ny contiguous lines; y lines: nz lines interleaved at stride nz
-    cplx<T>* r1 = fft_lines<T, false>(b0, b1, tz, az, ny, nz, 1, -1);
-    cplx<T>* r2 = fft_lines<T, true>(r1, r1 == b0 ? b1 : b0, ty, ay, nz, 1, nz, -1);
+    cplx<T>* r1 = spec_lines<T, false, SPEC_PLANE_THREADS, P, P>(b0, b1, tz, az, ny, nz, 1, -1);
+    cplx<T>* r2 = spec_lines<T, true, SPEC_PLANE_THREADS, P, P>(r1, r1 == b0 ? b1 : b0, ty, ay, nz, 1, nz, -1);
     cplx<T>* out = spec + (size_t)q * ns + base;
-    for (int c = threadIdx.x; c < np; c += blockDim.x) out[c] = r2[c];
+    for (int c = threadIdx.x; c < np; c += SPEC_PLANE_THREADS) out[c] = r2[c];
 }
+
+// Wave vector, 1/|k| and 1/k^2 of bin (x, pencil p = y nz + z) from the
+// spectral solver's tables.
+template <typename T>
+struct KPoint {
+    T k[3] = {T(0), T(0), T(0)};
+    T ki = T(0), ik2 = T(0);
+    __device__ static KPoint load(const T* kx, const T* ky, const T* kz, const T* kinv, const T* ik2t, int x, int p,
+                                  int np, int nz) {
+        KPoint r;
+        const size_t idx = (size_t)x * np + p;
+        r.k[0] = kx[x];
+        r.k[1] = ky[p / nz];
+        r.k[2] = kz[p % nz];
+        r.ki = kinv[idx];
+        r.ik2 = ik2t[idx];
+        return r;
+    }
+};
 
 // LDS of the pencil kernel: x twiddles and two buffers of nl lines of
 // nx + 1 points (padded against bank conflicts).
@@ -83,14 +134,14 @@ inline size_t spec_pencil_lds(int nx, int nl, size_t esz) {
 // Pencils p0 .. p0+pb-1 (p = y nz + z) of the nsig stress components: forward
 // x transform, flow solve, inverse x transform of the nout = 6 (u, W) or 12
 // (u, W, E) outputs.
-template <typename T>
+template <typename T, int P>
 __global__ void __launch_bounds__(SPEC_PENCIL_THREADS)
     k_spec_pencils(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int np, int nz, int pb_max,
                    int nsig, int nout, SpecAxis ax, const cplx<T>* __restrict__ twx, const T* __restrict__ kx,
                    const T* __restrict__ ky, const T* __restrict__ kz, const T* __restrict__ ik2,
                    const T* __restrict__ kinv, T screening, T norm) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-    const int nx = ax.n, ls = nx + 1;
+    const int nx = P ? P : ax.n, ls = nx + 1;
     const int nl = (nsig > nout ? nsig : nout) * pb_max;
     cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* A = tw + nx;
@@ -103,28 +154,32 @@ __global__ void __launch_bounds__(SPEC_PENCIL_THREADS)
     const int g = groups % 8 == 0 ? (blockIdx.x % 8) * (groups / 8) + blockIdx.x / 8 : blockIdx.x;
     const int p0 = g * pb_max;
     const int pb = min(pb_max, np - p0);
-    for (int t = threadIdx.x; t < nx; t += blockDim.x) tw[t] = twx[t];
+    for (int t = threadIdx.x; t < nx; t += SPEC_PENCIL_THREADS) tw[t] = twx[t];
     // line (c, p) at A[(c pb_max + p) ls]; lanes over p for coalesced loads
-    for (int f = threadIdx.x; f < nsig * nx * pb_max; f += blockDim.x) {
-        const int p = f % pb_max, xx = (f / pb_max) % nx, c = f / (pb_max * nx);
-        A[(size_t)(c * pb_max + p) * ls + xx] =
-            p < pb ? in[(size_t)c * ns + (size_t)xx * np + p0 + p] : cplx<T>{T(0), T(0)};
-    }
+    batched_loop<SPEC_PENCIL_THREADS>(
+        nsig * nx * pb_max,
+        [&](int f) {
+            const int p = f % pb_max, xx = (f / pb_max) % nx, c = f / (pb_max * nx);
+            return p < pb ? in[(size_t)c * ns + (size_t)xx * np + p0 + p] : cplx<T>{T(0), T(0)};
+        },
+        [&](int f, cplx<T> v) {
+            const int p = f % pb_max, xx = (f / pb_max) % nx, c = f / (pb_max * nx);
+            A[(size_t)(c * pb_max + p) * ls + xx] = v;
+        });
     __syncthreads();
-    cplx<T>* S = fft_lines<T, false>(A, B, tw, ax, nsig * pb_max, ls, 1, -1);
+    cplx<T>* S = spec_lines<T, false, SPEC_PENCIL_THREADS, P, 0>(A, B, tw, ax, nsig * pb_max, ls, 1, -1);
     cplx<T>* O = S == A ? B : A;
-    for (int f = threadIdx.x; f < nx * pb_max; f += blockDim.x) {
+    for (int f = threadIdx.x; f < nx * pb_max; f += SPEC_PENCIL_THREADS) {
         const int p = f % pb_max, xx = f / pb_max;
         if (p >= pb) {
             for (int o = 0; o < nout; ++o) O[(size_t)(o * pb_max + p) * ls + xx] = cplx<T>{T(0), T(0)};
             continue;
         }
-        const int pp = p0 + p;
-        const size_t idx = (size_t)xx * np + pp;
-        const T k[3] = {kx[xx], ky[pp / nz], kz[pp % nz]};
+        const KPoint<T> kp = KPoint<T>::load(kx, ky, kz, kinv, ik2, xx, p0 + p, np, nz);
+        const T* k = kp.k;
         T ur[3], ui[3];
-        flow_point<T>([&](int q) { return S[(size_t)(q * pb_max + p) * ls + xx]; }, nsig, k, kinv[idx], ik2[idx],
-                      screening, norm, ur, ui);
+        flow_point<T>([&](int q) { return S[(size_t)(q * pb_max + p) * ls + xx]; }, nsig, k, kp.ki, kp.ik2, screening,
+                      norm, ur, ui);
 #pragma unroll
         for (int i = 0; i < 3; ++i) O[(size_t)(i * pb_max + p) * ls + xx] = cplx<T>{ur[i], ui[i]};
 #pragma unroll
@@ -135,8 +190,8 @@ __global__ void __launch_bounds__(SPEC_PENCIL_THREADS)
         }
     }
     __syncthreads();
-    cplx<T>* R = fft_lines<T, false>(O, S, tw, ax, nout * pb_max, ls, 1, +1);
-    for (int f = threadIdx.x; f < nout * nx * pb_max; f += blockDim.x) {
+    cplx<T>* R = spec_lines<T, false, SPEC_PENCIL_THREADS, P, 0>(O, S, tw, ax, nout * pb_max, ls, 1, +1);
+    for (int f = threadIdx.x; f < nout * nx * pb_max; f += SPEC_PENCIL_THREADS) {
         const int p = f % pb_max, xx = (f / pb_max) % nx, o = f / (pb_max * nx);
         if (p < pb) out[(size_t)o * ns + (size_t)xx * np + p0 + p] = R[(size_t)(o * pb_max + p) * ls + xx];
     }
@@ -144,28 +199,28 @@ __global__ void __launch_bounds__(SPEC_PENCIL_THREADS)
 
 // Inverse: field component o of plane x; the real part goes to record slot
 // o (u, W) or REC_E + o - 6 (E).
-template <typename T>
+template <typename T, int P>
 __global__ void __launch_bounds__(SPEC_PLANE_THREADS)
     k_spec_planes_inv(const cplx<T>* __restrict__ spec, int nx, SpecAxis ay, SpecAxis az,
                       const cplx<T>* __restrict__ twy, const cplx<T>* __restrict__ twz, T* __restrict__ rec,
                       int rs) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-    const int ny = ay.n, nz = az.n, np = ny * nz;
+    const int ny = P ? P : ay.n, nz = P ? P : az.n, np = ny * nz;
     cplx<T>* ty = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* tz = ty + ny;
     cplx<T>* b0 = tz + nz;
     cplx<T>* b1 = b0 + np;
     const int x = blockIdx.x % nx, o = blockIdx.x / nx;
     const size_t ns = (size_t)nx * np, base = (size_t)x * np;
-    for (int t = threadIdx.x; t < ny; t += blockDim.x) ty[t] = twy[t];
-    for (int t = threadIdx.x; t < nz; t += blockDim.x) tz[t] = twz[t];
+    for (int t = threadIdx.x; t < ny; t += SPEC_PLANE_THREADS) ty[t] = twy[t];
+    for (int t = threadIdx.x; t < nz; t += SPEC_PLANE_THREADS) tz[t] = twz[t];
     const cplx<T>* src = spec + (size_t)o * ns + base;
-    for (int c = threadIdx.x; c < np; c += blockDim.x) b0[c] = src[c];
+    batched_loop<SPEC_PLANE_THREADS>(np, [&](int c) { return src[c]; }, [&](int c, cplx<T> v) { b0[c] = v; });
     __syncthreads();
-    cplx<T>* r1 = fft_lines<T, true>(b0, b1, ty, ay, nz, 1, nz, +1);
-    cplx<T>* r2 = fft_lines<T, false>(r1, r1 == b0 ? b1 : b0, tz, az, ny, nz, 1, +1);
+    cplx<T>* r1 = spec_lines<T, true, SPEC_PLANE_THREADS, P, P>(b0, b1, ty, ay, nz, 1, nz, +1);
+    cplx<T>* r2 = spec_lines<T, false, SPEC_PLANE_THREADS, P, P>(r1, r1 == b0 ? b1 : b0, tz, az, ny, nz, 1, +1);
     const int slot = o < 6 ? o : REC_E + o - 6;
-    for (int c = threadIdx.x; c < np; c += blockDim.x) rec[(base + c) * rs + slot] = r2[c].x;
+    for (int c = threadIdx.x; c < np; c += SPEC_PLANE_THREADS) rec[(base + c) * rs + slot] = r2[c].x;
 }
 
 // ---------------------------------------------------------------------------
@@ -186,14 +241,14 @@ __global__ void __launch_bounds__(SPEC_PLANE_THREADS)
 
 // Forward: packed component c = moments 2c + i 2c+1 of plane x (2c+1 may be
 // absent); blocks with c == 0 also convert the count component.
-template <typename T, int NM>
+template <typename T, int NM, int P>
 __global__ void __launch_bounds__(SPEC_PLANE_THREADS)
     k_spec_planes_fwd2(const long long* __restrict__ main, long long* __restrict__ inflow, int nx, double scale,
                        double rho_scale, SpecAxis ay, SpecAxis az, const cplx<T>* __restrict__ twy,
                        const cplx<T>* __restrict__ twz, cplx<T>* __restrict__ spec, T* __restrict__ rho,
                        T* __restrict__ rec, int rs) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-    const int ny = ay.n, nz = az.n, np = ny * nz;
+    const int ny = P ? P : ay.n, nz = P ? P : az.n, np = ny * nz;
     cplx<T>* ty = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* tz = ty + ny;
     cplx<T>* b0 = tz + nz;
@@ -201,41 +256,46 @@ __global__ void __launch_bounds__(SPEC_PLANE_THREADS)
     const int x = blockIdx.x % nx, c = blockIdx.x / nx;
     const int q0 = 2 * c, q1 = 2 * c + 1;
     const size_t ns = (size_t)nx * np, base = (size_t)x * np;
-    for (int t = threadIdx.x; t < ny; t += blockDim.x) ty[t] = twy[t];
-    for (int t = threadIdx.x; t < nz; t += blockDim.x) tz[t] = twz[t];
+    for (int t = threadIdx.x; t < ny; t += SPEC_PLANE_THREADS) ty[t] = twy[t];
+    for (int t = threadIdx.x; t < nz; t += SPEC_PLANE_THREADS) tz[t] = twz[t];
     const long long* m0 = main + (size_t)q0 * ns + base;
     long long* i0 = inflow + (size_t)q0 * ns + base;
     const long long* m1 = main + (size_t)q1 * ns + base;
     long long* i1 = inflow + (size_t)q1 * ns + base;
-    const bool two = q1 < NM;
-    for (int t = threadIdx.x; t < np; t += blockDim.x) {
-        const long long v0 = m0[t] + i0[t];
-        i0[t] = 0;
-        long long v1 = 0;
-        if (two) {
-            v1 = m1[t] + i1[t];
-            i1[t] = 0;
-        }
-        b0[t] = cplx<T>{(T)((double)v0 * scale), (T)((double)v1 * scale)};
-    }
+    if (q1 < NM)
+        batched_loop<SPEC_PLANE_THREADS>(
+            np, [&](int t) { return longlong2{m0[t] + i0[t], m1[t] + i1[t]}; },
+            [&](int t, longlong2 v) {
+                i0[t] = 0;
+                i1[t] = 0;
+                b0[t] = cplx<T>{(T)((double)v.x * scale), (T)((double)v.y * scale)};
+            });
+    else
+        batched_loop<SPEC_PLANE_THREADS>(
+            np, [&](int t) { return m0[t] + i0[t]; },
+            [&](int t, long long v) {
+                i0[t] = 0;
+                b0[t] = cplx<T>{(T)((double)v * scale), T(0)};
+            });
     if (c == 0) {
         const long long* mc = main + (size_t)NM * ns + base;
         long long* ic = inflow + (size_t)NM * ns + base;
-        for (int t = threadIdx.x; t < np; t += blockDim.x) {
-            const long long v = mc[t] + ic[t];
-            ic[t] = 0;
-            if (rho) {
-                const T r = (T)((double)v * rho_scale);
-                rho[base + t] = r;
-                rec[(base + t) * rs + REC_RHO] = r;
-            }
-        }
+        batched_loop<SPEC_PLANE_THREADS>(
+            np, [&](int t) { return mc[t] + ic[t]; },
+            [&](int t, long long v) {
+                ic[t] = 0;
+                if (rho) {
+                    const T r = (T)((double)v * rho_scale);
+                    rho[base + t] = r;
+                    rec[(base + t) * rs + REC_RHO] = r;
+                }
+            });
     }
     __syncthreads();
-    cplx<T>* r1 = fft_lines<T, false>(b0, b1, tz, az, ny, nz, 1, -1);
-    cplx<T>* r2 = fft_lines<T, true>(r1, r1 == b0 ? b1 : b0, ty, ay, nz, 1, nz, -1);
+    cplx<T>* r1 = spec_lines<T, false, SPEC_PLANE_THREADS, P, P>(b0, b1, tz, az, ny, nz, 1, -1);
+    cplx<T>* r2 = spec_lines<T, true, SPEC_PLANE_THREADS, P, P>(r1, r1 == b0 ? b1 : b0, ty, ay, nz, 1, nz, -1);
     cplx<T>* out = spec + (size_t)c * ns + base;
-    for (int t = threadIdx.x; t < np; t += blockDim.x) out[t] = r2[t];
+    for (int t = threadIdx.x; t < np; t += SPEC_PLANE_THREADS) out[t] = r2[t];
 }
 
 // (a - conj b) / 2i
@@ -248,7 +308,7 @@ __device__ inline cplx<T> half_diff_over_i(cplx<T> a, cplx<T> b) {
 // pencils that are their own mirror).  npk packed stress components in, nout
 // outputs (u, W [, E]) out as nout/2 packed fields.  Line (c, s, w) of pair s,
 // pencil w (0: p, 1: m) sits at A[((c pb + s) 2 + w) (nx + 1)].
-template <typename T>
+template <typename T, int P>
 __global__ void __launch_bounds__(SPEC_PENCIL_THREADS)
     k_spec_pencils2(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, const uint2* __restrict__ pairs,
                     int npairs, int np, int nz, int ny, int pb_max, int nsig, int nout, SpecAxis ax,
@@ -256,7 +316,7 @@ __global__ void __launch_bounds__(SPEC_PENCIL_THREADS)
                     const T* __restrict__ kz, const T* __restrict__ ik2, const T* __restrict__ kinv, T screening,
                     T norm) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-    const int nx = ax.n, ls = nx + 1;
+    const int nx = P ? P : ax.n, ls = nx + 1;
     const int npk = (nsig + 1) / 2, nop = nout / 2;
     const int nl = 2 * (npk > nop ? npk : nop) * pb_max;
     cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
@@ -268,23 +328,29 @@ __global__ void __launch_bounds__(SPEC_PENCIL_THREADS)
     const int g = groups % 8 == 0 ? (blockIdx.x % 8) * (groups / 8) + blockIdx.x / 8 : blockIdx.x;
     const int s0 = g * pb_max;
     const int pb = min(pb_max, npairs - s0);
-    for (int t = threadIdx.x; t < nx; t += blockDim.x) tw[t] = twx[t];
-    // lanes over (w, s) for loads of neighbouring pencils
-    for (int f = threadIdx.x; f < npk * nx * 2 * pb_max; f += blockDim.x) {
-        const int ws = f % (2 * pb_max), xx = (f / (2 * pb_max)) % nx, c = f / (2 * pb_max * nx);
-        const int s = ws >> 1, w = ws & 1;
-        cplx<T> v = cplx<T>{T(0), T(0)};
-        if (s < pb) {
-            const uint2 pr = pairs[s0 + s];
-            v = in[(size_t)c * ns + (size_t)xx * np + (w ? pr.y : pr.x)];
-        }
-        A[(size_t)((c * pb_max + s) * 2 + w) * ls + xx] = v;
-    }
+    __shared__ uint2 prs[SPEC_MAX_PENCILS];
+    for (int t = threadIdx.x; t < nx; t += SPEC_PENCIL_THREADS) tw[t] = twx[t];
+    if (threadIdx.x < pb) prs[threadIdx.x] = pairs[s0 + threadIdx.x];
     __syncthreads();
-    cplx<T>* S = fft_lines<T, false>(A, B, tw, ax, npk * pb_max * 2, ls, 1, -1);
+    // lanes over (w, s) for loads of neighbouring pencils
+    batched_loop<SPEC_PENCIL_THREADS>(
+        npk * nx * 2 * pb_max,
+        [&](int f) {
+            const int ws = f % (2 * pb_max), xx = (f / (2 * pb_max)) % nx, c = f / (2 * pb_max * nx);
+            const int s = ws >> 1, w = ws & 1;
+            cplx<T> v = cplx<T>{T(0), T(0)};
+            if (s < pb) v = in[(size_t)c * ns + (size_t)xx * np + (w ? prs[s].y : prs[s].x)];
+            return v;
+        },
+        [&](int f, cplx<T> v) {
+            const int ws = f % (2 * pb_max), xx = (f / (2 * pb_max)) % nx, c = f / (2 * pb_max * nx);
+            A[(size_t)((c * pb_max + (ws >> 1)) * 2 + (ws & 1)) * ls + xx] = v;
+        });
+    __syncthreads();
+    cplx<T>* S = spec_lines<T, false, SPEC_PENCIL_THREADS, P, 0>(A, B, tw, ax, npk * pb_max * 2, ls, 1, -1);
     cplx<T>* O = S == A ? B : A;
     // one thread per mirror pair of bins: (p, i) and (m, -i)
-    for (int f = threadIdx.x; f < nx * pb_max; f += blockDim.x) {
+    for (int f = threadIdx.x; f < nx * pb_max; f += SPEC_PENCIL_THREADS) {
         const int s = f % pb_max, i = f / pb_max;
         const int im = i == 0 ? 0 : nx - i;
         if (s >= pb) {
@@ -294,7 +360,7 @@ __global__ void __launch_bounds__(SPEC_PENCIL_THREADS)
             }
             continue;
         }
-        const uint2 pr = pairs[s0 + s];
+        const uint2 pr = prs[s];
         const int p = (int)pr.x, m = (int)pr.y;
         if (p == m && i > im) continue;  // its own mirror: the pair (i, -i) once
         // unpacked stress spectra at (p, i); at (m, -i) they are the conjugates
@@ -305,13 +371,14 @@ __global__ void __launch_bounds__(SPEC_PENCIL_THREADS)
             sg[2 * c] = cplx<T>{(zp.x + zm.x) * T(0.5), (zp.y - zm.y) * T(0.5)};
             if (2 * c + 1 < nsig) sg[2 * c + 1] = half_diff_over_i(zp, zm);
         }
-        const size_t cp = (size_t)i * np + p, cm = (size_t)im * np + m;
-        const T k1[3] = {kx[i], ky[p / nz], kz[p % nz]};
-        const T k2[3] = {kx[im], ky[m / nz], kz[m % nz]};
+        const KPoint<T> q1 = KPoint<T>::load(kx, ky, kz, kinv, ik2, i, p, np, nz);
+        const KPoint<T> q2 = KPoint<T>::load(kx, ky, kz, kinv, ik2, im, m, np, nz);
+        const T* k1 = q1.k;
+        const T* k2 = q2.k;
         T u1r[3], u1i[3], u2r[3], u2i[3];
-        flow_point<T>([&](int q) { return sg[q]; }, nsig, k1, kinv[cp], ik2[cp], screening, norm, u1r, u1i);
-        flow_point<T>([&](int q) { return cplx<T>{sg[q].x, -sg[q].y}; }, nsig, k2, kinv[cm], ik2[cm], screening,
-                      norm, u2r, u2i);
+        flow_point<T>([&](int q) { return sg[q]; }, nsig, k1, q1.ki, q1.ik2, screening, norm, u1r, u1i);
+        flow_point<T>([&](int q) { return cplx<T>{sg[q].x, -sg[q].y}; }, nsig, k2, q2.ki, q2.ik2, screening, norm,
+                      u2r, u2i);
         // outputs at k and at -k, then their Hermitian part
         cplx<T> h[12];
         for (int o = 0; o < nout; ++o) {
@@ -340,12 +407,12 @@ __global__ void __launch_bounds__(SPEC_PENCIL_THREADS)
         }
     }
     __syncthreads();
-    cplx<T>* R = fft_lines<T, false>(O, S, tw, ax, nop * pb_max * 2, ls, 1, +1);
-    for (int f = threadIdx.x; f < nop * nx * 2 * pb_max; f += blockDim.x) {
+    cplx<T>* R = spec_lines<T, false, SPEC_PENCIL_THREADS, P, 0>(O, S, tw, ax, nop * pb_max * 2, ls, 1, +1);
+    for (int f = threadIdx.x; f < nop * nx * 2 * pb_max; f += SPEC_PENCIL_THREADS) {
         const int ws = f % (2 * pb_max), xx = (f / (2 * pb_max)) % nx, d = f / (2 * pb_max * nx);
         const int s = ws >> 1, w = ws & 1;
         if (s >= pb) continue;
-        const uint2 pr = pairs[s0 + s];
+        const uint2 pr = prs[s];
         if (w == 1 && pr.x == pr.y) continue;
         out[(size_t)d * ns + (size_t)xx * np + (w ? pr.y : pr.x)] = R[(size_t)((d * pb_max + s) * 2 + w) * ls + xx];
     }
@@ -353,29 +420,29 @@ __global__ void __launch_bounds__(SPEC_PENCIL_THREADS)
 
 // Inverse: packed field d (outputs 2d + i 2d+1) of plane x; the real part is
 // output 2d, the imaginary part output 2d+1.
-template <typename T>
+template <typename T, int P>
 __global__ void __launch_bounds__(SPEC_PLANE_THREADS)
     k_spec_planes_inv2(const cplx<T>* __restrict__ spec, int nx, SpecAxis ay, SpecAxis az,
                        const cplx<T>* __restrict__ twy, const cplx<T>* __restrict__ twz, T* __restrict__ rec,
                        int rs) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-    const int ny = ay.n, nz = az.n, np = ny * nz;
+    const int ny = P ? P : ay.n, nz = P ? P : az.n, np = ny * nz;
     cplx<T>* ty = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* tz = ty + ny;
     cplx<T>* b0 = tz + nz;
     cplx<T>* b1 = b0 + np;
     const int x = blockIdx.x % nx, d = blockIdx.x / nx;
     const size_t ns = (size_t)nx * np, base = (size_t)x * np;
-    for (int t = threadIdx.x; t < ny; t += blockDim.x) ty[t] = twy[t];
-    for (int t = threadIdx.x; t < nz; t += blockDim.x) tz[t] = twz[t];
+    for (int t = threadIdx.x; t < ny; t += SPEC_PLANE_THREADS) ty[t] = twy[t];
+    for (int t = threadIdx.x; t < nz; t += SPEC_PLANE_THREADS) tz[t] = twz[t];
     const cplx<T>* src = spec + (size_t)d * ns + base;
-    for (int t = threadIdx.x; t < np; t += blockDim.x) b0[t] = src[t];
+    batched_loop<SPEC_PLANE_THREADS>(np, [&](int t) { return src[t]; }, [&](int t, cplx<T> v) { b0[t] = v; });
     __syncthreads();
-    cplx<T>* r1 = fft_lines<T, true>(b0, b1, ty, ay, nz, 1, nz, +1);
-    cplx<T>* r2 = fft_lines<T, false>(r1, r1 == b0 ? b1 : b0, tz, az, ny, nz, 1, +1);
+    cplx<T>* r1 = spec_lines<T, true, SPEC_PLANE_THREADS, P, P>(b0, b1, ty, ay, nz, 1, nz, +1);
+    cplx<T>* r2 = spec_lines<T, false, SPEC_PLANE_THREADS, P, P>(r1, r1 == b0 ? b1 : b0, tz, az, ny, nz, 1, +1);
     const int o0 = 2 * d, o1 = 2 * d + 1;
     const int sl0 = o0 < 6 ? o0 : REC_E + o0 - 6, sl1 = o1 < 6 ? o1 : REC_E + o1 - 6;
-    for (int t = threadIdx.x; t < np; t += blockDim.x) {
+    for (int t = threadIdx.x; t < np; t += SPEC_PLANE_THREADS) {
         rec[(base + t) * rs + sl0] = r2[t].x;
         rec[(base + t) * rs + sl1] = r2[t].y;
     }

@@ -22,7 +22,7 @@ FLAG_MAGNETIC_ALWAYS = 2
 FLAG_KEEP_DISTRIBUTION = 4
 FLAG_NO_GRAPH = 8
 FLAG_AXIS_PASSES = 16
-FLAG_UNPAIRED_SPECTRAL = 32
+FLAG_PAIRED_SPECTRAL = 32
 
 SS_OK = 0
 ERRORS = {

@@ -347,6 +347,8 @@ def test_graph_replays_match_eager_steps(precision, binning):
     "grid, extra",
     [
         ((16, 16, 16, 8, 8), {}),
+        ((32, 32, 32, 4, 4), {}),
+        ((64, 64, 64, 2, 2), dict(shape=0.5)),
         ((12, 20, 15, 6, 4), dict(shape=0.3, volume_exclusion=0.2)),
         ((16, 8, 10, 8, 8), dict(stress_magnetic=0.7)),
         ((12, 10, 1, 6, 4), dict(flags=ss.FLAG_QUASI2D)),
@@ -355,14 +357,15 @@ def test_graph_replays_match_eager_steps(precision, binning):
 def test_fused_spectral_stage_equals_axis_passes(precision, grid, extra):
     """The fused plane/pencil kernels of the spectral stage run the same radix
     stages, axis order and solve as convert + one pass per FFT axis + solve +
-    pack, so particles and fields agree bit for bit."""
+    pack, so particles and fields agree bit for bit -- also where all axes are
+    16, 32 or 64 long and the stages have their sizes fixed at compile time."""
     extra = dict(extra)
     flags = extra.pop("flags", 0)
     box = tuple(0.8 * g for g in grid[:3])
     kw = dict(grid=grid, box=box, n=6000, precision=precision, binning=ss.BIN_SORTED, resort_interval=3, **extra)
     parts = ss.initial_condition(small_params(**kw), amplitude=0.1, seed=12)
     sims = [ss.Simulation(small_params(flags=flags | f, **kw))
-            for f in (ss.FLAG_UNPAIRED_SPECTRAL, ss.FLAG_AXIS_PASSES)]
+            for f in (0, ss.FLAG_AXIS_PASSES)]
     assert sims[0].get_info()["fused_spectral"] and not sims[1].get_info()["fused_spectral"]
     assert not sims[0].get_info()["paired_spectral"]
     out = []
@@ -381,6 +384,7 @@ def test_fused_spectral_stage_equals_axis_passes(precision, grid, extra):
 @pytest.mark.parametrize("precision,tol", [(ss.FP64, 1e-13), (ss.FP32, 2e-6)])
 @pytest.mark.parametrize("grid,extra", [
     ((16, 16, 16, 8, 8), {}),
+    ((32, 32, 32, 4, 4), {"shape": 0.6}),
     ((12, 10, 15, 6, 6), {"shape": 0.6, "volume_exclusion": 0.3}),
     ((15, 9, 5, 4, 6), {"shape": -0.4}),
     ((8, 12, 16, 6, 4), {"hydro_screening": 0.5}),
@@ -394,9 +398,9 @@ def test_paired_spectral_stage_matches_unpaired(precision, tol, grid, extra):
     kw = dict(grid=grid, box=box, n=6000, precision=precision, binning=ss.BIN_SORTED, resort_interval=3,
               flags=ss.FLAG_KEEP_DISTRIBUTION, **extra)
     parts = ss.initial_condition(small_params(**kw), amplitude=0.1, seed=13)
-    a = ss.Simulation(small_params(**kw))
-    kw["flags"] |= ss.FLAG_UNPAIRED_SPECTRAL
     b = ss.Simulation(small_params(**kw))
+    kw["flags"] |= ss.FLAG_PAIRED_SPECTRAL
+    a = ss.Simulation(small_params(**kw))
     assert a.get_info()["paired_spectral"] and not b.get_info()["paired_spectral"]
     for sim in (a, b):
         sim.init(parts)
