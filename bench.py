@@ -246,9 +246,15 @@ def main():
     if not args.no_fp64:
         print("[bench] FP64 plain path", file=sys.stderr, flush=True)
         ms64, st64 = time_gpu(ss.FP64, parts, max(1, args.steps // 4), min(args.warmup, 5),
-                              min(args.profile_steps, 10))
+                              min(args.profile_steps, 10), binning=ss.BIN_HISTOGRAM)
         fp64 = {"value": n / (ms64 * 1e-3), "ms_per_step": ms64, "steps": max(1, args.steps // 4),
                 "dtype": "f64", "binning": "histogram", "stages_ms": st64}
+        # the same FP64 step on the tiled path (binning fused into the update)
+        print("[bench] FP64 tiled path", file=sys.stderr, flush=True)
+        ms64t, st64t = time_gpu(ss.FP64, parts, max(1, args.steps // 4), min(args.warmup, 5),
+                                min(args.profile_steps, 10), binning=ss.BIN_SORTED)
+        fp64["tiled"] = {"value": n / (ms64t * 1e-3), "ms_per_step": ms64t, "binning": "sorted",
+                         "stages_ms": st64t}
 
     fftv = None
     if not args.no_fft_study:
@@ -281,7 +287,7 @@ def main():
             traffic = json.load(f).get("traffic_bytes_per_launch")
     achieved = UPDATE_BYTES["f32"] * n / (upd_ms * 1e-3) / 1e9
     roofline = {
-        "kernel": "ss::k_tile_update<float, 6, true, false, false, false>",
+        "kernel": "ss::k_tile_update<float, 6, true, 0, false, false>",
         "bound": "hbm",
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
