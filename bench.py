@@ -26,6 +26,14 @@ HBM_PEAK_GBS = 8000.0
 # Philox: read x,y,z,phi,theta + 3 compensation words, write the same) and of
 # the FP64 plain update (read 40 + write 40; noise is generated in-kernel).
 UPDATE_BYTES = {"f32": 64.0, "f64": 80.0}
+# The plain path's binning kernel reads the five FP64 coordinates (40 B); on
+# the tiled path the binning of the new state is fused into the update.
+BIN_BYTES = {"f64": 40.0}
+
+
+def hbm_frac(nbytes, n, ms):
+    """Fraction of the HBM peak that nbytes per particle at n particles in ms is."""
+    return nbytes * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS if ms and ms > 0 else None
 
 
 def make(prec, n, binning=None, **extra):
@@ -248,13 +256,18 @@ def main():
         ms64, st64 = time_gpu(ss.FP64, parts, max(1, args.steps // 4), min(args.warmup, 5),
                               min(args.profile_steps, 10), binning=ss.BIN_HISTOGRAM)
         fp64 = {"value": n / (ms64 * 1e-3), "ms_per_step": ms64, "steps": max(1, args.steps // 4),
-                "dtype": "f64", "binning": "histogram", "stages_ms": st64}
+                "dtype": "f64", "binning": "histogram", "stages_ms": st64,
+                # the bin stage: clear, histogram and contraction kernels
+                "hbm_frac": {"bin_stage": hbm_frac(BIN_BYTES["f64"], n, st64.get("bin")),
+                             "update": hbm_frac(UPDATE_BYTES["f64"], n, st64.get("update")),
+                             "bytes_per_particle": {"bin": BIN_BYTES["f64"], "update": UPDATE_BYTES["f64"]}}}
         # the same FP64 step on the tiled path (binning fused into the update)
         print("[bench] FP64 tiled path", file=sys.stderr, flush=True)
         ms64t, st64t = time_gpu(ss.FP64, parts, max(1, args.steps // 4), min(args.warmup, 5),
                                 min(args.profile_steps, 10), binning=ss.BIN_SORTED)
         fp64["tiled"] = {"value": n / (ms64t * 1e-3), "ms_per_step": ms64t, "binning": "sorted",
-                         "stages_ms": st64t}
+                         "stages_ms": st64t,
+                         "hbm_frac": {"update_with_binning": hbm_frac(UPDATE_BYTES["f64"], n, st64t.get("update"))}}
 
     fftv = None
     if not args.no_fft_study:
