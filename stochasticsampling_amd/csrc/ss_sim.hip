@@ -406,7 +406,7 @@ static hipError_t launch_spec_planes_fwd(ss_sim* s) {
     double ms, ds;
     scales(s, ms, ds);
     const size_t lds = spec_plane_lds(s->ny, s->nz, sizeof(T));
-    const dim3 grid((unsigned)(s->nx * s->nm));
+    const dim3 grid((unsigned)(s->nx * (s->nm + 1)));  // + the count component
     T* rho = s->use_rho ? (T*)s->rho : nullptr;
     const SpecAxis ay = spec_axis(s->fft.ax[1]), az = spec_axis(s->fft.ax[2]);
     const cplx<T>* twy = (const cplx<T>*)s->fft.ax[1].tw;
@@ -448,7 +448,7 @@ static hipError_t launch_spec_planes_fwd2(ss_sim* s) {
     scales(s, ms, ds);
     const size_t lds = spec_plane_lds(s->ny, s->nz, sizeof(T));
     const int npk = (s->nm + 1) / 2;
-    const dim3 grid((unsigned)(s->nx * npk));
+    const dim3 grid((unsigned)(s->nx * (npk + 1)));  // + the count component
     T* rho = s->use_rho ? (T*)s->rho : nullptr;
     const SpecAxis ay = spec_axis(s->fft.ax[1]), az = spec_axis(s->fft.ax[2]);
     const cplx<T>* twy = (const cplx<T>*)s->fft.ax[1].tw;

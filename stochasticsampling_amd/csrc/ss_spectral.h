@@ -55,11 +55,30 @@ __device__ inline cplx<T>* spec_lines(cplx<T>* a, cplx<T>* b, const cplx<T>* tw,
         return fft_lines<T, LINES_FAST, NT>(a, b, tw, ax, nlines, ls, es, sign);
 }
 
+// The count component of one plane: main + inflow, inflow cleared, the
+// density into rho and the cell records when wanted.
+template <typename T, int NT>
+__device__ inline void spec_count_plane(const long long* __restrict__ mc, long long* __restrict__ ic, int np,
+                                        double rho_scale, T* __restrict__ rho, T* __restrict__ rec, size_t base,
+                                        int rs) {
+    batched_loop<NT>(
+        np, [&](int c) { return mc[c] + ic[c]; },
+        [&](int c, long long v) {
+            ic[c] = 0;
+            if (rho) {
+                const T r = (T)((double)v * rho_scale);
+                rho[base + c] = r;
+                rec[(base + c) * rs + REC_RHO] = r;
+            }
+        });
+}
+
 // LDS of one plane kernel: twiddles of y and z, two plane buffers.
 inline size_t spec_plane_lds(int ny, int nz, size_t esz) { return esz * 2 * ((size_t)ny + nz + 2 * (size_t)ny * nz); }
 
-// Forward: component q of plane x.  Blocks with q == 0 also convert the count
-// component (density, when wanted).  Clears the inflow entries it reads.
+// Forward: component q < NM of plane x; the blocks q == NM convert the count
+// component (density, when wanted) and transform nothing, so that no block
+// carries both.  Clears the inflow entries it reads.
 template <typename T, int NM, int P>
 __global__ void __launch_bounds__(SPEC_PLANE_THREADS)
     k_spec_planes_fwd(const long long* __restrict__ main, long long* __restrict__ inflow, int nx, double scale,
@@ -74,6 +93,11 @@ __global__ void __launch_bounds__(SPEC_PLANE_THREADS)
     cplx<T>* b1 = b0 + np;
     const int x = blockIdx.x % nx, q = blockIdx.x / nx;
     const size_t ns = (size_t)nx * np, base = (size_t)x * np;
+    if (q == NM) {
+        spec_count_plane<T, SPEC_PLANE_THREADS>(main + (size_t)NM * ns + base, inflow + (size_t)NM * ns + base, np,
+                                                rho_scale, rho, rec, base, rs);
+        return;
+    }
     for (int t = threadIdx.x; t < ny; t += SPEC_PLANE_THREADS) ty[t] = twy[t];
     for (int t = threadIdx.x; t < nz; t += SPEC_PLANE_THREADS) tz[t] = twz[t];
     const long long* mq = main + (size_t)q * ns + base;
@@ -84,20 +108,6 @@ __global__ void __launch_bounds__(SPEC_PLANE_THREADS)
             iq[c] = 0;
             b0[c] = cplx<T>{(T)((double)v * scale), T(0)};
         });
-    if (q == 0) {
-        const long long* mc = main + (size_t)NM * ns + base;
-        long long* ic = inflow + (size_t)NM * ns + base;
-        batched_loop<SPEC_PLANE_THREADS>(
-            np, [&](int c) { return mc[c] + ic[c]; },
-            [&](int c, long long v) {
-                ic[c] = 0;
-                if (rho) {
-                    const T r = (T)((double)v * rho_scale);
-                    rho[base + c] = r;
-                    rec[(base + c) * rs + REC_RHO] = r;
-                }
-            });
-    }
     __syncthreads();
     // z lines: ny contiguous lines; y lines: nz lines interleaved at stride nz
     cplx<T>* r1 = spec_lines<T, false, SPEC_PLANE_THREADS, P, P>(b0, b1, tz, az, ny, nz, 1, -1);
@@ -240,7 +250,7 @@ __global__ void __launch_bounds__(SPEC_PLANE_THREADS)
 // ---------------------------------------------------------------------------
 
 // Forward: packed component c = moments 2c + i 2c+1 of plane x (2c+1 may be
-// absent); blocks with c == 0 also convert the count component.
+// absent); the blocks c == (NM + 1) / 2 convert the count component only.
 template <typename T, int NM, int P>
 __global__ void __launch_bounds__(SPEC_PLANE_THREADS)
     k_spec_planes_fwd2(const long long* __restrict__ main, long long* __restrict__ inflow, int nx, double scale,
@@ -256,6 +266,11 @@ __global__ void __launch_bounds__(SPEC_PLANE_THREADS)
     const int x = blockIdx.x % nx, c = blockIdx.x / nx;
     const int q0 = 2 * c, q1 = 2 * c + 1;
     const size_t ns = (size_t)nx * np, base = (size_t)x * np;
+    if (c == (NM + 1) / 2) {
+        spec_count_plane<T, SPEC_PLANE_THREADS>(main + (size_t)NM * ns + base, inflow + (size_t)NM * ns + base, np,
+                                                rho_scale, rho, rec, base, rs);
+        return;
+    }
     for (int t = threadIdx.x; t < ny; t += SPEC_PLANE_THREADS) ty[t] = twy[t];
     for (int t = threadIdx.x; t < nz; t += SPEC_PLANE_THREADS) tz[t] = twz[t];
     const long long* m0 = main + (size_t)q0 * ns + base;
@@ -277,20 +292,6 @@ __global__ void __launch_bounds__(SPEC_PLANE_THREADS)
                 i0[t] = 0;
                 b0[t] = cplx<T>{(T)((double)v * scale), T(0)};
             });
-    if (c == 0) {
-        const long long* mc = main + (size_t)NM * ns + base;
-        long long* ic = inflow + (size_t)NM * ns + base;
-        batched_loop<SPEC_PLANE_THREADS>(
-            np, [&](int t) { return mc[t] + ic[t]; },
-            [&](int t, long long v) {
-                ic[t] = 0;
-                if (rho) {
-                    const T r = (T)((double)v * rho_scale);
-                    rho[base + t] = r;
-                    rec[(base + t) * rs + REC_RHO] = r;
-                }
-            });
-    }
     __syncthreads();
     cplx<T>* r1 = spec_lines<T, false, SPEC_PLANE_THREADS, P, P>(b0, b1, tz, az, ny, nz, 1, -1);
     cplx<T>* r2 = spec_lines<T, true, SPEC_PLANE_THREADS, P, P>(r1, r1 == b0 ? b1 : b0, ty, ay, nz, 1, nz, -1);
