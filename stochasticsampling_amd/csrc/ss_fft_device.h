@@ -9,8 +9,13 @@
 
 namespace ss {
 
+// Products and sums are contracted into FMAs only within one expression
+// (contract(on), fixed by the source), not at the compiler's choice across
+// statements: the axis-pass and fused kernels instantiate these functions with
+// run-time and compile-time sizes and must round alike.
 template <typename T>
 __device__ inline cplx<T> cmul(cplx<T> a, cplx<T> b) {
+#pragma clang fp contract(on)
     return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
 
@@ -18,6 +23,7 @@ __device__ inline cplx<T> cmul(cplx<T> a, cplx<T> b) {
 // twiddle table of the full length n at index t*(n/R).
 template <typename T, int R>
 __device__ inline void dft_small(cplx<T>* v, const cplx<T>* tw, int n, int sign) {
+#pragma clang fp contract(on)
     if constexpr (R == 2) {
         cplx<T> a = v[0], b = v[1];
         v[0] = {a.x + b.x, a.y + b.y};
@@ -61,6 +67,7 @@ __device__ inline void dft_small(cplx<T>* v, const cplx<T>* tw, int n, int sign)
 template <typename T, int R, bool LINES_FAST, int NT, int NC = 0, int NSC = 0, int NLC = 0>
 __device__ inline void stage(const cplx<T>* src, cplx<T>* dst, const cplx<T>* tw, int n_, int Ns_, int nlines_,
                              int ls, int es, int sign) {
+#pragma clang fp contract(on)
     const int n = NC ? NC : n_;
     const int Ns = NC ? NSC : Ns_;
     const int nlines = NLC ? NLC : nlines_;
@@ -127,30 +134,34 @@ __device__ inline cplx<T>* fft_lines(cplx<T>* a, cplx<T>* b, const cplx<T>* tw, 
     return src;
 }
 
-// fft_lines for a power-of-two length N known at compile time (radix-4
-// stages, then a radix 2 when log2 N is odd: the plan factorize() makes), over
-// NL lines when NL > 0.
+// The radix of the next stage of factorize()'s plan when m points remain:
+// the 4s first, then a 2, then the 3s and the 5s.
+constexpr int next_radix(int m) { return m % 4 == 0 ? 4 : m % 2 == 0 ? 2 : m % 3 == 0 ? 3 : 5; }
+
+// fft_lines for a length N known at compile time (factors 2, 3, 4 and 5, in
+// the order of factorize()), over NL lines when NL > 0.
 template <typename T, bool LINES_FAST, int NT, int N, int NL, int NS = 1>
-__device__ inline cplx<T>* fft_lines_pow2(cplx<T>* src, cplx<T>* dst, const cplx<T>* tw, int nlines, int ls,
-                                          int es, int sign) {
-    static_assert(N > 0 && (N & (N - 1)) == 0, "power of two");
+__device__ inline cplx<T>* fft_lines_ct(cplx<T>* src, cplx<T>* dst, const cplx<T>* tw, int nlines, int ls, int es,
+                                        int sign) {
+    static_assert(N > 1, "length");
     if constexpr (NS >= N) {
         return src;
     } else {
-        constexpr int R = (N / NS) % 4 == 0 ? 4 : 2;
+        constexpr int R = next_radix(N / NS);
+        static_assert((N / NS) % R == 0, "lengths with factors 2, 3 and 5 only");
         stage<T, R, LINES_FAST, NT, N, NS, NL>(src, dst, tw, N, NS, nlines, ls, es, sign);
         __syncthreads();
-        return fft_lines_pow2<T, LINES_FAST, NT, N, NL, NS * R>(dst, src, tw, nlines, ls, es, sign);
+        return fft_lines_ct<T, LINES_FAST, NT, N, NL, NS * R>(dst, src, tw, nlines, ls, es, sign);
     }
 }
 
-// Whether the runtime plan of an axis is the one fft_lines_pow2<N> runs.
-inline bool pow2_plan(const SpecAxis& a, int N) {
-    if (a.n != N || N <= 1 || (N & (N - 1))) return false;
+// Whether the runtime plan of an axis is the one fft_lines_ct<N> runs.
+inline bool ct_plan(const SpecAxis& a, int N) {
+    if (a.n != N || N <= 1) return false;
     int m = N, k = 0;
     while (m > 1) {
-        const int r = m % 4 == 0 ? 4 : 2;
-        if (k >= a.nf || a.fac[k] != r) return false;
+        const int r = next_radix(m);
+        if (m % r != 0 || k >= a.nf || a.fac[k] != r) return false;
         m /= r;
         ++k;
     }

@@ -147,6 +147,9 @@ struct SolveFlags {
 template <typename T, typename Sig>
 __device__ inline void flow_point(Sig sig, int nsig, const T k[3], T ki, T ik2, T screening, T norm, T ur[3],
                                   T ui[3]) {
+    // contractions fixed by the source: the solve kernel and the fused pencil
+    // kernels inline this with different surroundings and must round alike
+#pragma clang fp contract(on)
     const T kn[3] = {k[0] * ki, k[1] * ki, k[2] * ki};
     T sr[3], si[3];
 #pragma unroll
@@ -185,12 +188,14 @@ __device__ inline cplx<T> grad_u(const T k[3], const T ur[3], const T ui[3], int
 // E01, E02, E11, E12, E22 (w = 0..5) (src/bin/simulation/mod.rs:302-308).
 template <typename T>
 __device__ inline cplx<T> vort_part(const T k[3], const T ur[3], const T ui[3], int w) {
+#pragma clang fp contract(on)
     const int a = w == 2 ? 1 : 0, b = w == 0 ? 1 : 2;
     const cplx<T> gab = grad_u(k, ur, ui, a, b), gba = grad_u(k, ur, ui, b, a);
     return cplx<T>{(gab.x - gba.x) * T(0.5), (gab.y - gba.y) * T(0.5)};
 }
 template <typename T>
 __device__ inline cplx<T> strain_part(const T k[3], const T ur[3], const T ui[3], int w) {
+#pragma clang fp contract(on)
     const int a = w < 3 ? 0 : (w < 5 ? 1 : 2);
     const int b = w < 3 ? w : (w < 5 ? w - 2 : 2);
     const cplx<T> gab = grad_u(k, ur, ui, a, b), gba = grad_u(k, ur, ui, b, a);

@@ -158,7 +158,7 @@ struct ss_sim {
     uint2* pairs = nullptr;  // (pencil, mirror pencil) with pencil <= mirror
     int npairs = 0;
     int pencils = 8;  // (y, z) pencils per workgroup of the pencil kernel
-    int spec_p = 0;   // 16, 32 or 64 when all three axes have that length (stages fixed at compile time)
+    int spec_p = 0;   // 16, 32, 48 or 64 when all three axes have that length (stages fixed at compile time)
     uint32_t* home_ctr = nullptr;  // re-sort: particles placed from a segment's front
     bool mag = false;  // magnetic solve runs
     bool use_e = false, use_rho = false, use_b = false, use_gb = false;
@@ -396,6 +396,7 @@ static auto with_spec_p(int p, F f) {
     switch (p) {
         case 16: return f(std::integral_constant<int, 16>{});
         case 32: return f(std::integral_constant<int, 32>{});
+        case 48: return f(std::integral_constant<int, 48>{});
         case 64: return f(std::integral_constant<int, 64>{});
         default: return f(std::integral_constant<int, 0>{});
     }
@@ -955,8 +956,8 @@ static int choose_spectral(ss_sim* s) {
     if (plane > cap || pencil > cap) return SS_OK;
     const SpecAxis a0 = spec_axis(s->fft.ax[0]), a1 = spec_axis(s->fft.ax[1]), a2 = spec_axis(s->fft.ax[2]);
     s->spec_p = 0;
-    for (int P : {16, 32, 64})
-        if (pow2_plan(a0, P) && pow2_plan(a1, P) && pow2_plan(a2, P)) s->spec_p = P;
+    for (int P : {16, 32, 48, 64})
+        if (ct_plan(a0, P) && ct_plan(a1, P) && ct_plan(a2, P)) s->spec_p = P;
     const bool pair = (s->p.flags & SS_FLAG_PAIRED_SPECTRAL) != 0;
     hipError_t ae = with_spec_p(s->spec_p, [&](auto pc) {
         constexpr int P = decltype(pc)::value;
@@ -1886,6 +1887,20 @@ struct ss_ensemble {
     hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};  // [pending][slot]
     uint64_t timestep = 0;
     uint64_t steps_since_sort = 0;
+    // batched spectral stage: members on the fused per-component path, grouped
+    // by compile-time axis length; each group is one launch of each kernel
+    struct SpecGroup {
+        int p = 0;                 // spec_p of the members
+        std::vector<int> members;  // indices into m
+        int blocks[3] = {0, 0, 0}; // planes forward, pencils, planes inverse
+        size_t lds_plane = 0, lds_pencil = 0;
+        size_t job_off = 0;        // byte offset of its SpecJob array in d_specjobs
+        size_t first_off = 0;      // int offset of its three first[] tables in d_specfirst
+    };
+    std::vector<SpecGroup> sgroups;
+    std::vector<char> batched;  // per member: its spectral stage runs in a group
+    void* d_specjobs = nullptr;
+    int* d_specfirst = nullptr;
 };
 
 namespace ss {
@@ -1951,6 +1966,134 @@ static void ens_launch_update(ss_ensemble* e, int slot, bool resort, int k) {
     }
 }
 
+// Groups of the batched spectral stage: every member on the fused
+// per-component path joins the group of its compile-time axis length (moment
+// layouts of 6 or 9 components); the others keep their own launches.
+template <typename T>
+static int ens_build_spec(ss_ensemble* e) {
+    const int n = (int)e->m.size();
+    e->batched.assign(n, 0);
+    e->sgroups.clear();
+    if (e->nm != 6 && e->nm != 9) return SS_OK;
+    for (int j = 0; j < n; ++j) {
+        const ss_sim* s = e->m[j];
+        if (!s->fused || s->paired) continue;
+        ss_ensemble::SpecGroup* g = nullptr;
+        for (auto& q : e->sgroups)
+            if (q.p == s->spec_p) g = &q;
+        if (!g) {
+            e->sgroups.emplace_back();
+            g = &e->sgroups.back();
+            g->p = s->spec_p;
+        }
+        g->members.push_back(j);
+        e->batched[j] = 1;
+    }
+    std::vector<SpecJob<T>> jobs;
+    std::vector<int> first;
+    for (auto& g : e->sgroups) {
+        g.job_off = jobs.size() * sizeof(SpecJob<T>);
+        g.first_off = first.size();
+        const int gm = (int)g.members.size();
+        std::vector<int> f[3];
+        for (int k = 0; k < 3; ++k) f[k].assign(1, 0);
+        for (int j : g.members) {
+            ss_sim* s = e->m[j];
+            double ms, ds;
+            scales(s, ms, ds);
+            SpecJob<T> a;
+            a.main = s->mom_main;
+            a.inflow = s->mom_inflow;
+            a.spec_in = (cplx<T>*)s->spec_in;
+            a.spec_out = (cplx<T>*)s->spec_out;
+            a.rho = s->use_rho ? (T*)s->rho : nullptr;
+            a.rec = (T*)s->rec;
+            a.twx = (const cplx<T>*)s->fft.ax[0].tw;
+            a.twy = (const cplx<T>*)s->fft.ax[1].tw;
+            a.twz = (const cplx<T>*)s->fft.ax[2].tw;
+            a.kx = (const T*)s->kx;
+            a.ky = (const T*)s->ky;
+            a.kz = (const T*)s->kz;
+            a.ik2 = (const T*)s->ik2;
+            a.kinv = (const T*)s->kinv;
+            a.scale = s->fixed_scale * ms;
+            a.rho_scale = ds;
+            a.screening = (T)s->p.hydro_screening;
+            a.norm = (T)(double)s->ns;
+            a.ax = spec_axis(s->fft.ax[0]);
+            a.ay = spec_axis(s->fft.ax[1]);
+            a.az = spec_axis(s->fft.ax[2]);
+            a.nx = s->nx;
+            a.np = s->ny * s->nz;
+            a.nz = s->nz;
+            a.rs = s->rs;
+            a.pb = s->pencils;
+            a.nsig = s->nsig;
+            a.nout = s->use_e ? 12 : 6;
+            jobs.push_back(a);
+            // the block counts of launch_spec_planes_fwd / _pencils / _planes_inv
+            f[0].push_back(f[0].back() + s->nx * (s->nm + 1));
+            f[1].push_back(f[1].back() + (a.np + a.pb - 1) / a.pb);
+            f[2].push_back(f[2].back() + s->nx * a.nout);
+            g.lds_plane = std::max(g.lds_plane, spec_plane_lds(s->ny, s->nz, sizeof(T)));
+            g.lds_pencil = std::max(g.lds_pencil, spec_pencil_lds(s->nx, std::max(a.nsig, a.nout) * a.pb, sizeof(T)));
+        }
+        for (int k = 0; k < 3; ++k) {
+            g.blocks[k] = f[k].back();
+            first.insert(first.end(), f[k].begin(), f[k].end());
+        }
+        (void)gm;
+    }
+    if (e->sgroups.empty()) return SS_OK;
+    int lim = 0;
+    SS_HIP_CHECK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, e->m[0]->p.device));
+    const int cap = std::min(lim, 144 * 1024);
+    for (auto& g : e->sgroups) {
+        if (g.lds_plane > (size_t)cap || g.lds_pencil > (size_t)cap)
+            return fail(SS_ERR_UNSUPPORTED, "batched spectral stage: LDS of a group exceeds the device limit");
+        hipError_t err = with_spec_p(g.p, [&](auto pc) {
+            constexpr int P = decltype(pc)::value;
+            const hipFuncAttribute at = hipFuncAttributeMaxDynamicSharedMemorySize;
+            hipError_t r = hipFuncSetAttribute((const void*)k_spec_planes_fwd_batched<T, 6, P>, at, cap);
+            if (r == hipSuccess) r = hipFuncSetAttribute((const void*)k_spec_planes_fwd_batched<T, 9, P>, at, cap);
+            if (r == hipSuccess) r = hipFuncSetAttribute((const void*)k_spec_pencils_batched<T, P>, at, cap);
+            if (r == hipSuccess) r = hipFuncSetAttribute((const void*)k_spec_planes_inv_batched<T, P>, at, cap);
+            return r;
+        });
+        SS_HIP_CHECK(err);
+    }
+    if (hipMalloc(&e->d_specjobs, jobs.size() * sizeof(SpecJob<T>)) != hipSuccess ||
+        hipMalloc((void**)&e->d_specfirst, first.size() * sizeof(int)) != hipSuccess)
+        return fail(SS_ERR_ALLOC, "hipMalloc");
+    SS_HIP_CHECK(hipMemcpy(e->d_specjobs, jobs.data(), jobs.size() * sizeof(SpecJob<T>), hipMemcpyHostToDevice));
+    SS_HIP_CHECK(hipMemcpy(e->d_specfirst, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice));
+    return SS_OK;
+}
+
+// The spectral stage of group g on stream st: three launches.
+template <typename T>
+static hipError_t ens_launch_spec(ss_ensemble* e, const ss_ensemble::SpecGroup& g, hipStream_t st) {
+    const int n = (int)g.members.size();
+    const SpecJob<T>* jobs = reinterpret_cast<const SpecJob<T>*>((const char*)e->d_specjobs + g.job_off);
+    const int* f0 = e->d_specfirst + g.first_off;
+    const int* f1 = f0 + (n + 1);
+    const int* f2 = f1 + (n + 1);
+    return with_spec_p(g.p, [&](auto pc) {
+        constexpr int P = decltype(pc)::value;
+        if (e->nm == 6)
+            hipLaunchKernelGGL((k_spec_planes_fwd_batched<T, 6, P>), dim3(g.blocks[0]), dim3(SPEC_PLANE_THREADS),
+                               g.lds_plane, st, jobs, f0, n);
+        else
+            hipLaunchKernelGGL((k_spec_planes_fwd_batched<T, 9, P>), dim3(g.blocks[0]), dim3(SPEC_PLANE_THREADS),
+                               g.lds_plane, st, jobs, f0, n);
+        hipLaunchKernelGGL((k_spec_pencils_batched<T, P>), dim3(g.blocks[1]), dim3(SPEC_PENCIL_THREADS),
+                           g.lds_pencil, st, jobs, f1, n);
+        hipLaunchKernelGGL((k_spec_planes_inv_batched<T, P>), dim3(g.blocks[2]), dim3(SPEC_PLANE_THREADS),
+                           g.lds_plane, st, jobs, f2, n);
+        return hipGetLastError();
+    });
+}
+
 // Job-table slot matching the members' current buffers.
 static int ens_slot(const ss_ensemble* e) { return member_parity(e->m[0]) == e->built_for[0] ? 0 : 1; }
 
@@ -1961,7 +2104,17 @@ static int ens_step_once(ss_ensemble* e, int k, bool capturing) {
     hipError_t err;
     SS_HIP_CHECK(hipEventRecord(e->fork, e->stream));
     for (int q = 0; q < NSIDE; ++q) SS_HIP_CHECK(hipStreamWaitEvent(e->side[q], e->fork, 0));
-    for (ss_sim* s : e->m) {
+    for (size_t g = 0; g < e->sgroups.size(); ++g) {
+        if ((err = ens_launch_spec<T>(e, e->sgroups[g], e->side[g % NSIDE])) != hipSuccess)
+            return fail(SS_ERR_DEVICE, std::string("batched spectral stage: ") + hipGetErrorString(err));
+    }
+    for (size_t j = 0; j < e->m.size(); ++j) {
+        ss_sim* s = e->m[j];
+        if (e->batched[j]) {  // what spectral_stage records for the fused path
+            s->spec_partial = true;
+            s->spec_packed = false;
+            continue;
+        }
         int rc = spectral_stage<T>(s, nullptr);
         if (rc != SS_OK) return rc;
     }
@@ -2138,6 +2291,10 @@ int ss_ensemble_create(const ss_params* params, int nsys, ss_ensemble** out) {
         return bail(fail(SS_ERR_ALLOC, "hipMalloc"));
     if (hipMemcpy(e->d_start, start.data(), sizeof(int) * (nsys + 1), hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(SS_ERR_DEVICE, "copy"));
+    {
+        const int rc = e->prec == SS_FP64 ? ens_build_spec<double>(e) : ens_build_spec<float>(e);
+        if (rc != SS_OK) return bail(rc);
+    }
     *out = e;
     return SS_OK;
 }
@@ -2163,6 +2320,8 @@ int ss_ensemble_destroy(ss_ensemble* e) {
     if (e->d_step) (void)hipFree(e->d_step);
     if (e->d_jobs) (void)hipFree(e->d_jobs);
     if (e->d_start) (void)hipFree(e->d_start);
+    if (e->d_specjobs) (void)hipFree(e->d_specjobs);
+    if (e->d_specfirst) (void)hipFree(e->d_specfirst);
     delete e;
     return SS_OK;
 }

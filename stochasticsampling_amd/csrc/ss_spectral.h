@@ -44,13 +44,13 @@ __device__ inline void batched_loop(int n, Load load, Store store) {
 }
 
 // The transform of nlines lines of one axis: with P > 0 every axis has the
-// power-of-two length P and the stages run with their sizes fixed at compile
-// time (NL > 0: nlines = NL as well); the arithmetic is the same.
+// length P and the stages run with their sizes fixed at compile time (NL > 0:
+// nlines = NL as well); the arithmetic is the same.
 template <typename T, bool LINES_FAST, int NT, int P, int NL>
 __device__ inline cplx<T>* spec_lines(cplx<T>* a, cplx<T>* b, const cplx<T>* tw, const SpecAxis& ax, int nlines,
                                       int ls, int es, int sign) {
     if constexpr (P > 0)
-        return fft_lines_pow2<T, LINES_FAST, NT, P, NL>(a, b, tw, nlines, ls, es, sign);
+        return fft_lines_ct<T, LINES_FAST, NT, P, NL>(a, b, tw, nlines, ls, es, sign);
     else
         return fft_lines<T, LINES_FAST, NT>(a, b, tw, ax, nlines, ls, es, sign);
 }
@@ -79,19 +79,20 @@ inline size_t spec_plane_lds(int ny, int nz, size_t esz) { return esz * 2 * ((si
 // Forward: component q < NM of plane x; the blocks q == NM convert the count
 // component (density, when wanted) and transform nothing, so that no block
 // carries both.  Clears the inflow entries it reads.
+// bid: the block's index in the launch of one system.
 template <typename T, int NM, int P>
-__global__ void __launch_bounds__(SPEC_PLANE_THREADS)
-    k_spec_planes_fwd(const long long* __restrict__ main, long long* __restrict__ inflow, int nx, double scale,
-                      double rho_scale, SpecAxis ay, SpecAxis az, const cplx<T>* __restrict__ twy,
-                      const cplx<T>* __restrict__ twz, cplx<T>* __restrict__ spec, T* __restrict__ rho,
-                      T* __restrict__ rec, int rs) {
+__device__ __forceinline__ void spec_planes_fwd_body(const long long* __restrict__ main, long long* __restrict__ inflow, int nx,
+                                            double scale, double rho_scale, const SpecAxis& ay, const SpecAxis& az,
+                                            const cplx<T>* __restrict__ twy, const cplx<T>* __restrict__ twz,
+                                            cplx<T>* __restrict__ spec, T* __restrict__ rho, T* __restrict__ rec,
+                                            int rs, int bid) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     const int ny = P ? P : ay.n, nz = P ? P : az.n, np = ny * nz;
     cplx<T>* ty = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* tz = ty + ny;
     cplx<T>* b0 = tz + nz;
     cplx<T>* b1 = b0 + np;
-    const int x = blockIdx.x % nx, q = blockIdx.x / nx;
+    const int x = bid % nx, q = bid / nx;
     const size_t ns = (size_t)nx * np, base = (size_t)x * np;
     if (q == NM) {
         spec_count_plane<T, SPEC_PLANE_THREADS>(main + (size_t)NM * ns + base, inflow + (size_t)NM * ns + base, np,
@@ -114,6 +115,16 @@ __global__ void __launch_bounds__(SPEC_PLANE_THREADS)
     cplx<T>* r2 = spec_lines<T, true, SPEC_PLANE_THREADS, P, P>(r1, r1 == b0 ? b1 : b0, ty, ay, nz, 1, nz, -1);
     cplx<T>* out = spec + (size_t)q * ns + base;
     for (int c = threadIdx.x; c < np; c += SPEC_PLANE_THREADS) out[c] = r2[c];
+}
+
+template <typename T, int NM, int P>
+__global__ void __launch_bounds__(SPEC_PLANE_THREADS)
+    k_spec_planes_fwd(const long long* __restrict__ main, long long* __restrict__ inflow, int nx, double scale,
+                      double rho_scale, SpecAxis ay, SpecAxis az, const cplx<T>* __restrict__ twy,
+                      const cplx<T>* __restrict__ twz, cplx<T>* __restrict__ spec, T* __restrict__ rho,
+                      T* __restrict__ rec, int rs) {
+    spec_planes_fwd_body<T, NM, P>(main, inflow, nx, scale, rho_scale, ay, az, twy, twz, spec, rho, rec, rs,
+                                   (int)blockIdx.x);
 }
 
 // Wave vector, 1/|k| and 1/k^2 of bin (x, pencil p = y nz + z) from the
@@ -144,12 +155,15 @@ inline size_t spec_pencil_lds(int nx, int nl, size_t esz) {
 // Pencils p0 .. p0+pb-1 (p = y nz + z) of the nsig stress components: forward
 // x transform, flow solve, inverse x transform of the nout = 6 (u, W) or 12
 // (u, W, E) outputs.
+// bid, groups: the block's index and the number of blocks in the launch of
+// one system.
 template <typename T, int P>
-__global__ void __launch_bounds__(SPEC_PENCIL_THREADS)
-    k_spec_pencils(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int np, int nz, int pb_max,
-                   int nsig, int nout, SpecAxis ax, const cplx<T>* __restrict__ twx, const T* __restrict__ kx,
-                   const T* __restrict__ ky, const T* __restrict__ kz, const T* __restrict__ ik2,
-                   const T* __restrict__ kinv, T screening, T norm) {
+__device__ __forceinline__ void spec_pencils_body(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int np, int nz,
+                                         int pb_max, int nsig, int nout, const SpecAxis& ax,
+                                         const cplx<T>* __restrict__ twx, const T* __restrict__ kx,
+                                         const T* __restrict__ ky, const T* __restrict__ kz,
+                                         const T* __restrict__ ik2, const T* __restrict__ kinv, T screening, T norm,
+                                         int bid, int groups) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     const int nx = P ? P : ax.n, ls = nx + 1;
     const int nl = (nsig > nout ? nsig : nout) * pb_max;
@@ -160,13 +174,14 @@ __global__ void __launch_bounds__(SPEC_PENCIL_THREADS)
     // workgroups are dispatched round-robin over the XCDs: give each XCD a
     // contiguous run of pencil groups, so that the two groups sharing a
     // cache line of every x row are read through the same L2
-    const int groups = gridDim.x;
-    const int g = groups % 8 == 0 ? (blockIdx.x % 8) * (groups / 8) + blockIdx.x / 8 : blockIdx.x;
+    const int g = groups % 8 == 0 ? (bid % 8) * (groups / 8) + bid / 8 : bid;
     const int p0 = g * pb_max;
     const int pb = min(pb_max, np - p0);
     for (int t = threadIdx.x; t < nx; t += SPEC_PENCIL_THREADS) tw[t] = twx[t];
     // line (c, p) at A[(c pb_max + p) ls]; lanes over p for coalesced loads
-    batched_loop<SPEC_PENCIL_THREADS>(
+    // four loads at a time: eight take the kernel past 128 VGPRs (one
+    // workgroup per CU) where all axes have compile-time lengths
+    batched_loop<SPEC_PENCIL_THREADS, 4>(
         nsig * nx * pb_max,
         [&](int f) {
             const int p = f % pb_max, xx = (f / pb_max) % nx, c = f / (pb_max * nx);
@@ -207,20 +222,29 @@ __global__ void __launch_bounds__(SPEC_PENCIL_THREADS)
     }
 }
 
+template <typename T, int P>
+__global__ void __launch_bounds__(SPEC_PENCIL_THREADS)
+    k_spec_pencils(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int np, int nz, int pb_max,
+                   int nsig, int nout, SpecAxis ax, const cplx<T>* __restrict__ twx, const T* __restrict__ kx,
+                   const T* __restrict__ ky, const T* __restrict__ kz, const T* __restrict__ ik2,
+                   const T* __restrict__ kinv, T screening, T norm) {
+    spec_pencils_body<T, P>(in, out, np, nz, pb_max, nsig, nout, ax, twx, kx, ky, kz, ik2, kinv, screening, norm,
+                            (int)blockIdx.x, (int)gridDim.x);
+}
+
 // Inverse: field component o of plane x; the real part goes to record slot
 // o (u, W) or REC_E + o - 6 (E).
 template <typename T, int P>
-__global__ void __launch_bounds__(SPEC_PLANE_THREADS)
-    k_spec_planes_inv(const cplx<T>* __restrict__ spec, int nx, SpecAxis ay, SpecAxis az,
-                      const cplx<T>* __restrict__ twy, const cplx<T>* __restrict__ twz, T* __restrict__ rec,
-                      int rs) {
+__device__ __forceinline__ void spec_planes_inv_body(const cplx<T>* __restrict__ spec, int nx, const SpecAxis& ay,
+                                            const SpecAxis& az, const cplx<T>* __restrict__ twy,
+                                            const cplx<T>* __restrict__ twz, T* __restrict__ rec, int rs, int bid) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     const int ny = P ? P : ay.n, nz = P ? P : az.n, np = ny * nz;
     cplx<T>* ty = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* tz = ty + ny;
     cplx<T>* b0 = tz + nz;
     cplx<T>* b1 = b0 + np;
-    const int x = blockIdx.x % nx, o = blockIdx.x / nx;
+    const int x = bid % nx, o = bid / nx;
     const size_t ns = (size_t)nx * np, base = (size_t)x * np;
     for (int t = threadIdx.x; t < ny; t += SPEC_PLANE_THREADS) ty[t] = twy[t];
     for (int t = threadIdx.x; t < nz; t += SPEC_PLANE_THREADS) tz[t] = twz[t];
@@ -231,6 +255,72 @@ __global__ void __launch_bounds__(SPEC_PLANE_THREADS)
     cplx<T>* r2 = spec_lines<T, false, SPEC_PLANE_THREADS, P, P>(r1, r1 == b0 ? b1 : b0, tz, az, ny, nz, 1, +1);
     const int slot = o < 6 ? o : REC_E + o - 6;
     for (int c = threadIdx.x; c < np; c += SPEC_PLANE_THREADS) rec[(base + c) * rs + slot] = r2[c].x;
+}
+
+template <typename T, int P>
+__global__ void __launch_bounds__(SPEC_PLANE_THREADS)
+    k_spec_planes_inv(const cplx<T>* __restrict__ spec, int nx, SpecAxis ay, SpecAxis az,
+                      const cplx<T>* __restrict__ twy, const cplx<T>* __restrict__ twz, T* __restrict__ rec,
+                      int rs) {
+    spec_planes_inv_body<T, P>(spec, nx, ay, az, twy, twz, rec, rs, (int)blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Batched launches: the spectral stages of several systems (an ensemble's
+// members of one axis-length class) in one launch of each kernel.  first[m]
+// is the first block of system m (first[n] the total), so every block runs
+// the body of its own system with its index within that system.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct SpecJob {
+    const long long* main;
+    long long* inflow;
+    cplx<T>* spec_in;
+    cplx<T>* spec_out;
+    T* rho;  // nullptr: no density
+    T* rec;
+    const cplx<T> *twx, *twy, *twz;
+    const T *kx, *ky, *kz, *ik2, *kinv;
+    double scale, rho_scale;
+    T screening, norm;
+    SpecAxis ax, ay, az;
+    int nx, np, nz, rs, pb, nsig, nout;
+};
+
+// The system a block belongs to: the last m with first[m] <= b.
+__device__ inline int spec_job_of(const int* __restrict__ first, int n, int b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);  // uniform: the job stays in scalar registers
+}
+
+template <typename T, int NM, int P>
+__global__ void __launch_bounds__(SPEC_PLANE_THREADS)
+    k_spec_planes_fwd_batched(const SpecJob<T>* __restrict__ jobs, const int* __restrict__ first, int n) {
+    const int m = spec_job_of(first, n, (int)blockIdx.x);
+    const SpecJob<T>& a = jobs[m];
+    spec_planes_fwd_body<T, NM, P>(a.main, a.inflow, a.nx, a.scale, a.rho_scale, a.ay, a.az, a.twy, a.twz,
+                                   a.spec_in, a.rho, a.rec, a.rs, (int)blockIdx.x - first[m]);
+}
+
+template <typename T, int P>
+__global__ void __launch_bounds__(SPEC_PENCIL_THREADS)
+    k_spec_pencils_batched(const SpecJob<T>* __restrict__ jobs, const int* __restrict__ first, int n) {
+    const int m = spec_job_of(first, n, (int)blockIdx.x);
+    const SpecJob<T>& a = jobs[m];
+    spec_pencils_body<T, P>(a.spec_in, a.spec_out, a.np, a.nz, a.pb, a.nsig, a.nout, a.ax, a.twx, a.kx, a.ky, a.kz,
+                            a.ik2, a.kinv, a.screening, a.norm, (int)blockIdx.x - first[m], first[m + 1] - first[m]);
+}
+
+template <typename T, int P>
+__global__ void __launch_bounds__(SPEC_PLANE_THREADS)
+    k_spec_planes_inv_batched(const SpecJob<T>* __restrict__ jobs, const int* __restrict__ first, int n) {
+    const int m = spec_job_of(first, n, (int)blockIdx.x);
+    const SpecJob<T>& a = jobs[m];
+    spec_planes_inv_body<T, P>(a.spec_out, a.nx, a.ay, a.az, a.twy, a.twz, a.rec, a.rs, (int)blockIdx.x - first[m]);
 }
 
 // ---------------------------------------------------------------------------

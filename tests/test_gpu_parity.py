@@ -348,6 +348,7 @@ def test_graph_replays_match_eager_steps(precision, binning):
     [
         ((16, 16, 16, 8, 8), {}),
         ((32, 32, 32, 4, 4), {}),
+        ((48, 48, 48, 2, 2), dict(volume_exclusion=0.3)),
         ((64, 64, 64, 2, 2), dict(shape=0.5)),
         ((12, 20, 15, 6, 4), dict(shape=0.3, volume_exclusion=0.2)),
         ((16, 8, 10, 8, 8), dict(stress_magnetic=0.7)),
@@ -358,7 +359,8 @@ def test_fused_spectral_stage_equals_axis_passes(precision, grid, extra):
     """The fused plane/pencil kernels of the spectral stage run the same radix
     stages, axis order and solve as convert + one pass per FFT axis + solve +
     pack, so particles and fields agree bit for bit -- also where all axes are
-    16, 32 or 64 long and the stages have their sizes fixed at compile time."""
+    16, 32, 48 or 64 long and the stages have their sizes fixed at compile
+    time."""
     extra = dict(extra)
     flags = extra.pop("flags", 0)
     box = tuple(0.8 * g for g in grid[:3])
@@ -423,15 +425,21 @@ def test_paired_spectral_stage_matches_unpaired(precision, tol, grid, extra):
 @pytest.mark.parametrize("precision", [ss.FP32, ss.FP64])
 def test_ensemble_members_equal_single_systems(precision):
     """Members of a batched ensemble (different grids, boxes, particle counts,
-    alpha and D_r) evolve bit for bit like the same systems stepped alone."""
-    specs = [((16, 16, 16, 8, 8), 12.5, 5000, -2.0, 0.1), ((12, 12, 12, 6, 6), 9.4, 3000, 0.5, 1.0),
-             ((8, 8, 8, 4, 4), 6.2, 700, 1.7, 0.5), ((20, 16, 12, 8, 8), 15.6, 9000, -0.3, 0.5),
-             ((16, 16, 16, 8, 8), 12.5, 1, 1.0, 0.1)]
+    alpha and D_r) evolve bit for bit like the same systems stepped alone.
+    The members' spectral stages run batched by axis-length class (16^3 and
+    32^3 with compile-time sizes, the rest at run time, one with the strain
+    and the density), except a member on the paired stage, which keeps its
+    own launches."""
+    specs = [((16, 16, 16, 8, 8), 12.5, 5000, -2.0, 0.1, {}), ((12, 12, 12, 6, 6), 9.4, 3000, 0.5, 1.0, {}),
+             ((8, 8, 8, 4, 4), 6.2, 700, 1.7, 0.5, {}), ((20, 16, 12, 8, 8), 15.6, 9000, -0.3, 0.5, {}),
+             ((16, 16, 16, 8, 8), 12.5, 1, 1.0, 0.1, {}),
+             ((32, 32, 32, 4, 4), 25.0, 20000, 0.8, 0.5, dict(shape=0.5, volume_exclusion=0.2)),
+             ((16, 12, 10, 6, 6), 12.5, 4000, -1.0, 0.5, dict(flags=ss.FLAG_PAIRED_SPECTRAL))]
     plist, parts = [], []
-    for k, (grid, L, n, alpha, dr) in enumerate(specs):
+    for k, (grid, L, n, alpha, dr, extra) in enumerate(specs):
         p = ss.make_params(box=(L, L * grid[1] / grid[0], L * grid[2] / grid[0]), grid=grid, n=n, seed=100 + k,
                            stress_active=alpha, diffusion_rotational=dr, precision=precision,
-                           binning=ss.BIN_SORTED, resort_interval=4)
+                           binning=ss.BIN_SORTED, resort_interval=4, **extra)
         plist.append(p)
         parts.append(ss.initial_condition(p, amplitude=0.05, seed=100 + k))
     ens = ss.Ensemble(plist)
