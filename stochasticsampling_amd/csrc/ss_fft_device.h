@@ -64,43 +64,88 @@ __device__ inline void dft_small(cplx<T>* v, const cplx<T>* tw, int n, int sign)
 // memory, es > 1); otherwise consecutive butterflies of one line (contiguous
 // lines, es = 1).  NC > 0 fixes n = NC and Ns = NSC at compile time, NLC > 0
 // the number of lines (the same arithmetic, with the index divisions folded).
+// Butterfly b of a radix-R stage: line l, position j, twiddle index k.
+template <bool LINES_FAST>
+__device__ inline void butterfly_index(int b, int nlines, int m, int Ns, int& l, int& j, int& k) {
+    if (LINES_FAST) {
+        l = b % nlines;
+        j = b / nlines;
+    } else {
+        l = b / m;
+        j = b - l * m;
+    }
+    k = j % Ns;
+}
+
+// Reads butterfly (l, j, k) of line base s and transforms it in v.
+template <typename T, int R>
+__device__ inline void butterfly(const cplx<T>* s, const cplx<T>* tw, int n, int m, int tstep, int j, int k, int es,
+                                 int sign, cplx<T> v[R]) {
+#pragma clang fp contract(on)
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        cplx<T> x = s[(j + r * m) * es];
+        if (r > 0 && k > 0) {
+            cplx<T> c = tw[(r * k * tstep) % n];
+            if (sign > 0) c.y = -c.y;
+            x = cmul(x, c);
+        }
+        v[r] = x;
+    }
+    dft_small<T, R>(v, tw, n, sign);
+}
+
 template <typename T, int R, bool LINES_FAST, int NT, int NC = 0, int NSC = 0, int NLC = 0>
 __device__ inline void stage(const cplx<T>* src, cplx<T>* dst, const cplx<T>* tw, int n_, int Ns_, int nlines_,
                              int ls, int es, int sign) {
-#pragma clang fp contract(on)
     const int n = NC ? NC : n_;
     const int Ns = NC ? NSC : Ns_;
     const int nlines = NLC ? NLC : nlines_;
     const int m = n / R;
     const int tstep = n / (Ns * R);
     for (int b = threadIdx.x; b < nlines * m; b += NT) {
-        int l, j;
-        if (LINES_FAST) {
-            l = b % nlines;
-            j = b / nlines;
-        } else {
-            l = b / m;
-            j = b - l * m;
-        }
-        const int k = j % Ns;
-        const cplx<T>* s = src + l * ls;
+        int l, j, k;
+        butterfly_index<LINES_FAST>(b, nlines, m, Ns, l, j, k);
         cplx<T> v[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            cplx<T> x = s[(j + r * m) * es];
-            if (r > 0 && k > 0) {
-                cplx<T> c = tw[(r * k * tstep) % n];
-                if (sign > 0) c.y = -c.y;
-                x = cmul(x, c);
-            }
-            v[r] = x;
-        }
-        dft_small<T, R>(v, tw, n, sign);
+        butterfly<T, R>(src + l * ls, tw, n, m, tstep, j, k, es, sign, v);
         const int idxD = (j / Ns) * Ns * R + k;
         cplx<T>* d = dst + l * ls;
 #pragma unroll
         for (int r = 0; r < R; ++r) d[(idxD + r * Ns) * es] = v[r];
     }
+}
+
+// The same stage in place, for sizes fixed at compile time: every thread
+// keeps its butterflies in registers between a read phase and a write phase
+// (one LDS buffer instead of two).  Ends with a barrier.
+template <typename T, int R, bool LINES_FAST, int NT, int N, int NS, int NL>
+__device__ inline void stage_inplace(cplx<T>* buf, const cplx<T>* tw, int ls, int es, int sign) {
+    constexpr int m = N / R, tstep = N / (NS * R), total = NL * m;
+    constexpr int BPT = (total + NT - 1) / NT;
+    cplx<T> v[BPT][R];
+#pragma unroll
+    for (int u = 0; u < BPT; ++u) {
+        const int b = (int)threadIdx.x + u * NT;
+        if (total % NT == 0 || b < total) {
+            int l, j, k;
+            butterfly_index<LINES_FAST>(b, NL, m, NS, l, j, k);
+            butterfly<T, R>(buf + l * ls, tw, N, m, tstep, j, k, es, sign, v[u]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < BPT; ++u) {
+        const int b = (int)threadIdx.x + u * NT;
+        if (total % NT == 0 || b < total) {
+            int l, j, k;
+            butterfly_index<LINES_FAST>(b, NL, m, NS, l, j, k);
+            const int idxD = (j / NS) * NS * R + k;
+            cplx<T>* d = buf + l * ls;
+#pragma unroll
+            for (int r = 0; r < R; ++r) d[(idxD + r * NS) * es] = v[u][r];
+        }
+    }
+    __syncthreads();
 }
 
 // Radix plan of one axis (factors 4, 2, 3, 5 only).
@@ -152,6 +197,18 @@ __device__ inline cplx<T>* fft_lines_ct(cplx<T>* src, cplx<T>* dst, const cplx<T
         stage<T, R, LINES_FAST, NT, N, NS, NL>(src, dst, tw, N, NS, nlines, ls, es, sign);
         __syncthreads();
         return fft_lines_ct<T, LINES_FAST, NT, N, NL, NS * R>(dst, src, tw, nlines, ls, es, sign);
+    }
+}
+
+// fft_lines_ct in place (NL lines, all sizes at compile time); the result is
+// in buf.
+template <typename T, bool LINES_FAST, int NT, int N, int NL, int NS = 1>
+__device__ inline void fft_lines_ct_inplace(cplx<T>* buf, const cplx<T>* tw, int ls, int es, int sign) {
+    if constexpr (NS < N) {
+        constexpr int R = next_radix(N / NS);
+        static_assert((N / NS) % R == 0, "lengths with factors 2, 3 and 5 only");
+        stage_inplace<T, R, LINES_FAST, NT, N, NS, NL>(buf, tw, ls, es, sign);
+        fft_lines_ct_inplace<T, LINES_FAST, NT, N, NL, NS * R>(buf, tw, ls, es, sign);
     }
 }
 

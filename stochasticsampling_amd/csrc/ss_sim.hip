@@ -406,7 +406,7 @@ template <typename T>
 static hipError_t launch_spec_planes_fwd(ss_sim* s) {
     double ms, ds;
     scales(s, ms, ds);
-    const size_t lds = spec_plane_lds(s->ny, s->nz, sizeof(T));
+    const size_t lds = spec_plane_lds(s->ny, s->nz, sizeof(T), s->spec_p > 0);
     const dim3 grid((unsigned)(s->nx * (s->nm + 1)));  // + the count component
     T* rho = s->use_rho ? (T*)s->rho : nullptr;
     const SpecAxis ay = spec_axis(s->fft.ax[1]), az = spec_axis(s->fft.ax[2]);
@@ -447,7 +447,7 @@ template <typename T>
 static hipError_t launch_spec_planes_fwd2(ss_sim* s) {
     double ms, ds;
     scales(s, ms, ds);
-    const size_t lds = spec_plane_lds(s->ny, s->nz, sizeof(T));
+    const size_t lds = spec_plane_lds(s->ny, s->nz, sizeof(T), s->spec_p > 0);
     const int npk = (s->nm + 1) / 2;
     const dim3 grid((unsigned)(s->nx * (npk + 1)));  // + the count component
     T* rho = s->use_rho ? (T*)s->rho : nullptr;
@@ -489,7 +489,7 @@ static hipError_t launch_spec_pencils2(ss_sim* s) {
 template <typename T>
 static hipError_t launch_spec_planes_inv2(ss_sim* s) {
     const int nout = s->use_e ? 12 : 6;
-    const size_t lds = spec_plane_lds(s->ny, s->nz, sizeof(T));
+    const size_t lds = spec_plane_lds(s->ny, s->nz, sizeof(T), s->spec_p > 0);
     const dim3 grid((unsigned)(s->nx * (nout / 2)));
     return with_spec_p(s->spec_p, [&](auto pc) {
         constexpr int P = decltype(pc)::value;
@@ -503,7 +503,7 @@ static hipError_t launch_spec_planes_inv2(ss_sim* s) {
 template <typename T>
 static hipError_t launch_spec_planes_inv(ss_sim* s) {
     const int nout = s->use_e ? 12 : 6;
-    const size_t lds = spec_plane_lds(s->ny, s->nz, sizeof(T));
+    const size_t lds = spec_plane_lds(s->ny, s->nz, sizeof(T), s->spec_p > 0);
     const dim3 grid((unsigned)(s->nx * nout));
     return with_spec_p(s->spec_p, [&](auto pc) {
         constexpr int P = decltype(pc)::value;
@@ -946,18 +946,21 @@ static int choose_spectral(ss_sim* s) {
     int lim = 0;
     SS_HIP_CHECK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, s->p.device));
     const size_t cap = std::min<size_t>((size_t)lim, 144 * 1024);
-    const size_t plane = spec_plane_lds(s->ny, s->nz, sizeof(T));
+    const SpecAxis a0 = spec_axis(s->fft.ax[0]), a1 = spec_axis(s->fft.ax[1]), a2 = spec_axis(s->fft.ax[2]);
+    s->spec_p = 0;
+    for (int P : {16, 32, 48, 64})
+        if (ct_plan(a0, P) && ct_plan(a1, P) && ct_plan(a2, P)) s->spec_p = P;
+    const size_t plane = spec_plane_lds(s->ny, s->nz, sizeof(T), s->spec_p > 0);
     const int nmax = std::max(s->nsig, s->nout);
     // four pencils (pencil pairs) per workgroup: at 64^3 more workgroups of
     // fewer lines finish sooner than fewer larger ones, down to this size
     int pb = 4;
     while (pb > 1 && spec_pencil_lds(s->nx, nmax * pb, sizeof(T)) > 64 * 1024) pb /= 2;
     const size_t pencil = spec_pencil_lds(s->nx, nmax * pb, sizeof(T));
-    if (plane > cap || pencil > cap) return SS_OK;
-    const SpecAxis a0 = spec_axis(s->fft.ax[0]), a1 = spec_axis(s->fft.ax[1]), a2 = spec_axis(s->fft.ax[2]);
-    s->spec_p = 0;
-    for (int P : {16, 32, 48, 64})
-        if (ct_plan(a0, P) && ct_plan(a1, P) && ct_plan(a2, P)) s->spec_p = P;
+    if (plane > cap || pencil > cap) {
+        s->spec_p = 0;
+        return SS_OK;
+    }
     const bool pair = (s->p.flags & SS_FLAG_PAIRED_SPECTRAL) != 0;
     hipError_t ae = with_spec_p(s->spec_p, [&](auto pc) {
         constexpr int P = decltype(pc)::value;
@@ -2035,7 +2038,7 @@ static int ens_build_spec(ss_ensemble* e) {
             f[0].push_back(f[0].back() + s->nx * (s->nm + 1));
             f[1].push_back(f[1].back() + (a.np + a.pb - 1) / a.pb);
             f[2].push_back(f[2].back() + s->nx * a.nout);
-            g.lds_plane = std::max(g.lds_plane, spec_plane_lds(s->ny, s->nz, sizeof(T)));
+            g.lds_plane = std::max(g.lds_plane, spec_plane_lds(s->ny, s->nz, sizeof(T), s->spec_p > 0));
             g.lds_pencil = std::max(g.lds_pencil, spec_pencil_lds(s->nx, std::max(a.nsig, a.nout) * a.pb, sizeof(T)));
         }
         for (int k = 0; k < 3; ++k) {

@@ -74,7 +74,41 @@ __device__ inline void spec_count_plane(const long long* __restrict__ mc, long l
 }
 
 // LDS of one plane kernel: twiddles of y and z, two plane buffers.
-inline size_t spec_plane_lds(int ny, int nz, size_t esz) { return esz * 2 * ((size_t)ny + nz + 2 * (size_t)ny * nz); }
+// With sizes fixed at compile time (P > 0) the transforms run in place in one
+// plane buffer.
+inline size_t spec_plane_lds(int ny, int nz, size_t esz, bool inplace) {
+    return esz * 2 * ((size_t)ny + nz + (inplace ? 1 : 2) * (size_t)ny * nz);
+}
+
+// The two transforms of a plane held at b0 (b1: second buffer, run-time
+// sizes only): forward z then y, or inverse y then z.  Returns the buffer
+// with the result.
+template <typename T, int P>
+__device__ inline cplx<T>* spec_plane_fwd(cplx<T>* b0, cplx<T>* b1, const cplx<T>* ty, const cplx<T>* tz,
+                                          const SpecAxis& ay, const SpecAxis& az, int ny, int nz) {
+    // z lines: ny contiguous lines; y lines: nz lines interleaved at stride nz
+    if constexpr (P > 0) {
+        fft_lines_ct_inplace<T, false, SPEC_PLANE_THREADS, P, P>(b0, tz, nz, 1, -1);
+        fft_lines_ct_inplace<T, true, SPEC_PLANE_THREADS, P, P>(b0, ty, 1, nz, -1);
+        return b0;
+    } else {
+        cplx<T>* r1 = spec_lines<T, false, SPEC_PLANE_THREADS, 0, 0>(b0, b1, tz, az, ny, nz, 1, -1);
+        return spec_lines<T, true, SPEC_PLANE_THREADS, 0, 0>(r1, r1 == b0 ? b1 : b0, ty, ay, nz, 1, nz, -1);
+    }
+}
+
+template <typename T, int P>
+__device__ inline cplx<T>* spec_plane_inv(cplx<T>* b0, cplx<T>* b1, const cplx<T>* ty, const cplx<T>* tz,
+                                          const SpecAxis& ay, const SpecAxis& az, int ny, int nz) {
+    if constexpr (P > 0) {
+        fft_lines_ct_inplace<T, true, SPEC_PLANE_THREADS, P, P>(b0, ty, 1, nz, +1);
+        fft_lines_ct_inplace<T, false, SPEC_PLANE_THREADS, P, P>(b0, tz, nz, 1, +1);
+        return b0;
+    } else {
+        cplx<T>* r1 = spec_lines<T, true, SPEC_PLANE_THREADS, 0, 0>(b0, b1, ty, ay, nz, 1, nz, +1);
+        return spec_lines<T, false, SPEC_PLANE_THREADS, 0, 0>(r1, r1 == b0 ? b1 : b0, tz, az, ny, nz, 1, +1);
+    }
+}
 
 // Forward: component q < NM of plane x; the blocks q == NM convert the count
 // component (density, when wanted) and transform nothing, so that no block
@@ -110,9 +144,7 @@ __device__ __forceinline__ void spec_planes_fwd_body(const long long* __restrict
             b0[c] = cplx<T>{(T)((double)v * scale), T(0)};
         });
     __syncthreads();
-    // z lines: ny contiguous lines; y lines: nz lines interleaved at stride nz
-    cplx<T>* r1 = spec_lines<T, false, SPEC_PLANE_THREADS, P, P>(b0, b1, tz, az, ny, nz, 1, -1);
-    cplx<T>* r2 = spec_lines<T, true, SPEC_PLANE_THREADS, P, P>(r1, r1 == b0 ? b1 : b0, ty, ay, nz, 1, nz, -1);
+    cplx<T>* r2 = spec_plane_fwd<T, P>(b0, b1, ty, tz, ay, az, ny, nz);
     cplx<T>* out = spec + (size_t)q * ns + base;
     for (int c = threadIdx.x; c < np; c += SPEC_PLANE_THREADS) out[c] = r2[c];
 }
@@ -251,8 +283,7 @@ __device__ __forceinline__ void spec_planes_inv_body(const cplx<T>* __restrict__
     const cplx<T>* src = spec + (size_t)o * ns + base;
     batched_loop<SPEC_PLANE_THREADS>(np, [&](int c) { return src[c]; }, [&](int c, cplx<T> v) { b0[c] = v; });
     __syncthreads();
-    cplx<T>* r1 = spec_lines<T, true, SPEC_PLANE_THREADS, P, P>(b0, b1, ty, ay, nz, 1, nz, +1);
-    cplx<T>* r2 = spec_lines<T, false, SPEC_PLANE_THREADS, P, P>(r1, r1 == b0 ? b1 : b0, tz, az, ny, nz, 1, +1);
+    cplx<T>* r2 = spec_plane_inv<T, P>(b0, b1, ty, tz, ay, az, ny, nz);
     const int slot = o < 6 ? o : REC_E + o - 6;
     for (int c = threadIdx.x; c < np; c += SPEC_PLANE_THREADS) rec[(base + c) * rs + slot] = r2[c].x;
 }
@@ -383,8 +414,7 @@ __global__ void __launch_bounds__(SPEC_PLANE_THREADS)
                 b0[t] = cplx<T>{(T)((double)v * scale), T(0)};
             });
     __syncthreads();
-    cplx<T>* r1 = spec_lines<T, false, SPEC_PLANE_THREADS, P, P>(b0, b1, tz, az, ny, nz, 1, -1);
-    cplx<T>* r2 = spec_lines<T, true, SPEC_PLANE_THREADS, P, P>(r1, r1 == b0 ? b1 : b0, ty, ay, nz, 1, nz, -1);
+    cplx<T>* r2 = spec_plane_fwd<T, P>(b0, b1, ty, tz, ay, az, ny, nz);
     cplx<T>* out = spec + (size_t)c * ns + base;
     for (int t = threadIdx.x; t < np; t += SPEC_PLANE_THREADS) out[t] = r2[t];
 }
@@ -529,8 +559,7 @@ __global__ void __launch_bounds__(SPEC_PLANE_THREADS)
     const cplx<T>* src = spec + (size_t)d * ns + base;
     batched_loop<SPEC_PLANE_THREADS>(np, [&](int t) { return src[t]; }, [&](int t, cplx<T> v) { b0[t] = v; });
     __syncthreads();
-    cplx<T>* r1 = spec_lines<T, true, SPEC_PLANE_THREADS, P, P>(b0, b1, ty, ay, nz, 1, nz, +1);
-    cplx<T>* r2 = spec_lines<T, false, SPEC_PLANE_THREADS, P, P>(r1, r1 == b0 ? b1 : b0, tz, az, ny, nz, 1, +1);
+    cplx<T>* r2 = spec_plane_inv<T, P>(b0, b1, ty, tz, ay, az, ny, nz);
     const int o0 = 2 * d, o1 = 2 * d + 1;
     const int sl0 = o0 < 6 ? o0 : REC_E + o0 - 6, sl1 = o1 < 6 ? o1 : REC_E + o1 - 6;
     for (int t = threadIdx.x; t < np; t += SPEC_PLANE_THREADS) {
