@@ -167,11 +167,10 @@ def time_ensemble(steps, warmup):
     import stochasticsampling_amd as ss
     from stochasticsampling_amd import autotune
 
-    # tile shape per size class from the tuning table; the members share one
-    # re-sort interval (that of the first member's class)
-    plist = [autotune.tuned(p) for p in ensemble_configs()]
+    # tile shapes per grid class and the shared re-sort interval from the
+    # ensemble's entry of the tuning table (tuned on the whole ensemble)
+    plist = autotune.tuned_ensemble(ensemble_configs())
     for p in plist:
-        p.resort_interval = plist[0].resort_interval
         if not any(p.tile):
             p.tile = (ctypes.c_int32 * 3)(4, 4, 4)
     ens = ss.Ensemble(plist)
@@ -189,7 +188,10 @@ def time_ensemble(steps, warmup):
             "systems": len(plist), "particles": total, "dtype": "f32",
             "workload": "48 systems, alpha in [-2,2], D_r in {0.1,0.5,1}, grids 32^3x8x8/48^3x12x12/"
                         "64^3x16x16, L = 0.78 x grid points, N log-uniform in [2^18, 2^22], seeds 100-147, "
-                        "one persistent update launch per step, periods replayed from hipGraphs"}
+                        "spectral stages batched per grid class, one persistent update launch per step, "
+                        "periods replayed from hipGraphs, tile shapes tuned on the ensemble",
+            "tiles": {autotune.grid_class(p): list(p.tile) for p in plist},
+            "resort_interval": plist[0].resort_interval}
 
 
 def copy_probe(gib=1.0, reps=10):

@@ -14,6 +14,15 @@ Size class: precision, grid, particle count rounded to a power of two, the
 number of moment components (6 or 9), and whether the strain, density and
 quasi-2D options are on.
 
+Ensembles (``tune_ensemble`` / ``tuned_ensemble``): inside a batched ensemble
+the members' tiles run in one persistent launch with plenty of parallelism
+across members, so the best shape of a size class there is not the one it
+has alone (larger tiles pay their per-tile setup over more particles).  The
+search then runs on the whole ensemble: the shape of each grid class (size
+class without the particle count: the members of a grid share the kernels'
+shapes) in turn with the others fixed, then the shared re-sort interval; the
+entry is keyed by the ensemble's size classes and member count.
+
 Usage::
 
     python -m stochasticsampling_amd.autotune --grid 64 64 64 16 16 --n 16777216 --out table.json
@@ -29,6 +38,8 @@ from .params import BIN_SORTED, FLAG_QUASI2D, FP32, FP64, make_params
 
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json")
 TILES = [(4, 4, 4), (2, 4, 4), (4, 4, 2), (4, 4, 8), (4, 8, 4), (2, 2, 4), (8, 4, 4)]
+# ensembles also try larger tiles (fewer per member)
+ENSEMBLE_TILES = TILES + [(8, 8, 4), (8, 4, 8), (4, 8, 8), (8, 8, 8)]
 RESORTS = [8, 16, 32]
 LDS_LIMIT = 65536  # tile window + moment kernels of the tiled update
 
@@ -56,9 +67,9 @@ def _window(t, n):
     return ta, ta + 2 * h, (n + ta - 1) // ta
 
 
-def candidates(p, tiles=TILES):
+def candidates(p, tiles=TILES, min_tiles=256):
     """Tile shapes valid for ``p`` (window within the LDS budget, at least
-    256 tiles), without duplicates after clamping to the grid."""
+    ``min_tiles`` tiles), without duplicates after clamping to the grid."""
     g = p.grid_size
     dims = (g.x, g.y, g.z)
     nm = 9 if p.stress_magnetic != 0.0 else 6
@@ -69,7 +80,7 @@ def candidates(p, tiles=TILES):
         eff = tuple(a[0] for a in ax)
         ws = ax[0][1] * ax[1][1] * ax[2][1]
         ntiles = ax[0][2] * ax[1][2] * ax[2][2]
-        if eff in seen or ws * (nm + 1) * 8 + nm * na * 8 > LDS_LIMIT or ntiles < 256:
+        if eff in seen or ws * (nm + 1) * 8 + nm * na * 8 > LDS_LIMIT or ntiles < min_tiles:
             continue
         seen.add(eff)
         out.append(eff)
@@ -167,6 +178,109 @@ def tuned(p, table=None, path=None, tune_missing=False, **tune_kw):
     if entry is None:
         return p
     return _with(p, tuple(entry["tile"]), entry["resort_interval"])
+
+
+def grid_class(p):
+    """Size class without the particle count."""
+    parts = size_class(p).split("|")
+    return "|".join(parts[:2] + parts[3:])
+
+
+def ensemble_key(plist):
+    """Key of an ensemble: its members' size classes (sorted, without
+    repeats) and the member count."""
+    classes = sorted({size_class(p) for p in plist})
+    return "ensemble|" + "+".join(classes) + f"|m{len(plist)}"
+
+
+def measure_ensemble(plist, parts, periods=2):
+    """Wall-clock ms per ensemble step over ``periods`` re-sort periods after
+    one warm-up period (members share ``plist[0].resort_interval``)."""
+    from .simulation import Ensemble
+
+    ens = Ensemble(plist)
+    try:
+        for m, x in zip(ens.members, parts):
+            m.init(x)
+        k = max(1, plist[0].resort_interval)
+        ens.step(k)
+        t0 = time.perf_counter()
+        ens.step(periods * k)  # returns after the steps have run
+        return (time.perf_counter() - t0) * 1e3 / (periods * k)
+    finally:
+        ens.close()
+
+
+def tune_ensemble(plist, parts=None, tiles=ENSEMBLE_TILES, resorts=RESORTS, measure_fn=None, periods=3,
+                  start=None, min_tiles=16):
+    """Coordinate search on the whole ensemble: starting from ``start``
+    ({grid class: tile}, default 4x4x4), each grid class's tile shape in
+    turn with the others fixed (at a re-sort interval of 16), then the shared
+    re-sort interval.  Returns the table entry {"tiles": {grid class: tile},
+    "resort_interval", "ms_per_step", "trials"}."""
+    if measure_fn is None:
+        if parts is None:
+            from .simulation import initial_condition
+
+            parts = [initial_condition(p, amplitude=0.05, seed=p.seed) for p in plist]
+
+        def measure_fn(qs):
+            return measure_ensemble(qs, parts, periods)
+
+    classes = sorted({grid_class(p) for p in plist})
+    first = {}
+    for p in plist:
+        first.setdefault(grid_class(p), p)
+    if start is None:
+        start = {c: (4, 4, 4) for c in classes}
+    cur = {c: tuple(start[c]) for c in classes}
+
+    def build(tiles_of, resort):
+        return [_with(p, tiles_of[grid_class(p)], resort) for p in plist]
+
+    trials = []
+    best_ms = measure_fn(build(cur, 16))
+    trials.append({"tiles": {c: list(t) for c, t in cur.items()}, "resort_interval": 16, "ms_per_step": best_ms})
+    for c in classes:
+        for t in candidates(first[c], tiles, min_tiles):
+            if t == cur[c]:
+                continue
+            trial = dict(cur)
+            trial[c] = t
+            ms = measure_fn(build(trial, 16))
+            trials.append({"tiles": {k: list(v) for k, v in trial.items()}, "resort_interval": 16, "ms_per_step": ms})
+            if ms < best_ms:
+                best_ms, cur = ms, trial
+    best_r = 16
+    for r in resorts:
+        if r == 16:
+            continue
+        ms = measure_fn(build(cur, r))
+        trials.append({"tiles": {k: list(v) for k, v in cur.items()}, "resort_interval": r, "ms_per_step": ms})
+        if ms < best_ms:
+            best_ms, best_r = ms, r
+    return {"tiles": {c: list(t) for c, t in cur.items()}, "resort_interval": best_r, "ms_per_step": best_ms,
+            "trials": trials}
+
+
+def tuned_ensemble(plist, table=None, path=None, tune_missing=False, **tune_kw):
+    """Members of an ensemble with the tile shapes and the shared re-sort
+    interval of its entry (``ensemble_key``); on a miss, tune when
+    ``tune_missing``, else each member as ``tuned`` gives it, sharing the
+    first member's re-sort interval."""
+    table = load_table(path) if table is None else table
+    key = ensemble_key(plist)
+    entry = table.get(key)
+    if entry is None and tune_missing:
+        entry = tune_ensemble(plist, **tune_kw)
+        table[key] = entry
+        save_table(table, path)
+    if entry is None:
+        out = [type(p).from_buffer_copy(tuned(p, table)) for p in plist]
+        for q in out:
+            q.resort_interval = out[0].resort_interval
+        return out
+    return [_with(p, tuple(entry["tiles"][grid_class(p)]), entry["resort_interval"]) for p in plist]
 
 
 def main():

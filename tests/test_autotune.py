@@ -80,9 +80,23 @@ def test_tuned_uses_table_and_respects_explicit_choices(tmp_path):
 
 def test_shipped_table_entries_are_valid():
     for key, e in at.load_table().items():
-        assert len(e["tile"]) == 3 and all(v >= 1 for v in e["tile"])
         assert e["resort_interval"] >= 1
+        if key.startswith("ensemble|"):
+            assert key.rsplit("|m", 1)[1].isdigit()
+            for c, t in e["tiles"].items():
+                assert c.count("|") == 3 and len(t) == 3 and all(v >= 1 for v in t)
+            continue
+        assert len(e["tile"]) == 3 and all(v >= 1 for v in e["tile"])
         assert key.count("|") == 4
+
+
+def test_shipped_ensemble_entry_covers_the_benchmark_ensemble():
+    import bench
+
+    plist = bench.ensemble_configs()
+    assert at.ensemble_key(plist) in at.load_table()
+    out = at.tuned_ensemble(plist)
+    assert len({q.resort_interval for q in out}) == 1 and all(any(q.tile) for q in out)
 
 
 @pytest.mark.gpu
@@ -96,3 +110,50 @@ def test_tune_small_system_on_gpu():
     assert len(e["trials"]) == len(cands) + 1 and all(t["ms_per_step"] > 0 for t in e["trials"])
     sim = ss.Simulation(at.tuned(p, table={at.size_class(p): e}))
     assert list(sim.get_info()["tile"]) == e["tile"]
+
+
+def _ens():
+    return [_p(grid=(32, 32, 32, 8, 8), n=1 << 18), _p(grid=(64, 64, 64, 16, 16), n=1 << 20),
+            _p(grid=(32, 32, 32, 8, 8), n=(1 << 18) + 5)]
+
+
+def test_ensemble_key_is_the_set_of_classes_and_the_member_count():
+    plist = _ens()
+    assert at.grid_class(plist[0]) == "fp32|32x32x32x8x8|nm6|e0r0q0"
+    assert at.grid_class(_p(n=1 << 10)) == at.grid_class(_p())
+    k = at.ensemble_key(plist)
+    assert k == at.ensemble_key(list(reversed(plist)))
+    assert k.startswith("ensemble|") and k.endswith("|m3")
+    assert k.count("fp32|") == 2  # two distinct classes
+    assert at.ensemble_key(plist[:2]) != k
+
+
+def test_tune_ensemble_coordinate_search_with_model_timings():
+    """Each class's shape in turn with the others fixed, then the interval;
+    the model favours 4x4x8 for the 32^3 class and 8x8x4 for the 64^3 one."""
+    plist = _ens()
+    c32, c64 = at.grid_class(plist[0]), at.grid_class(plist[1])
+    calls = []
+
+    def fake(qs):
+        assert len(qs) == 3 and all(q.binning == ss.BIN_SORTED for q in qs)
+        assert len({q.resort_interval for q in qs}) == 1
+        t32 = {tuple(q.tile) for q in qs if at.grid_class(q) == c32}
+        assert len(t32) == 1  # members of one grid share the shape
+        t64 = tuple(next(q.tile for q in qs if at.grid_class(q) == c64))
+        calls.append(1)
+        cost = {(4, 4, 8): 1.0}.get(t32.pop(), 1.5) + {(8, 8, 4): 1.0}.get(t64, 2.0)
+        return cost * {8: 1.1, 16: 1.0, 32: 0.9}[qs[0].resort_interval]
+
+    e = at.tune_ensemble(plist, measure_fn=fake, start={c32: (4, 4, 4), c64: (4, 4, 4)})
+    assert e["tiles"] == {c32: [4, 4, 8], c64: [8, 8, 4]}
+    assert e["resort_interval"] == 32 and abs(e["ms_per_step"] - 1.8) < 1e-12
+    assert len(calls) == len(e["trials"])
+    # applying the entry
+    key = at.ensemble_key(plist)
+    out = at.tuned_ensemble(plist, table={key: e})
+    assert [tuple(q.tile) for q in out] == [(4, 4, 8), (8, 8, 4), (4, 4, 8)]
+    assert {q.resort_interval for q in out} == {32}
+    # a miss without tuning: the standalone entries, one shared interval
+    miss = at.tuned_ensemble(plist, table={})
+    assert len({q.resort_interval for q in miss}) == 1
