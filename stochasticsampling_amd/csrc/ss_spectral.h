@@ -12,7 +12,12 @@
 //
 // The transforms are the Stockham stages of ss_fft_device.h in the same axis
 // order as the axis-pass path (forward z, y, x; backward x, y, z) and the
-// solve is flow_point, so both paths give identical results.
+// solve is flow_point, so both paths give identical results.  Where all three
+// axes have one length P of 16, 32, 48 or 64 points the stages are
+// instantiated for it (sizes fixed at compile time; planes transformed in
+// place).  The *_batched kernels run the same bodies for several systems in
+// one launch (ensembles); the *2 kernels are the paired variant (two real
+// fields per complex transform).
 #pragma once
 #include "ss_fft_device.h"
 #include "ss_kernels.h"
