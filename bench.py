@@ -88,7 +88,8 @@ def state_error_study(parts, steps):
     from stochasticsampling_amd import autotune
 
     n = parts.shape[0]
-    sims = [ss.Simulation(autotune.tuned(make(ss.FP32, n))), ss.Simulation(make(ss.FP64, n, binning=ss.BIN_HISTOGRAM))]
+    p32 = autotune.tuned(make(ss.FP32, n))
+    sims = [ss.Simulation(p32), ss.Simulation(make(ss.FP64, n, binning=ss.BIN_HISTOGRAM))]
     out = []
     for sim in sims:
         sim.init(parts)
@@ -97,7 +98,7 @@ def state_error_study(parts, steps):
         out.append((sim.get_particles(), u, g))
         sim.close()
     (a, ua, ga), (b, ub, gb) = out
-    L = np.array([50.0, 50.0, 50.0])
+    L = np.array([p32.box_size.x, p32.box_size.y, p32.box_size.z])
     d = np.abs(a[:, :3] - b[:, :3])
     d = np.minimum(d, L - d) / L
 
