@@ -231,6 +231,13 @@ enum : int {
 };
 
 template <typename T>
+struct UpdateParams;
+
+// Whether any term that FULL = false leaves out is on.
+template <typename T>
+__host__ __device__ inline bool full_physics(const UpdateParams<T>& P);
+
+template <typename T>
 struct UpdateParams {
     T dt;
     T drag;            // magnetic_drag
@@ -246,6 +253,11 @@ struct UpdateParams {
     int quasi2d;
 };
 
+template <typename T>
+__host__ __device__ inline bool full_physics(const UpdateParams<T>& P) {
+    return P.use_gb || P.kappa != T(0) || P.use_b || P.use_e || P.use_rho || P.quasi2d;
+}
+
 // One particle: the LangevinBuilder chain (src/bin/simulation/mod.rs:348-363,
 // modifiers.rs).  rec: field record of the particle's cell.  rv: RandomVector.
 // Outputs the position increment dp (to be added with pbc) and the new
@@ -254,6 +266,7 @@ struct UpdateParams {
 // (|x| < 0.5, error below 1e-8), the library functions otherwise.
 template <typename T>
 __device__ inline void half_angle(T x, T* s, T* c) {
+#pragma clang fp contract(on)
     if constexpr (sizeof(T) == 4) {
         if (x < 0.5f && x > -0.5f) {
             const float x2 = x * x;
@@ -265,9 +278,15 @@ __device__ inline void half_angle(T x, T* s, T* c) {
     Math<T>::sincos(x, s, c);
 }
 
-template <typename T>
+// FULL = false compiles out the terms that need a field or option the
+// parameters leave off (magnetic force and rotation, external field, strain,
+// density-dependent diffusion, quasi-2D); the caller guarantees they are off.
+template <typename T, bool FULL = true>
 __device__ inline void langevin(const UpdateParams<T>& P, T phi, T theta, const T* rec, const Noise<T>& rv,
                                 T dp[3], T& ophi, T& otheta) {
+    // contractions fixed by the source, so that both instantiations (and the
+    // ensemble kernel) round alike
+#pragma clang fp contract(on)
     T sp, cp, st, ct;
     Math<T>::sincos(phi, &sp, &cp);
     Math<T>::sincos(theta, &st, &ct);
@@ -278,7 +297,7 @@ __device__ inline void langevin(const UpdateParams<T>& P, T phi, T theta, const 
     T p2 = n2 + rec[REC_U + 2];
     T o0 = T(0), o1 = T(0), o2 = T(0);
     // magnetic_dipole_dipole_force (modifiers.rs:59-69, magnetic_interaction/mod.rs:13-17)
-    if (P.use_gb) {
+    if (FULL && P.use_gb) {
         const T* g = rec + REC_GB;
         T f0 = (g[0] * n0 + g[1] * n1) + g[2] * n2;
         T f1 = (g[3] * n0 + g[4] * n1) + g[5] * n2;
@@ -288,7 +307,7 @@ __device__ inline void langevin(const UpdateParams<T>& P, T phi, T theta, const 
         p2 = p2 + f2 * P.drag;
     }
     // external_field_alignment (modifiers.rs:103-118)
-    if (P.kappa != T(0)) {
+    if (FULL && P.kappa != T(0)) {
         T b0, b1, b2;
         if (P.default_dir) {
             b0 = T(0);
@@ -305,7 +324,7 @@ __device__ inline void langevin(const UpdateParams<T>& P, T phi, T theta, const 
         o2 = o2 + (b2 - n2 * nb);
     }
     // magnetic_dipole_dipole_rotation (modifiers.rs:173-186)
-    if (P.use_b) {
+    if (FULL && P.use_b) {
         const T* b = rec + REC_B;
         T nb = (n0 * b[0] + n1 * b[1]) + n2 * b[2];
         o0 = o0 + (b[0] - n0 * nb);
@@ -323,7 +342,7 @@ __device__ inline void langevin(const UpdateParams<T>& P, T phi, T theta, const 
         o2 = o2 - f2;
     }
     // jeffrey_strain (modifiers.rs:147-169)
-    if (P.use_e) {
+    if (FULL && P.use_e) {
         const T* e = rec + REC_E;
         T f0 = ((e[0] * n0 + e[1] * n1) + e[2] * n2) * P.shape;
         T f1 = ((e[1] * n0 + e[3] * n1) + e[4] * n2) * P.shape;
@@ -342,7 +361,7 @@ __device__ inline void langevin(const UpdateParams<T>& P, T phi, T theta, const 
     o2 *= P.dt;
     // translational_diffusion (modifiers.rs:89-99), std of mod.rs:346
     {
-        T diff = P.use_rho ? Math<T>::sqrt(T(2) * P.dt * (P.d_trans + P.vol_excl * rec[REC_RHO]))
+        T diff = (FULL && P.use_rho) ? Math<T>::sqrt(T(2) * P.dt * (P.d_trans + P.vol_excl * rec[REC_RHO]))
                            : Math<T>::sqrt(T(2) * P.dt * P.d_trans);
         p0 = p0 + rv.x * diff;
         p1 = p1 + rv.y * diff;
@@ -377,7 +396,7 @@ __device__ inline void langevin(const UpdateParams<T>& P, T phi, T theta, const 
     const T nphi = Math<T>::atan2(v1, v0);
     const T ntheta = Math<T>::PI / T(2) - Math<T>::atan2(v2, rxy);
     ang_pbc_from_vector(nphi, ntheta, ophi, otheta);
-    if (P.quasi2d) otheta = Math<T>::PI / T(2);
+    if (FULL && P.quasi2d) otheta = Math<T>::PI / T(2);
     dp[0] = p0;
     dp[1] = p1;
     dp[2] = p2;

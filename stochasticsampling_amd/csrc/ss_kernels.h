@@ -361,17 +361,17 @@ __device__ inline void load_particle(const State<T>& s, uint64_t i, Loaded<T>& q
 
 // Update of one loaded particle with its cell record (global memory or
 // registers): run the LangevinBuilder chain, move, wrap, store at slot i.
-template <typename T>
+template <typename T, bool FULL = true>
 __device__ inline void move_loaded(const State<T>& s, uint64_t i, const Loaded<T>& q, const UpdateParams<T>& P,
                                    const T* rec, const Noise<T>& rv, T& nx, T& ny, T& nz, T& ophi, T& otheta) {
     T dp[3];
-    langevin<T>(P, q.phi, q.theta, rec, rv, dp, ophi, otheta);
+    langevin<T, FULL>(P, q.phi, q.theta, rec, rv, dp, ophi, otheta);
     if constexpr (sizeof(T) == 4) {
         float lx = q.lx, ly = q.ly, lz = q.lz;
         nx = add_wrap<T>(q.x, dp[0], P.box[0], &lx);
         ny = add_wrap<T>(q.y, dp[1], P.box[1], &ly);
         nz = add_wrap<T>(q.z, dp[2], P.box[2], &lz);
-        if (P.quasi2d) {
+        if (FULL && P.quasi2d) {
             nz = T(0);
             lz = 0.0f;
         }
@@ -382,7 +382,7 @@ __device__ inline void move_loaded(const State<T>& s, uint64_t i, const Loaded<T
         nx = add_wrap<T>(q.x, dp[0], P.box[0], nullptr);
         ny = add_wrap<T>(q.y, dp[1], P.box[1], nullptr);
         nz = add_wrap<T>(q.z, dp[2], P.box[2], nullptr);
-        if (P.quasi2d) nz = T(0);
+        if (FULL && P.quasi2d) nz = T(0);
     }
     s.x[i] = nx;
     s.y[i] = ny;
@@ -408,14 +408,14 @@ __device__ inline void gather_record(const float* __restrict__ src, int rs, floa
 }
 
 // Update of every particle, plain path (particles in their original order).
-template <typename T>
+template <typename T, bool FULL = true>
 __device__ inline void move_particle(const State<T>& s, uint64_t i, const Geo<T>& g, const UpdateParams<T>& P,
                                      const T* __restrict__ rec, int rs, const Noise<T>& rv, T& nx, T& ny, T& nz,
                                      T& ophi, T& otheta) {
     Loaded<T> q;
     load_particle(s, i, q);
     const int c = cell_of(g, q.x, q.y, q.z);
-    move_loaded(s, i, q, P, rec + (size_t)c * rs, rv, nx, ny, nz, ophi, otheta);
+    move_loaded<T, FULL>(s, i, q, P, rec + (size_t)c * rs, rv, nx, ny, nz, ophi, otheta);
 }
 
 // Noise sources of the update: Philox drawn in the kernel; caller-supplied
@@ -459,7 +459,7 @@ __device__ inline void draw_noise(const UpdateParams<T>& P, uint64_t seed, uint6
     }
 }
 
-template <typename T, int NZ>
+template <typename T, int NZ, bool FULL = true>
 __global__ void __launch_bounds__(256)
     k_update_plain(State<T> s, uint64_t n, Geo<T> g, UpdateParams<T> P, const T* __restrict__ rec, int rs,
                    uint64_t seed, const uint64_t* __restrict__ step, uint32_t step_off,
@@ -469,7 +469,7 @@ __global__ void __launch_bounds__(256)
     Noise<T> rv;
     draw_noise<T, NZ>(P, seed, *step + step_off, (uint32_t)i, n, noise, rv);
     T nx, ny, nz, ophi, otheta;
-    move_particle<T>(s, i, g, P, rec, rs, rv, nx, ny, nz, ophi, otheta);
+    move_particle<T, FULL>(s, i, g, P, rec, rs, rv, nx, ny, nz, ophi, otheta);
 }
 
 // Global diagnostics in FP64 (mean orientation of the particles, mean

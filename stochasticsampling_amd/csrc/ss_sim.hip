@@ -519,6 +519,19 @@ static void launch_tile_kernel(ss_sim* s, const TileArgs<T, NM>& a) {
     const int per = (s->tg.ntiles + s->tg.nxcd - 1) / s->tg.nxcd;
     const dim3 grid((unsigned)(per * s->tg.nxcd));
     const size_t lds = tile_lds_bytes(s->tg.ws, NM, s->na);
+    // the default physics (no magnetic, strain, density or quasi-2D terms)
+    // with those terms compiled out
+    if constexpr (MOVE && NZ == NZ_PHILOX && NM == 6) {
+        if (!full_physics(a.P)) {
+            if (a.dtoff)
+                hipLaunchKernelGGL((k_tile_update<T, NM, MOVE, NZ, KEEP, true, false>), grid, dim3(256), lds,
+                                   s->stream, a);
+            else
+                hipLaunchKernelGGL((k_tile_update<T, NM, MOVE, NZ, KEEP, false, false>), grid, dim3(256), lds,
+                                   s->stream, a);
+            return;
+        }
+    }
     if (MOVE && a.dtoff)
         hipLaunchKernelGGL((k_tile_update<T, NM, MOVE, NZ, KEEP, MOVE>), grid, dim3(256), lds, s->stream, a);
     else
@@ -613,8 +626,11 @@ static hipError_t launch_update(ss_sim* s) {
     else if (s->p.noise_mode == SS_NOISE_REFERENCE)
         hipLaunchKernelGGL((k_update_plain<T, NZ_REC>), grid, dim3(256), 0, s->stream, st, s->n, g, P,
                            (const T*)s->rec, s->rs, s->p.seed, s->d_step, s->step_off, (const T*)s->noise);
-    else
+    else if (full_physics(P))
         hipLaunchKernelGGL((k_update_plain<T, NZ_PHILOX>), grid, dim3(256), 0, s->stream, st, s->n, g, P,
+                           (const T*)s->rec, s->rs, s->p.seed, s->d_step, s->step_off, (const T*)s->noise);
+    else  // the default physics with the optional terms compiled out
+        hipLaunchKernelGGL((k_update_plain<T, NZ_PHILOX, false>), grid, dim3(256), 0, s->stream, st, s->n, g, P,
                            (const T*)s->rec, s->rs, s->p.seed, s->d_step, s->step_off, (const T*)s->noise);
     return hipGetLastError();
 }

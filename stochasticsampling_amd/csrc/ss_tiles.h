@@ -93,7 +93,7 @@ inline size_t tile_lds_bytes(int ws, int nm, int na) {
 // also runs for a tile without particles); a pass that does not own the tile
 // adds its own cells to the inflow accumulators instead.  kf must hold the
 // fixed-point moment kernels; acc is the window.  Ends with a barrier.
-template <typename T, int NM, bool MOVE, int NZ, bool KEEP, bool SCATTER>
+template <typename T, int NM, bool MOVE, int NZ, bool KEEP, bool SCATTER, bool FULL = true>
 __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0, uint32_t q1, bool owner,
                                  long long* acc, const long long* kf) {
     constexpr int NC = NM + 1;
@@ -165,9 +165,9 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
                 out.id[o] = cid;
             }
             if constexpr (sizeof(T) == 4)
-                move_loaded<T>(out, o, cur, a.P, reinterpret_cast<const T*>(r), rv, nx, ny, nz, ophi, otheta);
+                move_loaded<T, FULL>(out, o, cur, a.P, reinterpret_cast<const T*>(r), rv, nx, ny, nz, ophi, otheta);
             else
-                move_loaded<T>(out, o, cur, a.P, recp, rv, nx, ny, nz, ophi, otheta);
+                move_loaded<T, FULL>(out, o, cur, a.P, recp, rv, nx, ny, nz, ophi, otheta);
             cur = nxt;
             cid = nid;
             if constexpr (NZ == NZ_REC) ncur = nnxt;
@@ -241,7 +241,7 @@ __device__ inline void load_kf(const long long* __restrict__ kfix, int na, long 
 
 // One system: workgroup `blockIdx.x` moves and bins the particles of one
 // tile (tile_for_block).
-template <typename T, int NM, bool MOVE, int NZ, bool KEEP, bool SCATTER>
+template <typename T, int NM, bool MOVE, int NZ, bool KEEP, bool SCATTER, bool FULL = true>
 __global__ void __launch_bounds__(256) k_tile_update(TileArgs<T, NM> a) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     const int tile = tile_for_block(a.tg, blockIdx.x);
@@ -249,7 +249,7 @@ __global__ void __launch_bounds__(256) k_tile_update(TileArgs<T, NM> a) {
     long long* acc = reinterpret_cast<long long*>(smem_raw);  // [ws][NM+1]
     long long* kf = acc + (size_t)a.tg.ws * (NM + 1);         // [NM][na]
     load_kf<NM>(a.kfix, a.g.na, kf);
-    tile_pass<T, NM, MOVE, NZ, KEEP, SCATTER>(a, tile, a.toff[tile], a.toff[tile + 1], true, acc, kf);
+    tile_pass<T, NM, MOVE, NZ, KEEP, SCATTER, FULL>(a, tile, a.toff[tile], a.toff[tile + 1], true, acc, kf);
 }
 
 // Many systems in one persistent launch: work item w of the concatenated
