@@ -70,55 +70,51 @@ __global__ void __launch_bounds__(256)
     atomicAdd(hist + (size_t)c * g.na + a, 1u);
 }
 
-// Contraction of one cell's angular counts against the moment kernels
+// Contraction of the angular counts of each cell against the moment kernels
 // (average_stress, stress.rs:60-96; DirectorField, director.rs:36-61; density,
-// src/bin/simulation/mod.rs:312-316), one wavefront per cell.  kern: [nm][na]
-// in LDS; out: moments m (times scale) as complex with zero imaginary part.
-template <typename T, int NM>
-__device__ inline void contract_cell(const uint32_t* cnt, int na, const T* kern, T scale, T rho_scale,
-                                     int cell, int ns, cplx<T>* spec, T* rho) {
-    const int lane = __lane_id();
-    T acc[NM];
-    T tot = T(0);
-#pragma unroll
-    for (int q = 0; q < NM; ++q) acc[q] = T(0);
-    for (int a = lane; a < na; a += 64) {
-        const T c = (T)cnt[a];
-        tot += c;
-#pragma unroll
-        for (int q = 0; q < NM; ++q) acc[q] += kern[q * na + a] * c;
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-        for (int q = 0; q < NM; ++q) acc[q] += __shfl_xor(acc[q], off);
-        tot += __shfl_xor(tot, off);
-    }
-    if (lane < NM) {
-        T v = acc[0];
-#pragma unroll
-        for (int q = 1; q < NM; ++q)
-            if (lane == q) v = acc[q];
-        spec[(size_t)lane * ns + cell] = cplx<T>{v * scale, T(0)};
-    }
-    if (rho && lane == 0) rho[cell] = tot * rho_scale;
-}
-
+// src/bin/simulation/mod.rs:312-316).  kern: [nm][na]; out: moments m (times
+// scale) as complex with zero imaginary part.  One thread per cell: the
+// cell's counts are read along its row (16-byte loads when the row length
+// allows), the kernels are the same for every thread (LDS broadcast).  A
+// wavefront per cell spent most of its time in the cross-lane reduction of
+// seven sums.
 template <typename T, int NM>
 __global__ void __launch_bounds__(256)
     k_contract_hist(const uint32_t* __restrict__ hist, Geo<T> g, const T* __restrict__ kern_g, T scale,
                     T rho_scale, cplx<T>* __restrict__ spec, T* __restrict__ rho) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     T* kern = reinterpret_cast<T*>(smem_raw);
-    for (int t = threadIdx.x; t < NM * g.na; t += blockDim.x) kern[t] = kern_g[t];
+    const int na = g.na;
+    for (int t = threadIdx.x; t < NM * na; t += blockDim.x) kern[t] = kern_g[t];
     __syncthreads();
-    const int wave = threadIdx.x >> 6;
-    for (int k = 0; k < 16; ++k) {
-        const int cell = blockIdx.x * 64 + k * 4 + wave;
-        if (cell >= g.ns) return;
-        contract_cell<T, NM>(hist + (size_t)cell * g.na, g.na, kern, scale, rho_scale, cell, g.ns, spec,
-                             rho);
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= g.ns) return;
+    const uint32_t* cnt = hist + (size_t)cell * na;
+    T acc[NM];
+    T tot = T(0);
+#pragma unroll
+    for (int q = 0; q < NM; ++q) acc[q] = T(0);
+    auto add = [&](int a, uint32_t n) {
+        const T c = (T)n;
+        tot += c;
+#pragma unroll
+        for (int q = 0; q < NM; ++q) acc[q] += kern[q * na + a] * c;
+    };
+    if ((na & 3) == 0) {
+        const uint4* c4 = reinterpret_cast<const uint4*>(cnt);
+        for (int a = 0; a < na; a += 4) {
+            const uint4 v = c4[a >> 2];
+            add(a, v.x);
+            add(a + 1, v.y);
+            add(a + 2, v.z);
+            add(a + 3, v.w);
+        }
+    } else {
+        for (int a = 0; a < na; ++a) add(a, cnt[a]);
     }
+#pragma unroll
+    for (int q = 0; q < NM; ++q) spec[(size_t)q * g.ns + cell] = cplx<T>{acc[q] * scale, T(0)};
+    if (rho) rho[cell] = tot * rho_scale;
 }
 
 // ---------------------------------------------------------------------------

@@ -333,7 +333,7 @@ static hipError_t launch_bin(ss_sim* s) {
         hipLaunchKernelGGL(k_bin_hist<T>, dim3(blocks_for(s->n)), dim3(256), 0, s->stream, (const T*)s->x,
                            (const T*)s->y, (const T*)s->z, (const T*)s->phi, (const T*)s->theta, s->n, g,
                            s->hist);
-        const unsigned nb = (unsigned)((s->ns + 63) / 64);
+        const unsigned nb = (unsigned)((s->ns + 255) / 256);
         const size_t lds = (size_t)s->nm * s->na * sizeof(T);
         if (s->nm == 6)
             hipLaunchKernelGGL((k_contract_hist<T, 6>), dim3(nb), dim3(256), lds, s->stream, s->hist, g,
