@@ -354,7 +354,7 @@ def main():
             traffic = json.load(f).get("traffic_bytes_per_launch")
     achieved = UPDATE_BYTES["f32"] * n / (upd_ms * 1e-3) / 1e9
     roofline = {
-        "kernel": "ss::k_tile_update<float, 6, true, 0, false, false>",
+        "kernel": "ss::k_tile_update<float, 6, true, 0, false, false, false>",
         "bound": "hbm",
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
