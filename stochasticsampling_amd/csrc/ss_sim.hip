@@ -2013,7 +2013,6 @@ static int ens_build_spec(ss_ensemble* e) {
     for (auto& g : e->sgroups) {
         g.job_off = jobs.size() * sizeof(SpecJob<T>);
         g.first_off = first.size();
-        const int gm = (int)g.members.size();
         std::vector<int> f[3];
         for (int k = 0; k < 3; ++k) f[k].assign(1, 0);
         for (int j : g.members) {
@@ -2061,7 +2060,6 @@ static int ens_build_spec(ss_ensemble* e) {
             g.blocks[k] = f[k].back();
             first.insert(first.end(), f[k].begin(), f[k].end());
         }
-        (void)gm;
     }
     if (e->sgroups.empty()) return SS_OK;
     int lim = 0;
