@@ -13,6 +13,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <type_traits>
 #include <string>
@@ -1896,6 +1897,7 @@ struct ss_ensemble {
     int nm = 6;
     int resort = 16;
     bool ext = false;
+    bool full = true;  // some member needs the terms the default-physics update compiles out
     // persistent-update job tables: [slot 0/1][in place / re-sort] -> nsys TileArgs
     void* d_jobs = nullptr;
     int* d_start = nullptr;
@@ -1915,6 +1917,7 @@ struct ss_ensemble {
         size_t lds_plane = 0, lds_pencil = 0;
         size_t job_off = 0;        // byte offset of its SpecJob array in d_specjobs
         size_t first_off = 0;      // int offset of its three first[] tables in d_specfirst
+        int side = 0;              // side stream of its launches
     };
     std::vector<SpecGroup> sgroups;
     std::vector<char> batched;  // per member: its spectral stage runs in a group
@@ -1954,6 +1957,8 @@ static int ens_build_jobs(ss_ensemble* e) {
             }
     }
     SS_HIP_CHECK(hipMemcpy(e->d_jobs, h.data(), h.size() * sizeof(TileArgs<T, NM>), hipMemcpyHostToDevice));
+    e->full = false;
+    for (int j = 0; j < n; ++j) e->full = e->full || full_physics(h[j].P);
     e->built_for.clear();
     for (ss_sim* s : e->m) e->built_for.push_back(member_parity(s));
     return SS_OK;
@@ -1975,6 +1980,13 @@ static void ens_launch_update(ss_ensemble* e, int slot, bool resort, int k) {
         else
             hipLaunchKernelGGL((k_tile_update_batched<T, NM, true, true, false>), dim3(grid), dim3(256), lds,
                                e->stream, jobs, e->d_start, n, nj, ko);
+    } else if (NM == 6 && !e->full) {  // every member on the default physics
+        if (resort)
+            hipLaunchKernelGGL((k_tile_update_batched<T, NM, true, NZ_PHILOX, true, false>), dim3(grid), dim3(256),
+                               lds, e->stream, jobs, e->d_start, n, nj, ko);
+        else
+            hipLaunchKernelGGL((k_tile_update_batched<T, NM, true, NZ_PHILOX, false, false>), dim3(grid), dim3(256),
+                               lds, e->stream, jobs, e->d_start, n, nj, ko);
     } else {
         if (resort)
             hipLaunchKernelGGL((k_tile_update_batched<T, NM, true, false, true>), dim3(grid), dim3(256), lds,
@@ -2007,6 +2019,51 @@ static int ens_build_spec(ss_ensemble* e) {
         }
         g->members.push_back(j);
         e->batched[j] = 1;
+    }
+    // Balance the side streams: a group with more cells than a stream's even
+    // share is split into chunks of consecutive members of about equal cells,
+    // and the groups go to the streams largest first, each to the stream with
+    // the fewest cells so far.
+    {
+        auto cells = [&](int j) { return (double)e->m[j]->nx * e->m[j]->ny * e->m[j]->nz; };
+        auto work = [&](const ss_ensemble::SpecGroup& g) {
+            double w = 0;
+            for (int j : g.members) w += cells(j);
+            return w;
+        };
+        double total = 0;
+        for (const auto& g : e->sgroups) total += work(g);
+        std::vector<ss_ensemble::SpecGroup> split;
+        for (const auto& g : e->sgroups) {
+            const double w = work(g);
+            const int k = std::max(1, std::min((int)g.members.size(), (int)std::lround(w * NSIDE / total)));
+            double acc = 0;
+            int c = 0;
+            split.emplace_back();
+            split.back().p = g.p;
+            for (int j : g.members) {
+                if (c + 1 < k && acc >= (c + 1) * w / k) {
+                    ++c;
+                    split.emplace_back();
+                    split.back().p = g.p;
+                }
+                split.back().members.push_back(j);
+                acc += cells(j);
+            }
+        }
+        std::vector<size_t> order(split.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](size_t a, size_t b) { return work(split[a]) > work(split[b]); });
+        double load[NSIDE] = {0};
+        for (size_t i : order) {
+            int q = 0;
+            for (int r = 1; r < NSIDE; ++r)
+                if (load[r] < load[q]) q = r;
+            split[i].side = q;
+            load[q] += work(split[i]);
+        }
+        e->sgroups.swap(split);
     }
     std::vector<SpecJob<T>> jobs;
     std::vector<int> first;
@@ -2122,7 +2179,7 @@ static int ens_step_once(ss_ensemble* e, int k, bool capturing) {
     SS_HIP_CHECK(hipEventRecord(e->fork, e->stream));
     for (int q = 0; q < NSIDE; ++q) SS_HIP_CHECK(hipStreamWaitEvent(e->side[q], e->fork, 0));
     for (size_t g = 0; g < e->sgroups.size(); ++g) {
-        if ((err = ens_launch_spec<T>(e, e->sgroups[g], e->side[g % NSIDE])) != hipSuccess)
+        if ((err = ens_launch_spec<T>(e, e->sgroups[g], e->side[e->sgroups[g].side])) != hipSuccess)
             return fail(SS_ERR_DEVICE, std::string("batched spectral stage: ") + hipGetErrorString(err));
     }
     for (size_t j = 0; j < e->m.size(); ++j) {

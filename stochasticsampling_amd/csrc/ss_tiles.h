@@ -260,7 +260,7 @@ constexpr int MAX_BATCH = 256;
 // Dynamic LDS: [lds_tile bytes for the tile body][njobs + 1 ints of start]; no
 // static LDS, so the 64-bit accumulators at the region's start stay 8-byte
 // aligned.
-template <typename T, int NM, bool MOVE, int NZ, bool SCATTER>
+template <typename T, int NM, bool MOVE, int NZ, bool SCATTER, bool FULL = true>
 __global__ void __launch_bounds__(256)
     k_tile_update_batched(const TileArgs<T, NM>* __restrict__ jobs, const int* __restrict__ start, int njobs,
                           int lds_tile, uint32_t step_off) {
@@ -286,8 +286,8 @@ __global__ void __launch_bounds__(256)
             long long* acc = reinterpret_cast<long long*>(smem_raw);
             long long* kf = acc + (size_t)a.tg.ws * (NM + 1);
             load_kf<NM>(a.kfix, a.g.na, kf);
-            tile_pass<T, NM, MOVE, NZ, false, SCATTER>(a, tile, a.toff[tile], a.toff[tile + 1], true, acc,
-                                                               kf);
+            tile_pass<T, NM, MOVE, NZ, false, SCATTER, FULL>(a, tile, a.toff[tile], a.toff[tile + 1], true, acc,
+                                                                     kf);
         }
     }
 }
