@@ -2358,6 +2358,11 @@ int ss_ensemble_create(const ss_params* params, int nsys, ss_ensemble** out) {
     }
     e->total_blocks = start.back();
     e->lds = (lds + 15) / 16 * 16;
+    // the persistent update keeps the member table beside the largest window
+    if (e->lds + sizeof(int) * (nsys + 1) > 65536)
+        return bail(fail(SS_ERR_UNSUPPORTED,
+                         "ensemble: the largest member tile window plus the member table exceed 64 KiB of LDS; "
+                         "use smaller tiles"));
     e->job_bytes = e->prec == SS_FP32 ? (e->nm == 6 ? sizeof(TileArgs<float, 6>) : e->nm == 9 ? sizeof(TileArgs<float, 9>) : sizeof(TileArgs<float, 12>))
                                       : (e->nm == 6 ? sizeof(TileArgs<double, 6>) : e->nm == 9 ? sizeof(TileArgs<double, 9>) : sizeof(TileArgs<double, 12>));
     if (hipMalloc(&e->d_jobs, e->job_bytes * 4 * nsys) != hipSuccess ||
