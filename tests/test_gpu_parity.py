@@ -422,14 +422,19 @@ def test_paired_spectral_stage_matches_unpaired(precision, tol, grid, extra):
     assert np.max(np.abs(pa[:, 4] - pb[:, 4])) < 1e3 * tol
 
 
+@pytest.mark.parametrize("default_physics", [False, True])
 @pytest.mark.parametrize("precision", [ss.FP32, ss.FP64])
-def test_ensemble_members_equal_single_systems(precision):
+def test_ensemble_members_equal_single_systems(precision, default_physics):
     """Members of a batched ensemble (different grids, boxes, particle counts,
     alpha and D_r) evolve bit for bit like the same systems stepped alone.
     The members' spectral stages run batched by axis-length class (16^3,
     32^3, 48^3 and 64^3 with compile-time sizes, the rest at run time; with
-    the strain and the density, with screening), except a member on the
-    paired stage, which keeps its own launches."""
+    the strain and the density, with screening; groups larger than a side
+    stream's share split into chunks), except a member on the paired stage,
+    which keeps its own launches.  With a member on the strain and the
+    density the persistent update runs the full Langevin chain; with every
+    member on the default physics, the variant with those terms compiled
+    out."""
     specs = [((16, 16, 16, 8, 8), 12.5, 5000, -2.0, 0.1, {}), ((12, 12, 12, 6, 6), 9.4, 3000, 0.5, 1.0, {}),
              ((8, 8, 8, 4, 4), 6.2, 700, 1.7, 0.5, {}), ((20, 16, 12, 8, 8), 15.6, 9000, -0.3, 0.5, {}),
              ((16, 16, 16, 8, 8), 12.5, 1, 1.0, 0.1, {}),
@@ -437,6 +442,8 @@ def test_ensemble_members_equal_single_systems(precision):
              ((48, 48, 48, 2, 2), 37.4, 30000, -1.5, 1.0, {}), ((64, 64, 64, 2, 2), 50.0, 30000, 1.2, 0.1, {}),
              ((64, 64, 64, 2, 2), 50.0, 10000, -0.7, 0.5, dict(hydro_screening=0.3)),
              ((16, 12, 10, 6, 6), 12.5, 4000, -1.0, 0.5, dict(flags=ss.FLAG_PAIRED_SPECTRAL))]
+    if default_physics:
+        specs[5] = ((32, 32, 32, 4, 4), 25.0, 20000, 0.8, 0.5, {})
     plist, parts = [], []
     for k, (grid, L, n, alpha, dr, extra) in enumerate(specs):
         p = ss.make_params(box=(L, L * grid[1] / grid[0], L * grid[2] / grid[0]), grid=grid, n=n, seed=100 + k,
