@@ -66,15 +66,17 @@ template <typename T, int NT>
 __device__ inline void spec_count_plane(const long long* __restrict__ mc, long long* __restrict__ ic, int np,
                                         double rho_scale, T* __restrict__ rho, T* __restrict__ rec, size_t base,
                                         int rs) {
+    if (!rho) {  // the counts are not needed: only clear the inflow
+        for (int c = threadIdx.x; c < np; c += NT) ic[c] = 0;
+        return;
+    }
     batched_loop<NT>(
         np, [&](int c) { return mc[c] + ic[c]; },
         [&](int c, long long v) {
             ic[c] = 0;
-            if (rho) {
-                const T r = (T)((double)v * rho_scale);
-                rho[base + c] = r;
-                rec[(base + c) * rs + REC_RHO] = r;
-            }
+            const T r = (T)((double)v * rho_scale);
+            rho[base + c] = r;
+            rec[(base + c) * rs + REC_RHO] = r;
         });
 }
 
