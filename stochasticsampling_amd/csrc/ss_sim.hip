@@ -1886,6 +1886,10 @@ int ss_initial_condition(const ss_params* params, double amplitude, uint64_t see
 // systems is one persistent launch over the concatenated tile ranges.
 // ===========================================================================
 static const int NSIDE = 4;
+// pencils per workgroup of a batched spectral stage (when the LDS allows):
+// 8 against the single system's 4 took the 48-system benchmark ensemble from
+// 2.51 to 2.47 ms per step; 16 made it 2.59 ms
+static const int ENS_PENCILS = 8;
 
 struct ss_ensemble {
     std::vector<ss_sim*> m;
@@ -2065,6 +2069,10 @@ static int ens_build_spec(ss_ensemble* e) {
         }
         e->sgroups.swap(split);
     }
+    if (e->sgroups.empty()) return SS_OK;
+    int lim = 0;
+    SS_HIP_CHECK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, e->m[0]->p.device));
+    const int cap = std::min(lim, 144 * 1024);
     std::vector<SpecJob<T>> jobs;
     std::vector<int> first;
     for (auto& g : e->sgroups) {
@@ -2102,9 +2110,14 @@ static int ens_build_spec(ss_ensemble* e) {
             a.np = s->ny * s->nz;
             a.nz = s->nz;
             a.rs = s->rs;
-            a.pb = s->pencils;
             a.nsig = s->nsig;
             a.nout = s->use_e ? 12 : 6;
+            // a batched launch has workgroups to spare: more pencils per
+            // workgroup make the x-line reads longer runs of each row
+            a.pb = s->pencils;
+            if (ENS_PENCILS > a.pb && a.np >= ENS_PENCILS &&
+                spec_pencil_lds(s->nx, std::max(a.nsig, a.nout) * ENS_PENCILS, sizeof(T)) <= (size_t)cap)
+                a.pb = ENS_PENCILS;
             jobs.push_back(a);
             // the block counts of launch_spec_planes_fwd / _pencils / _planes_inv
             f[0].push_back(f[0].back() + s->nx * (s->nm + 1));
@@ -2118,10 +2131,6 @@ static int ens_build_spec(ss_ensemble* e) {
             first.insert(first.end(), f[k].begin(), f[k].end());
         }
     }
-    if (e->sgroups.empty()) return SS_OK;
-    int lim = 0;
-    SS_HIP_CHECK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, e->m[0]->p.device));
-    const int cap = std::min(lim, 144 * 1024);
     for (auto& g : e->sgroups) {
         if (g.lds_plane > (size_t)cap || g.lds_pencil > (size_t)cap)
             return fail(SS_ERR_UNSUPPORTED, "batched spectral stage: LDS of a group exceeds the device limit");
