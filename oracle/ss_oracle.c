@@ -651,6 +651,72 @@ int o_mean_flow_field(const ss_params* p, double screening, const double* dist, 
     return 0;
 }
 
+/* SpectralSolver::solve_flow_field (spectral_solver.rs:85-131), the
+   deprecated second formulation of the same solve (no screening):
+   sigma_k_i = i (sum_j sigma^_ij k_j) / k^2, u^ = sigma_k - k (k . sigma_k) / k^2
+   with the unnormalised k mesh and 1/k^2 (k = 0 -> 0), backward transforms,
+   then u = Re / N.  Sums over a component axis run in order (ndarray's
+   sum_axis over an outer axis adds the sub-views one after another).
+   u: [3][Ns] real parts. */
+int o_solve_flow_field(const ss_params* p, const double* dist, double* u) {
+    osolver s;
+    solver_init(&s, p);
+    double* sig = malloc(sizeof(double) * 2 * 9 * s.ns);
+    double* uc = malloc(sizeof(double) * 2 * 3 * s.ns);
+    if (!sig || !uc) {
+        free(sig);
+        free(uc);
+        solver_free(&s);
+        return -1;
+    }
+    for (int q = 0; q < 9; ++q)
+        for (size_t c = 0; c < s.ns; ++c) {
+            sig[2 * ((size_t)q * s.ns + c)] =
+                unrolled_dot(s.skern + (size_t)q * s.na, dist + c * s.na, s.na) * s.measure;
+            sig[2 * ((size_t)q * s.ns + c) + 1] = 0.0;
+        }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int q = 0; q < 9; ++q) o_fft3d(s.nx, s.ny, s.nz, sig + 2 * (size_t)q * s.ns, -1);
+#pragma omp parallel for schedule(static)
+    for (long long idx = 0; idx < (long long)s.ns; ++idx) {
+        double k[3];
+        kvec(&s, (size_t)idx, k);
+        double ik2 = s.ik2[idx];
+        double skr[3], ski[3];
+        for (int i = 0; i < 3; ++i) {
+            double ar = 0.0, ai = 0.0;
+            for (int j = 0; j < 3; ++j) {
+                size_t off = 2 * ((size_t)(3 * i + j) * s.ns + idx);
+                ar = ar + sig[off] * k[j];
+                ai = ai + sig[off + 1] * k[j];
+            }
+            ar = ar * ik2;
+            ai = ai * ik2;
+            skr[i] = -ai; /* times i */
+            ski[i] = ar;
+        }
+        double kr = 0.0, ki = 0.0;
+        for (int i = 0; i < 3; ++i) {
+            kr = kr + k[i] * skr[i];
+            ki = ki + k[i] * ski[i];
+        }
+        kr = kr * ik2;
+        ki = ki * ik2;
+        for (int i = 0; i < 3; ++i) {
+            uc[2 * ((size_t)i * s.ns + idx)] = skr[i] - k[i] * kr;
+            uc[2 * ((size_t)i * s.ns + idx) + 1] = ski[i] - k[i] * ki;
+        }
+    }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int c = 0; c < 3; ++c) o_fft3d(s.nx, s.ny, s.nz, uc + 2 * (size_t)c * s.ns, +1);
+    double norm = (double)s.ns;
+    for (size_t i = 0; i < 3 * s.ns; ++i) u[i] = uc[2 * i] / norm;
+    free(sig);
+    free(uc);
+    solver_free(&s);
+    return 0;
+}
+
 int o_mean_magnetic_field(const ss_params* p, const double* dist, double* b, double* grad_b) {
     osolver s;
     solver_init(&s, p);

@@ -59,6 +59,9 @@ void o_sample_from(const ss_params* p, const uint32_t* counts, uint64_t n, doubl
 int o_mean_flow_field(const ss_params* p, double screening, const double* dist, double* u,
                       double* grad_u);
 int o_mean_magnetic_field(const ss_params* p, const double* dist, double* b, double* grad_b);
+/* SpectralSolver::solve_flow_field (deprecated formulation, screening 0):
+   u [3][Ns] real parts. */
+int o_solve_flow_field(const ss_params* p, const double* dist, double* u);
 void o_update_one(const ss_params* p, const double part[5], const double fields[34],
                   const double rv[5], double out[5]);
 void o_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

@@ -1127,9 +1127,12 @@ static int create_impl(ss_sim* s) {
     s->owned.push_back(s->kern);
     if (s->sorted) {
         // Fixed-point moment kernels: the largest possible cell sum,
-        // N * max|K|, stays below 2^62.
-        double kmax = 1.0;
+        // N * max|K|, stays below 2^62.  The scale follows the largest kernel
+        // value itself (small stress coefficients keep their relative
+        // resolution); only all-zero kernels fall back to max|K| = 1.
+        double kmax = 0.0;
         for (double v : kern) kmax = std::max(kmax, std::fabs(v));
+        if (!(kmax > 0.0)) kmax = 1.0;
         const int shift = (int)std::floor(62.0 - std::log2((double)s->n * kmax));
         s->fixed_scale = std::ldexp(1.0, -shift);
         std::vector<long long> kf(kern.size());

@@ -101,6 +101,11 @@ class Params(C.Structure):
         g = self.grid_size
         return (g.x, g.y, g.z, g.phi, g.theta)
 
+    @property
+    def box(self):
+        b = self.box_size
+        return (b.x, b.y, b.z)
+
 
 class Info(C.Structure):
     """ss_info: the execution plan chosen at ss_create."""

@@ -36,10 +36,12 @@ class OMcg128(C.Structure):
 
 
 class Oracle:
-    def __init__(self):
-        if not os.path.exists(ORACLE_SO):
-            build_oracle()
-        L = C.CDLL(ORACLE_SO)
+    def __init__(self, so_path=None):
+        if so_path is None:
+            so_path = ORACLE_SO
+            if not os.path.exists(so_path):
+                build_oracle()
+        L = C.CDLL(so_path)
         self.L = L
         pp = _P(Params)
         L.osim_create.argtypes = [pp]
@@ -75,6 +77,7 @@ class Oracle:
         L.o_sample_from.argtypes = [pp, _P(C.c_uint32), C.c_uint64, _d]
         L.o_mean_flow_field.argtypes = [pp, C.c_double, _d, _d, _d]
         L.o_mean_magnetic_field.argtypes = [pp, _d, _d, _d]
+        L.o_solve_flow_field.argtypes = [pp, _d, _d]
         L.o_update_one.argtypes = [pp, _d, _d, _d, _d]
         L.o_philox4x32_10.argtypes = [_P(C.c_uint32), _P(C.c_uint32), _P(C.c_uint32)]
         L.o_noise.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, _d]
@@ -204,6 +207,13 @@ class Oracle:
         rc = self.L.o_mean_flow_field(C.byref(params), screening, _dp(d), _dp(u), _dp(g))
         assert rc == 0
         return u.reshape((3,) + params.grid_shape), g.reshape((3, 3) + params.grid_shape)
+
+    def solve_flow_field(self, params, dist):
+        """The deprecated formulation (spectral_solver.rs:85-131): real u."""
+        d = np.ascontiguousarray(dist, dtype=np.float64)
+        u = np.zeros(3 * params.ns)
+        assert self.L.o_solve_flow_field(C.byref(params), _dp(d), _dp(u)) == 0
+        return u.reshape((3,) + params.grid_shape)
 
     def mean_magnetic_field(self, params, dist):
         d = np.ascontiguousarray(dist, dtype=np.float64)

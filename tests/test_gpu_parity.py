@@ -496,14 +496,18 @@ def test_fp32_fields_within_target_of_fp64(oracle):
     osim = oracle.simulation(p64)
     osim.init(parts)
     osim.step(1)
-    # same counts (positions rounded to FP32 stay in their bins for this IC)
+    # the counts of the FP32 binning: every particle binned, and the same
+    # bins as the FP64 oracle except where an FP32-rounded position crossed
+    # a cell edge (a handful of the 2^15 particles)
     cnt = sim.get_counts()
-    np.testing.assert_array_equal(cnt.sum(), p32.number_of_particles)
+    assert int(cnt.sum()) == p32.number_of_particles
+    moved = int(np.abs(cnt.astype(np.int64) - osim.counts().astype(np.int64)).sum())
+    assert moved <= 20, moved
+    # the FP32 fields against the FP64 oracle solve of the same distribution
     u, g = sim.get_flow_field(with_gradient=True)
-    ue, ge = osim.flow_field()
-    if np.array_equal(cnt, osim.counts()):
-        assert rel_err(u, ue) < FP32_FIELD_TOL
-        assert rel_err(g, ge) < FP32_FIELD_TOL
+    ue, ge = oracle.mean_flow_field(p64, 0.0, sim.get_distribution())
+    assert rel_err(u, ue) < FP32_FIELD_TOL
+    assert rel_err(g, ge) < FP32_FIELD_TOL
     a, b = sim.get_particles(), osim.particles()
     assert np.median(periodic_diff(a[:, :3], b[:, :3], 10.0)) < 1e-6
     assert np.percentile(periodic_diff(a[:, :3], b[:, :3], 10.0), 99) < 1e-5
@@ -524,11 +528,12 @@ def test_fp32_dense_dft_fields_against_fp64(oracle):
     osim = oracle.simulation(p64)
     osim.init(parts)
     osim.step(1)
+    moved = int(np.abs(sim.get_counts().astype(np.int64) - osim.counts().astype(np.int64)).sum())
+    assert moved <= 20, moved
     u, g = sim.get_flow_field(with_gradient=True)
-    ue, ge = osim.flow_field()
-    if np.array_equal(sim.get_counts(), osim.counts()):
-        assert rel_err(u, ue) < 2e-4
-        assert rel_err(g, ge) < 2e-4
+    ue, ge = oracle.mean_flow_field(p64, 0.0, sim.get_distribution())
+    assert rel_err(u, ue) < 2e-4
+    assert rel_err(g, ge) < 2e-4
 
 
 def test_snapshot_resume_is_exact():
