@@ -155,7 +155,10 @@ def test_full_size_fp64_paths_agree():
     (c0, u0, x0), (c1, u1, x1) = res
     np.testing.assert_array_equal(c0, c1)
     assert int(c0.sum(dtype=np.uint64)) == n
-    assert rel_err(u1, u0) < 1e-12
+    # the tiled path sums kernel values rounded to multiples of 2^-37 (the
+    # fixed-point scale that keeps 2^24 particle values below 2^62): about
+    # 4e-12 of u against the FP64 contraction of the counts, inside the FP64 bar
+    assert rel_err(u1, u0) < FP64_TOL
     d, _, dth = _state_diffs(x0, x1, BIG_BOX)
     assert d.max() < 1e-11 and dth.max() < 1e-11
     assert abs(u0.mean()) < 1e-12 * np.abs(u0).max()
