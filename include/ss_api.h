@@ -330,6 +330,18 @@ int ss_fft_execute(ss_fft_plan* plan, void* device_data, void* hip_stream);
 /* Same on host memory: copies in, executes, copies out. */
 int ss_fft_execute_host(ss_fft_plan* plan, void* host_data);
 
+/* Largest plan batch (nx*ny*nz*batch complex values, in bytes) that
+   ss_fft_execute_host_noalloc stages: 64 MiB. */
+#define SS_FFT_NOALLOC_MAX_BYTES (64u << 20)
+
+/* No counterpart in the reference.  ss_fft_execute_host without a device
+   allocation, for callers that cannot allocate (tight loops, harnesses that
+   forbid hipMalloc): the plan's batch is staged in a buffer the library holds
+   on each GPU until the process ends (SS_FFT_NOALLOC_MAX_BYTES; a larger
+   batch returns SS_ERR_UNSUPPORTED).  All calls on one GPU share that buffer:
+   calls that overlap on one GPU (other host threads) race. */
+int ss_fft_execute_host_noalloc(ss_fft_plan* plan, void* host_data);
+
 /* Drop for FFTPlan (fft.rs:235-244). */
 int ss_fft_plan_destroy(ss_fft_plan* plan);
 

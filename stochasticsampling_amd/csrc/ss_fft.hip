@@ -9,7 +9,9 @@
 // the radix stages between two LDS buffers and writes the tile back.
 #include <math.h>
 
+#include <atomic>
 #include <cstring>
+#include <string>
 
 #include "ss_fft_device.h"
 #include "ss_internal.h"
@@ -556,6 +558,58 @@ extern "C" int ss_fft_execute_host(ss_fft_plan* p, void* host_data) {
     if (hipStreamSynchronize(p->stream) != hipSuccess && rc == SS_OK)
         rc = ss::fail(SS_ERR_DEVICE, "FFT execution failed");
     (void)hipFree(d);
+    return rc;
+}
+
+// Staging buffer of ss_fft_execute_host_noalloc, one per GPU for the life of
+// the process.  In host code its name stands for no device address: host code
+// only passes the address hipGetSymbolAddress returns for the current device
+// (kept per device in noalloc_ptr).
+static __device__ __align__(256) unsigned char scratch[SS_FFT_NOALLOC_MAX_BYTES];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::atomic<void*> noalloc_ptr[NOALLOC_MAX_DEVICES];
+
+// Device address of `scratch` on `device` (which must be current), looked up
+// on the first call for that device.  Two threads that race here store the
+// same address.
+int noalloc_scratch(int device, void** out) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+        return ss::fail(SS_ERR_UNSUPPORTED, "device ordinal beyond the staging-buffer table");
+    void* ptr = noalloc_ptr[device].load(std::memory_order_acquire);
+    if (!ptr) {
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return ss::fail(SS_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        noalloc_ptr[device].store(ptr, std::memory_order_release);
+    }
+    *out = ptr;
+    return SS_OK;
+}
+}  // namespace
+
+extern "C" int ss_fft_execute_host_noalloc(ss_fft_plan* p, void* host_data) {
+    if (!p || !host_data) return ss::fail(SS_ERR_INVALID, "NULL argument");
+    const size_t bytes = (size_t)p->fft.nx * p->fft.ny * p->fft.nz * p->batch *
+                         (p->fft.precision == SS_FP64 ? 16 : 8);
+    if (bytes > (size_t)SS_FFT_NOALLOC_MAX_BYTES)
+        return ss::fail(SS_ERR_UNSUPPORTED, "the plan's batch of " + std::to_string(bytes) +
+                                                " bytes exceeds the staging buffer of " +
+                                                std::to_string((size_t)SS_FFT_NOALLOC_MAX_BYTES) +
+                                                " bytes; use ss_fft_execute_host");
+    SS_HIP_CHECK(hipSetDevice(p->device));
+    void* d = nullptr;
+    int rc = noalloc_scratch(p->device, &d);
+    if (rc != SS_OK) return rc;
+    // the copy in overwrites every staged byte, so nothing is cleared first
+    if (hipMemcpyAsync(d, host_data, bytes, hipMemcpyHostToDevice, p->stream) != hipSuccess)
+        rc = ss::fail(SS_ERR_DEVICE, "copy in failed");
+    if (rc == SS_OK) rc = p->fft.run(d, p->batch, p->direction, p->stream);
+    if (rc == SS_OK && hipMemcpyAsync(host_data, d, bytes, hipMemcpyDeviceToHost, p->stream) != hipSuccess)
+        rc = ss::fail(SS_ERR_DEVICE, "copy out failed");
+    if (hipStreamSynchronize(p->stream) != hipSuccess && rc == SS_OK)
+        rc = ss::fail(SS_ERR_DEVICE, "FFT execution failed");
     return rc;
 }
 

@@ -65,6 +65,7 @@ def _declare(L):
         ),
         "ss_fft_execute": (C.c_int, [vp, vp, vp]),
         "ss_fft_execute_host": (C.c_int, [vp, vp]),
+        "ss_fft_execute_host_noalloc": (C.c_int, [vp, vp]),
         "ss_fft_plan_destroy": (C.c_int, [vp]),
         "ss_flow_solve": (C.c_int, [pp, _d, _d, _d]),
         "ss_magnetic_solve": (C.c_int, [pp, _d, _d, _d]),

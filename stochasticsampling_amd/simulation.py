@@ -378,6 +378,15 @@ class FFTPlan:
         _l.check(self.L.ss_fft_execute_host(self._h, a.ctypes.data_as(C.c_void_p)))
         return a
 
+    def reexecute3d_noalloc(self, a):
+        """Same transform through ss_fft_execute_host_noalloc: staged in the
+        library's per-GPU buffer instead of a device allocation per call."""
+        dt = np.complex128 if self.precision == 0 else np.complex64
+        if a.dtype != dt or not a.flags.c_contiguous:
+            raise TypeError(f"array must be C-contiguous {np.dtype(dt).name}")
+        _l.check(self.L.ss_fft_execute_host_noalloc(self._h, a.ctypes.data_as(C.c_void_p)))
+        return a
+
     def close(self):
         if getattr(self, "_h", None):
             self.L.ss_fft_plan_destroy(self._h)
