@@ -49,6 +49,10 @@ def make(prec, n, binning=None, **extra):
 
 
 def time_gpu(prec, parts, steps, warmup, profile_steps, tune=False, **extra):
+    """Times exactly `steps` steps after `warmup` untimed ones.  On the tiled
+    path the timed window starts with a re-sorting update (ss_resort), so it
+    holds ceil(steps / R) re-sorts for a re-sort interval R: never fewer than
+    its share of a whole period."""
     import stochasticsampling_amd as ss
     from stochasticsampling_amd import autotune
 
@@ -59,6 +63,7 @@ def time_gpu(prec, parts, steps, warmup, profile_steps, tune=False, **extra):
     sim.init(parts)
     if warmup:
         sim.step(warmup)
+    sim.resort()
     sim.synchronize()
     t0 = time.perf_counter()
     sim.step(steps)
@@ -162,6 +167,98 @@ def fft_variant_study(parts, steps):
     return out
 
 
+def host_cores():
+    """Cores this process may use: the CPU affinity mask, capped by
+    OMP_NUM_THREADS when the environment sets it (the share of the host's
+    cores given to one GPU)."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return (min(aff, omp) if omp > 0 else aff), {"nproc": os.cpu_count(), "affinity": aff,
+                                                   "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
+def small_case(steps, threads):
+    """The CPU-runnable case, FP64: L = 10, 16^3 x 8 x 8, 2^16 particles,
+    dt = 1e-3, D_t = 0.1, D_r = 0.5, alpha = -1, seed 1; the GPU (plain FP64
+    path) and the CPU oracle (reference-faithful step structure, `threads`
+    OpenMP threads) each timed over all `steps` steps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle
+
+    import stochasticsampling_amd as ss
+
+    n = 1 << 16
+    p = ss.make_params(box=(10.0, 10.0, 10.0), grid=(16, 16, 16, 8, 8), n=n, timestep=1e-3, seed=1,
+                       diffusion_translational=0.1, diffusion_rotational=0.5, stress_active=-1.0)
+    parts = ss.initial_condition(p, amplitude=0.05, seed=1)
+    sim = ss.Simulation(p)
+    sim.init(parts)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.synchronize()
+    gpu_s = time.perf_counter() - t0
+    sim.close()
+    o = Oracle()
+    o.set_threads(threads)
+    po = ss.make_params(box=(10.0, 10.0, 10.0), grid=(16, 16, 16, 8, 8), n=n, timestep=1e-3, seed=1,
+                        diffusion_translational=0.1, diffusion_rotational=0.5, stress_active=-1.0,
+                        flags=ss.FLAG_MAGNETIC_ALWAYS)
+    osim = o.simulation(po)
+    osim.init(parts)
+    t0 = time.perf_counter()
+    osim.step(steps)
+    cpu_s = time.perf_counter() - t0
+    return {"workload": "FP64, L=10, 16^3x8x8, 2^16 particles, dt=1e-3, D_t=0.1, D_r=0.5, alpha=-1, seed 1",
+            "steps": steps, "gpu": {"value": n * steps / gpu_s, "ms_per_step": gpu_s * 1e3 / steps,
+                                    "path": "FP64 plain (histogram binning)"},
+            "cpu": {"value": n * steps / cpu_s, "ms_per_step": cpu_s * 1e3 / steps, "cores": threads,
+                    "kind": "port"},
+            "gpu_over_cpu": cpu_s / gpu_s, "unit": "particle-steps/s"}
+
+
+def magnetic_case(parts, steps, warmup, profile_steps):
+    """The magnetic suspension at full size: the 2^24-particle, 64^3 x 16 x 16
+    workload in FP32 plus an external field of strength 2 along z acting as
+    a torque, the dipole-dipole mean field (mu_dd = 0.5) and its force (drag
+    0.1), Philox noise from seed 3; FP64 global diagnostics of the final
+    state."""
+    import stochasticsampling_amd as ss
+    from stochasticsampling_amd import autotune
+
+    n = parts.shape[0]
+    kw = dict(magnetic_reorientation=2.0, field_direction=(0.0, 0.0, 1.0), magnetic_dipole_dipole=0.5,
+              magnetic_drag=0.1)
+    p = autotune.tuned(make(ss.FP32, n, **kw))
+    p.seed = 3
+    sim = ss.Simulation(p)
+    sim.init(parts)
+    sim.step(warmup)
+    sim.resort()
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    stages = sim.profile_steps(profile_steps)
+    stages["plan"] = sim.get_info()
+    diag = sim.get_diagnostics()
+    sim.close()
+    upd = stages.get("update")
+    return {"value": n / (ms * 1e-3), "unit": "particle-steps/s", "ms_per_step": ms, "steps": steps,
+            "dtype": "f32", "stages_ms": stages,
+            "workload": "2^24 particles, 64^3x16x16, L=50, FP32, field strength 2 along z (torque), "
+                        "mu_dd=0.5, drag=0.1, Philox seed 3",
+            "roofline": {"bound": "hbm", "achieved": UPDATE_BYTES["f32"] * n / (upd * 1e-3) / 1e9,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": UPDATE_BYTES["f32"] * n / (upd * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "bytes_per_particle": UPDATE_BYTES["f32"], "launch_ms": upd},
+            "mean_polarisation": [float(v) for v in diag["mean_orientation"]]}
+
+
 def cpu_baseline(parts, threads, steps):
     """The FP64 CPU oracle in reference-faithful mode (serial binning and stress
     contraction, magnetic solve every step, 36 c2c FFTs, parallel update) on
@@ -229,8 +326,18 @@ def time_ensemble(steps, warmup):
     ens.step(steps)
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / steps
+    # per-stage HIP events over eager steps: the persistent update launch
+    # moves 64 algorithmic bytes per particle of every member
+    stages = ens.profile_steps(32)
     ens.close()
+    upd = stages.get("update")
+    achieved = UPDATE_BYTES["f32"] * total / (upd * 1e-3) / 1e9 if upd else None
     return {"value": total / (ms * 1e-3), "unit": "particle-steps/s", "ms_per_step": ms, "steps": steps,
+            "stages_ms": stages,
+            "roofline": {"kernel": "ss::k_tile_update_batched", "bound": "hbm", "achieved": achieved,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS if achieved else None,
+                         "bytes_per_particle": UPDATE_BYTES["f32"], "particles": total, "launch_ms": upd},
             "systems": len(plist), "particles": total, "dtype": "f32",
             "workload": "48 systems, alpha in [-2,2], D_r in {0.1,0.5,1}, grids 32^3x8x8/48^3x12x12/"
                         "64^3x16x16, L = 0.78 x grid points, N log-uniform in [2^18, 2^22], seeds 100-147, "
@@ -265,7 +372,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--particles", type=int, default=1 << 24)
     ap.add_argument("--profile-steps", type=int, default=50)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -278,6 +385,10 @@ def main():
     ap.add_argument("--no-refnoise", action="store_true")
     ap.add_argument("--refnoise-steps", type=int, default=200)
     ap.add_argument("--refnoise-streams", type=int, default=64)
+    ap.add_argument("--no-small", action="store_true")
+    ap.add_argument("--small-steps", type=int, default=100)
+    ap.add_argument("--no-magnetic", action="store_true")
+    ap.add_argument("--magnetic-steps", type=int, default=100)
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -365,15 +476,28 @@ def main():
         "launch_ms": upd_ms,
     }
 
+    mag = None
+    if not args.no_magnetic:
+        print("[bench] FP32 magnetic suspension", file=sys.stderr, flush=True)
+        mag = magnetic_case(parts, args.magnetic_steps, min(args.warmup, 10), min(args.profile_steps, 20))
+
+    cores, core_info = host_cores()
+    threads = args.cpu_threads or cores
+    small = None
+    if not args.no_small:
+        print(f"[bench] small case, GPU and CPU oracle ({threads} threads), {args.small_steps} steps",
+              file=sys.stderr, flush=True)
+        small = small_case(args.small_steps, threads)
+
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         print(f"[bench] CPU oracle baseline on {threads} threads", file=sys.stderr, flush=True)
         v, per = cpu_baseline(parts, threads, args.cpu_steps)
         cpu = {"value": v, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+               "host": core_info,
                "sample": f"FP64 oracle, reference-faithful step structure, full workload "
                          f"(2^24 particles, 64^3x16x16), 1 warm-up + {args.cpu_steps} timed steps, "
-                         f"{per:.2f} s/step"}
+                         f"{per:.2f} s/step, {threads} OpenMP threads (all cores of this process's share)"}
 
     value = n / (ms32 * 1e-3)
     line = {
@@ -393,7 +517,8 @@ def main():
             "workload": "2^24 particles, grid 64^3 x 16 x 16, L=50, dt=1e-3, D_t=0.1, D_r=0.5, "
                         "alpha=-1, FP32 states with compensated positions, FP32 FFT, Philox noise "
                         "seed 2, particles grouped by tiles (tile shape and re-sort interval from "
-                        "the autotuner's table, see stages_ms.plan)",
+                        "the autotuner's table, see stages_ms.plan); the timed steps start with a "
+                        "re-sorting update",
             "particles": n,
             "grid": [64, 64, 64, 16, 16],
             "box": [50.0, 50.0, 50.0],
@@ -405,6 +530,8 @@ def main():
         "fft_variants": fftv,
         "fp32_vs_fp64": serr,
         "reference_noise": refn,
+        "small_case": small,
+        "magnetic": mag,
         "stages_ms": st32,
     }
     if fp64:

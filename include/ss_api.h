@@ -220,6 +220,14 @@ int ss_get_noise(ss_sim* sim, double* rv, uint64_t n);
 int ss_profile_steps(ss_sim* sim, uint64_t nsteps, double* ms_per_step, char* names,
                      int name_stride, int max_stages);
 
+/* Starts a new re-sort period: the next step's update writes the particles
+   in tile order (offsets from the counts of the last binning), as every
+   resort_interval-th step does.  No effect on the plain path or when a
+   re-sort is already due; results do not depend on it (the tiled binning is
+   exact).  No reference counterpart (the reference keeps particles unsorted).
+   SS_ERR_STATE for a member of an ensemble. */
+int ss_resort(ss_sim* sim);
+
 /* Global diagnostics of the current particles, reduced in FP64 (exact
    64-bit fixed-point sums, independent of the particle order), 15 doubles:
    out[0..2]  mean orientation vector n over the particles;
@@ -267,14 +275,23 @@ typedef struct ss_ensemble ss_ensemble;
 int ss_ensemble_create(const ss_params* params, int nsys, ss_ensemble** out);
 int ss_ensemble_destroy(ss_ensemble* ens);
 
-/* Borrowed handle of member `sys` for ss_init / ss_set_noise / getters (do
-   not ss_step or ss_destroy it).  All members must be initialised together
+/* Borrowed handle of member `sys` for ss_init / ss_set_noise / getters
+   (ss_step, ss_profile_steps and ss_resort return SS_ERR_STATE on it; do not
+   ss_destroy it).  All members must be initialised together
    before ss_ensemble_step. */
 int ss_ensemble_member(ss_ensemble* ens, int sys, ss_sim** out);
 
 /* nsteps steps of every member; timestep_out (may be NULL) receives the
    members' common step counter. */
 int ss_ensemble_step(ss_ensemble* ens, uint64_t nsteps, uint64_t* timestep_out);
+
+/* nsteps eager ensemble steps timed with HIP events on the ensemble's
+   stream, like ss_profile_steps: stages "spectral" (all members' spectral
+   stages, from the fork to the join of the side streams), "update" (the
+   persistent update launch over every member's tiles) and "sort" (the
+   offsets of a due re-sort).  Returns the number of stages written. */
+int ss_ensemble_profile_steps(ss_ensemble* ens, uint64_t nsteps, double* ms_per_step, char* names,
+                              int name_stride, int max_stages);
 
 /* ---------------------------------------------------------------------------
  * Initial conditions (no reference counterpart for the perturbed state; the

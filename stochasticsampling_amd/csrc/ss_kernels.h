@@ -16,19 +16,33 @@ struct Geo {
     int ns, na;
     T gx, gy, gz, gphi, gtheta;  // GridWidth (grid_width.rs:23-31)
     T box[3];
+    // per axis (x, y, z, phi, theta): fl(1 / width) and the tolerance of
+    // grid_index_r (4 u (n + 1))
+    T rg[5], tol[5];
 };
+
+// Distribution::coord_to_grid per axis (distribution/mod.rs:112-137).
+template <typename T>
+__device__ inline int bin_x(const Geo<T>& g, T q) { return grid_index_r(q, g.gx, g.rg[0], g.tol[0], g.nx); }
+template <typename T>
+__device__ inline int bin_y(const Geo<T>& g, T q) { return grid_index_r(q, g.gy, g.rg[1], g.tol[1], g.ny); }
+template <typename T>
+__device__ inline int bin_z(const Geo<T>& g, T q) { return grid_index_r(q, g.gz, g.rg[2], g.tol[2], g.nz); }
+template <typename T>
+__device__ inline int bin_phi(const Geo<T>& g, T q) { return grid_index_r(q, g.gphi, g.rg[3], g.tol[3], g.np); }
+template <typename T>
+__device__ inline int bin_theta(const Geo<T>& g, T q) {
+    return grid_index_r(q, g.gtheta, g.rg[4], g.tol[4], g.nt);
+}
 
 template <typename T>
 __device__ inline int cell_of(const Geo<T>& g, T x, T y, T z) {
-    int ix = grid_index(x, g.gx, g.nx);
-    int iy = grid_index(y, g.gy, g.ny);
-    int iz = grid_index(z, g.gz, g.nz);
-    return (ix * g.ny + iy) * g.nz + iz;
+    return (bin_x(g, x) * g.ny + bin_y(g, y)) * g.nz + bin_z(g, z);
 }
 
 template <typename T>
 __device__ inline int angle_of(const Geo<T>& g, T phi, T theta) {
-    return grid_index(phi, g.gphi, g.np) * g.nt + grid_index(theta, g.gtheta, g.nt);
+    return bin_phi(g, phi) * g.nt + bin_theta(g, theta);
 }
 
 // Wave64 run detection on a key that is mostly constant across neighbouring
@@ -331,6 +345,19 @@ __device__ inline float add_wrap<float>(float x, float d, float L, float* lo) {
     return t;
 }
 
+// Element i of a particle array.  A 32-bit slot (the tiled path, whose host
+// side keeps n * 8 bytes below 2^32) addresses through a 32-bit byte offset,
+// so that loads and stores take the array's scalar base plus one vector offset
+// instead of a 64-bit vector address per array.
+template <typename T>
+__device__ inline T& at(T* base, uint64_t i) {
+    return base[i];
+}
+template <typename T>
+__device__ inline T& at(T* base, uint32_t i) {
+    return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (uint32_t)sizeof(T));
+}
+
 // One particle's state as loaded from the arrays (the low-order position
 // parts only on the FP32 path).
 template <typename T>
@@ -339,17 +366,17 @@ struct Loaded {
     float lx, ly, lz;
 };
 
-template <typename T>
-__device__ inline void load_particle(const State<T>& s, uint64_t i, Loaded<T>& q) {
-    q.x = s.x[i];
-    q.y = s.y[i];
-    q.z = s.z[i];
-    q.phi = s.phi[i];
-    q.theta = s.theta[i];
+template <typename T, typename I>
+__device__ inline void load_particle(const State<T>& s, I i, Loaded<T>& q) {
+    q.x = at(s.x, i);
+    q.y = at(s.y, i);
+    q.z = at(s.z, i);
+    q.phi = at(s.phi, i);
+    q.theta = at(s.theta, i);
     if constexpr (sizeof(T) == 4) {
-        q.lx = s.lx[i];
-        q.ly = s.ly[i];
-        q.lz = s.lz[i];
+        q.lx = at(s.lx, i);
+        q.ly = at(s.ly, i);
+        q.lz = at(s.lz, i);
     } else {
         q.lx = q.ly = q.lz = 0.0f;
     }
@@ -357,8 +384,8 @@ __device__ inline void load_particle(const State<T>& s, uint64_t i, Loaded<T>& q
 
 // Update of one loaded particle with its cell record (global memory or
 // registers): run the LangevinBuilder chain, move, wrap, store at slot i.
-template <typename T, bool FULL = true>
-__device__ inline void move_loaded(const State<T>& s, uint64_t i, const Loaded<T>& q, const UpdateParams<T>& P,
+template <typename T, bool FULL = true, typename I>
+__device__ inline void move_loaded(const State<T>& s, I i, const Loaded<T>& q, const UpdateParams<T>& P,
                                    const T* rec, const Noise<T>& rv, T& nx, T& ny, T& nz, T& ophi, T& otheta) {
     T dp[3];
     langevin<T, FULL>(P, q.phi, q.theta, rec, rv, dp, ophi, otheta);
@@ -371,20 +398,20 @@ __device__ inline void move_loaded(const State<T>& s, uint64_t i, const Loaded<T
             nz = T(0);
             lz = 0.0f;
         }
-        s.lx[i] = lx;
-        s.ly[i] = ly;
-        s.lz[i] = lz;
+        at(s.lx, i) = lx;
+        at(s.ly, i) = ly;
+        at(s.lz, i) = lz;
     } else {
         nx = add_wrap<T>(q.x, dp[0], P.box[0], nullptr);
         ny = add_wrap<T>(q.y, dp[1], P.box[1], nullptr);
         nz = add_wrap<T>(q.z, dp[2], P.box[2], nullptr);
         if (FULL && P.quasi2d) nz = T(0);
     }
-    s.x[i] = nx;
-    s.y[i] = ny;
-    s.z[i] = nz;
-    s.phi[i] = ophi;
-    s.theta[i] = otheta;
+    at(s.x, i) = nx;
+    at(s.y, i) = ny;
+    at(s.z, i) = nz;
+    at(s.phi, i) = ophi;
+    at(s.theta, i) = otheta;
 }
 
 // FP32 cell record into registers with 16-byte loads: the first 8 values
@@ -789,11 +816,11 @@ __global__ void k_coord_to_grid(const double* __restrict__ aos, uint64_t n, Geo<
     if (i >= n) return;
     const T* dummy = nullptr;
     (void)dummy;
-    out[5 * i + 0] = grid_index((T)aos[5 * i + 0], g.gx, g.nx);
-    out[5 * i + 1] = grid_index((T)aos[5 * i + 1], g.gy, g.ny);
-    out[5 * i + 2] = grid_index((T)aos[5 * i + 2], g.gz, g.nz);
-    out[5 * i + 3] = grid_index((T)aos[5 * i + 3], g.gphi, g.np);
-    out[5 * i + 4] = grid_index((T)aos[5 * i + 4], g.gtheta, g.nt);
+    out[5 * i + 0] = bin_x(g, (T)aos[5 * i + 0]);
+    out[5 * i + 1] = bin_y(g, (T)aos[5 * i + 1]);
+    out[5 * i + 2] = bin_z(g, (T)aos[5 * i + 2]);
+    out[5 * i + 3] = bin_phi(g, (T)aos[5 * i + 3]);
+    out[5 * i + 4] = bin_theta(g, (T)aos[5 * i + 4]);
 }
 
 // Debug update of n independent particles with explicit per-particle fields

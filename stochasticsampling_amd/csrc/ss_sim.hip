@@ -188,6 +188,7 @@ struct ss_sim {
     // current state, so that the noise records follow the storage order
     uint32_t* inv = nullptr;
     bool last_resort = false;  // the last step's update changed the storage order
+    bool in_ensemble = false;  // borrowed member of an ensemble: stepped only by it
     // tiled path
     TileGeo tg;
     int resort = 16;             // steps between tile sorts
@@ -248,6 +249,13 @@ static Geo<T> geo(const ss_sim* s) {
     g.box[0] = (T)s->p.box_size.x;
     g.box[1] = (T)s->p.box_size.y;
     g.box[2] = (T)s->p.box_size.z;
+    const T w[5] = {g.gx, g.gy, g.gz, g.gphi, g.gtheta};
+    const int n[5] = {g.nx, g.ny, g.nz, g.np, g.nt};
+    const double u = sizeof(T) == 4 ? 0x1p-24 : 0x1p-53;
+    for (int a = 0; a < 5; ++a) {
+        g.rg[a] = T(1) / w[a];  // correctly rounded in T on the host
+        g.tol[a] = (T)(4.0 * u * ((double)n[a] + 1.0));
+    }
     return g;
 }
 
@@ -1270,6 +1278,14 @@ int ss_create(const ss_params* params, ss_sim** out) {
     s->esz = s->prec == SS_FP64 ? 8 : 4;
     s->sorted = p.binning == SS_BIN_SORTED || (p.binning == SS_BIN_AUTO && p.precision == SS_FP32);
     s->n = p.number_of_particles;
+    // the tiled update addresses particle arrays through 32-bit byte offsets
+    if (s->sorted && s->n > ((uint64_t)1 << 29)) {
+        if (p.binning == SS_BIN_SORTED) {
+            delete s;
+            return fail(SS_ERR_UNSUPPORTED, "the tiled path holds at most 2^29 particles");
+        }
+        s->sorted = false;
+    }
     s->nx = p.grid_size.x;
     s->ny = p.grid_size.y;
     s->nz = p.grid_size.z;
@@ -1365,6 +1381,7 @@ int ss_set_noise(ss_sim* s, const double* rv, uint64_t n) {
 int ss_step(ss_sim* s, uint64_t nsteps, uint64_t* timestep_out) {
     if (!s) return fail(SS_ERR_INVALID, "NULL handle");
     if (!s->initialised) return fail(SS_ERR_STATE, "ss_step before ss_init");
+    if (s->in_ensemble) return fail(SS_ERR_STATE, "an ensemble member steps with ss_ensemble_step");
     if (s->p.noise_mode == SS_NOISE_EXTERNAL && !s->have_noise)
         return fail(SS_ERR_STATE, "SS_NOISE_EXTERNAL needs ss_set_noise before ss_step");
     SS_HIP_CHECK(hipSetDevice(s->p.device));
@@ -1582,10 +1599,24 @@ int ss_synchronize(ss_sim* s) {
     return SS_OK;
 }
 
+int ss_resort(ss_sim* s) {
+    if (!s) return fail(SS_ERR_INVALID, "NULL handle");
+    if (!s->initialised) return fail(SS_ERR_STATE, "ss_resort before ss_init");
+    if (s->in_ensemble) return fail(SS_ERR_STATE, "ensemble members re-sort together with the ensemble");
+    if (!s->sorted || s->pending) return SS_OK;
+    SS_HIP_CHECK(hipSetDevice(s->p.device));
+    // offsets from the counts binned by the last update (or by ss_init); the
+    // next step's update writes the particles in the new tile order
+    hipError_t e = launch_count_scan(s);
+    if (e != hipSuccess) return fail(SS_ERR_DEVICE, std::string("resort: ") + hipGetErrorString(e));
+    return SS_OK;
+}
+
 int ss_profile_steps(ss_sim* s, uint64_t nsteps, double* ms_per_step, char* names, int name_stride,
                      int max_stages) {
     if (!s || !ms_per_step) return fail(SS_ERR_INVALID, "NULL argument");
     if (!s->initialised) return fail(SS_ERR_STATE, "ss_profile_steps before ss_init");
+    if (s->in_ensemble) return fail(SS_ERR_STATE, "an ensemble member steps with the ensemble");
     if (s->p.noise_mode == SS_NOISE_EXTERNAL && !s->have_noise)
         return fail(SS_ERR_STATE, "SS_NOISE_EXTERNAL needs ss_set_noise");
     if (nsteps == 0) return fail(SS_ERR_INVALID, "nsteps must be >= 1");
@@ -2186,8 +2217,9 @@ static int ens_slot(const ss_ensemble* e) { return member_parity(e->m[0]) == e->
 // One ensemble step; `k` is the step's index within the re-sort period (the
 // step counter offset while capturing).
 template <typename T>
-static int ens_step_once(ss_ensemble* e, int k, bool capturing) {
+static int ens_step_once(ss_ensemble* e, int k, bool capturing, hipEvent_t* ev = nullptr) {
     hipError_t err;
+    if (ev) SS_HIP_CHECK(hipEventRecord(ev[0], e->stream));
     SS_HIP_CHECK(hipEventRecord(e->fork, e->stream));
     for (int q = 0; q < NSIDE; ++q) SS_HIP_CHECK(hipStreamWaitEvent(e->side[q], e->fork, 0));
     for (size_t g = 0; g < e->sgroups.size(); ++g) {
@@ -2208,6 +2240,7 @@ static int ens_step_once(ss_ensemble* e, int k, bool capturing) {
         SS_HIP_CHECK(hipEventRecord(e->join[q], e->side[q]));
         SS_HIP_CHECK(hipStreamWaitEvent(e->stream, e->join[q], 0));
     }
+    if (ev) SS_HIP_CHECK(hipEventRecord(ev[1], e->stream));
     const int slot = ens_slot(e);
     const bool resort = e->m[0]->pending;
     if (e->nm == 6)
@@ -2217,6 +2250,7 @@ static int ens_step_once(ss_ensemble* e, int k, bool capturing) {
     else
         ens_launch_update<T, 12>(e, slot, resort, k);
     SS_HIP_CHECK(hipGetLastError());
+    if (ev) SS_HIP_CHECK(hipEventRecord(ev[2], e->stream));
     e->steps_since_sort += 1;
     for (ss_sim* s : e->m) {
         if (resort) swap_sorted(s);
@@ -2236,6 +2270,7 @@ static int ens_step_once(ss_ensemble* e, int k, bool capturing) {
         }
         e->steps_since_sort = 0;
     }
+    if (ev) SS_HIP_CHECK(hipEventRecord(ev[3], e->stream));
     if (!capturing) hipLaunchKernelGGL(k_add_step, dim3(1), dim3(64), 0, e->stream, e->d_step, 1u);
     e->timestep += 1;
     return SS_OK;
@@ -2363,6 +2398,7 @@ int ss_ensemble_create(const ss_params* params, int nsys, ss_ensemble** out) {
         ss_sim* s = e->m[j];
         (void)hipStreamDestroy(s->stream);
         s->stream = e->side[j % NSIDE];
+        s->in_ensemble = true;
         s->d_step = e->d_step;
         const int per = (s->tg.ntiles + s->tg.nxcd - 1) / s->tg.nxcd;
         start.push_back(start.back() + per * s->tg.nxcd);
@@ -2415,6 +2451,57 @@ int ss_ensemble_destroy(ss_ensemble* e) {
     if (e->d_specfirst) (void)hipFree(e->d_specfirst);
     delete e;
     return SS_OK;
+}
+
+int ss_ensemble_profile_steps(ss_ensemble* e, uint64_t nsteps, double* ms_per_step, char* names, int name_stride,
+                              int max_stages) {
+    static const char* NAMES[3] = {"spectral", "update", "sort"};
+    if (!e || !ms_per_step) return fail(SS_ERR_INVALID, "NULL argument");
+    if (nsteps == 0) return fail(SS_ERR_INVALID, "nsteps must be >= 1");
+    for (ss_sim* s : e->m) {
+        if (!s->initialised) return fail(SS_ERR_STATE, "every member needs ss_init before profiling");
+        if (s->p.noise_mode == SS_NOISE_EXTERNAL && !s->have_noise)
+            return fail(SS_ERR_STATE, "SS_NOISE_EXTERNAL needs ss_set_noise for every member");
+    }
+    SS_HIP_CHECK(hipSetDevice(e->m[0]->p.device));
+    e->timestep = e->m[0]->timestep;
+    e->steps_since_sort = e->m[0]->steps_since_sort;
+    for (int q = 0; q < NSIDE; ++q) SS_HIP_CHECK(hipStreamSynchronize(e->side[q]));
+    SS_HIP_CHECK(hipMemcpy(e->d_step, &e->timestep, sizeof(uint64_t), hipMemcpyHostToDevice));
+    hipEvent_t ev[4];
+    for (int i = 0; i < 4; ++i) SS_HIP_CHECK(hipEventCreate(&ev[i]));
+    double acc[3] = {0.0, 0.0, 0.0};
+    int rc = SS_OK;
+    for (uint64_t k = 0; k < nsteps && rc == SS_OK; ++k) {
+        const int slot = e->prec == SS_FP64 ? ens_jobs_for_current<double>(e) : ens_jobs_for_current<float>(e);
+        if (slot < 0) {
+            rc = slot;
+            break;
+        }
+        rc = e->prec == SS_FP64 ? ens_step_once<double>(e, 0, false, ev) : ens_step_once<float>(e, 0, false, ev);
+        if (rc != SS_OK) break;
+        if (hipEventSynchronize(ev[3]) != hipSuccess) {
+            rc = fail(SS_ERR_DEVICE, "event sync failed");
+            break;
+        }
+        for (int st = 0; st < 3; ++st) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, ev[st], ev[st + 1]);
+            acc[st] += ms;
+        }
+    }
+    for (int i = 0; i < 4; ++i) (void)hipEventDestroy(ev[i]);
+    if (rc != SS_OK) return rc;
+    SS_HIP_CHECK(hipStreamSynchronize(e->stream));
+    const int nst = std::min(max_stages, 3);
+    for (int st = 0; st < nst; ++st) {
+        ms_per_step[st] = acc[st] / (double)nsteps;
+        if (names && name_stride > 0) {
+            std::strncpy(names + (size_t)st * name_stride, NAMES[st], name_stride - 1);
+            names[(size_t)st * name_stride + name_stride - 1] = 0;
+        }
+    }
+    return nst;
 }
 
 int ss_ensemble_member(ss_ensemble* e, int sys, ss_sim** out) {

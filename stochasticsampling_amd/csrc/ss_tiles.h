@@ -116,7 +116,7 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
     uint32_t cid = 0;
     NoiseRec<T> ncur{};  // NZ_REC: the record is prefetched with the state
     if (MOVE && i < p1) {
-        cid = a.s.id[i];
+        cid = at(a.s.id, i);
         load_particle(a.s, i, cur);
         if constexpr (NZ == NZ_REC) ncur = load_noise_rec(a.noise, i);
     }
@@ -125,8 +125,7 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
         if (MOVE) {
             // the cell record is requested before the next particle, so that
             // waiting for it leaves the next particle's loads in flight
-            const int cx = grid_index(cur.x, g.gx, g.nx), cy = grid_index(cur.y, g.gy, g.ny),
-                      cz = grid_index(cur.z, g.gz, g.nz);
+            const int cx = bin_x(g, cur.x), cy = bin_y(g, cur.y), cz = bin_z(g, cur.z);
             const T* recp = a.rec + (size_t)((cx * g.ny + cy) * g.nz + cz) * a.rs;
             float r[REC_FULL];
             if constexpr (sizeof(T) == 4) gather_record(recp, a.rs, r);
@@ -134,7 +133,7 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
             // here would make the wait for the record a wait for everything
             const uint32_t j = min(i + blockDim.x, p1 - 1);
             Loaded<T> nxt;
-            const uint32_t nid = a.s.id[j];
+            const uint32_t nid = at(a.s.id, j);
             load_particle(a.s, j, nxt);
             NoiseRec<T> nnxt{};
             if constexpr (NZ == NZ_REC) nnxt = load_noise_rec(a.noise, j);
@@ -162,7 +161,7 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
                 }
                 if (!h) o = a.dtoff[ts + 1] - 1u - atomicAdd(a.cross + ts, 1u);
                 out = a.d;
-                out.id[o] = cid;
+                at(out.id, o) = cid;
             }
             if constexpr (sizeof(T) == 4)
                 move_loaded<T, FULL>(out, o, cur, a.P, reinterpret_cast<const T*>(r), rv, nx, ny, nz, ophi, otheta);
@@ -172,14 +171,13 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
             cid = nid;
             if constexpr (NZ == NZ_REC) ncur = nnxt;
         } else {
-            nx = a.s.x[i];
-            ny = a.s.y[i];
-            nz = a.s.z[i];
-            ophi = a.s.phi[i];
-            otheta = a.s.theta[i];
+            nx = at(a.s.x, i);
+            ny = at(a.s.y, i);
+            nz = at(a.s.z, i);
+            ophi = at(a.s.phi, i);
+            otheta = at(a.s.theta, i);
         }
-        const int ix = grid_index(nx, g.gx, g.nx), iy = grid_index(ny, g.gy, g.ny),
-                  iz = grid_index(nz, g.gz, g.nz);
+        const int ix = bin_x(g, nx), iy = bin_y(g, ny), iz = bin_z(g, nz);
         const int ang = angle_of(g, ophi, otheta);
         const int wx = window_coord(ix, ox, tg.t[0], tg.h[0], g.nx);
         const int wy = window_coord(iy, oy, tg.t[1], tg.h[1], g.ny);
@@ -345,8 +343,7 @@ __global__ void __launch_bounds__(256)
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t t = INVALID_KEY;
     if (i < n)
-        t = (uint32_t)tile_of(tg, grid_index(s.x[i], g.gx, g.nx), grid_index(s.y[i], g.gy, g.ny),
-                              grid_index(s.z[i], g.gz, g.nz));
+        t = (uint32_t)tile_of(tg, bin_x(g, s.x[i]), bin_y(g, s.y[i]), bin_z(g, s.z[i]));
     Run r = wave_run(t);
     if (i < n && r.is_head) atomicAdd(tcnt + t, (uint32_t)r.len);
 }
@@ -371,8 +368,7 @@ __device__ inline void copy_particle(const State<T>& a, const State<T>& b, uint6
 
 template <typename T>
 __device__ inline uint32_t tile_of_particle(const State<T>& a, uint64_t i, const Geo<T>& g, const TileGeo& tg) {
-    return (uint32_t)tile_of(tg, grid_index(a.x[i], g.gx, g.nx), grid_index(a.y[i], g.gy, g.ny),
-                             grid_index(a.z[i], g.gz, g.nz));
+    return (uint32_t)tile_of(tg, bin_x(g, a.x[i]), bin_y(g, a.y[i]), bin_z(g, a.z[i]));
 }
 
 template <typename T>

@@ -50,12 +50,14 @@ def _declare(L):
         "ss_get_noise": (C.c_int, [vp, _d, C.c_uint64]),
         "ss_profile_steps": (C.c_int, [vp, C.c_uint64, _d, C.c_char_p, C.c_int, C.c_int]),
         "ss_synchronize": (C.c_int, [vp]),
+        "ss_resort": (C.c_int, [vp]),
         "ss_get_diagnostics": (C.c_int, [vp, _d]),
         "ss_get_info": (C.c_int, [vp, _P(Info)]),
         "ss_ensemble_create": (C.c_int, [pp, C.c_int, _P(vp)]),
         "ss_ensemble_destroy": (C.c_int, [vp]),
         "ss_ensemble_member": (C.c_int, [vp, C.c_int, _P(vp)]),
         "ss_ensemble_step": (C.c_int, [vp, C.c_uint64, _P(C.c_uint64)]),
+        "ss_ensemble_profile_steps": (C.c_int, [vp, C.c_uint64, _d, C.c_char_p, C.c_int, C.c_int]),
         "ss_initial_condition": (C.c_int, [pp, C.c_double, C.c_uint64, _d, C.c_uint64]),
         "ss_create_isotropic": (C.c_int, [C.c_uint64, _P(Box), C.c_uint64, C.c_int32, _d]),
         "ss_create_homogeneous": (C.c_int, [C.c_uint64, C.c_double, _P(Box), C.c_uint64, C.c_int32, _d]),

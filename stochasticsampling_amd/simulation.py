@@ -140,6 +140,10 @@ class Simulation:
     def synchronize(self):
         _l.check(self.L.ss_synchronize(self._h))
 
+    def resort(self):
+        """Start a new re-sort period at the next step (ss_resort)."""
+        _l.check(self.L.ss_resort(self._h))
+
     # getters ----------------------------------------------------------------
     def get_timestep(self):
         t = C.c_uint64()
@@ -263,6 +267,16 @@ class Ensemble:
         t = C.c_uint64()
         _l.check(self.L.ss_ensemble_step(self._h, nsteps, C.byref(t)))
         return t.value
+
+    def profile_steps(self, nsteps, max_stages=8):
+        """nsteps eager steps timed per stage (ss_ensemble_profile_steps):
+        {"spectral", "update", "sort": mean ms per step}."""
+        ms = np.zeros(max_stages)
+        stride = 32
+        names = C.create_string_buffer(stride * max_stages)
+        k = _l.check(self.L.ss_ensemble_profile_steps(self._h, nsteps, _l.dptr(ms), names, stride, max_stages))
+        raw = names.raw
+        return {raw[i * stride:(i + 1) * stride].split(b"\0", 1)[0].decode(): float(ms[i]) for i in range(k)}
 
     def close(self):
         if getattr(self, "_h", None):

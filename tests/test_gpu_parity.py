@@ -111,6 +111,40 @@ def test_coord_to_grid_fp32_matches_oracle_rule(oracle):
     np.testing.assert_array_equal(got, exp)
 
 
+@pytest.mark.parametrize("precision", [ss.FP32, ss.FP64])
+def test_coord_to_grid_on_bin_edges_all_axes(oracle, precision):
+    """The binning multiplies by the rounded reciprocal of the bin width and
+    divides only next to an integer quotient; on every axis, at each bin edge
+    k * g (rounded to the working precision), its neighbours one ulp away and
+    random values, the index equals floor(fl(q / g)) of the reference rule."""
+    grid = (64, 48, 40, 16, 12)
+    box = (50.0, 37.0, 29.5)
+    p = ss.make_params(box=box, grid=grid, precision=precision)
+    dt = np.float32 if precision == ss.FP32 else np.float64
+    widths = [box[0] / grid[0], box[1] / grid[1], box[2] / grid[2], 2 * np.pi / grid[3], np.pi / grid[4]]
+    tops = [box[0], box[1], box[2], 2 * np.pi, np.pi]
+    rng = np.random.default_rng(21)
+    cols = []
+    for a in range(5):
+        g = dt(widths[a])
+        k = np.arange(grid[a] + 1, dtype=dt)
+        edge = (k * g).astype(dt)
+        vals = np.concatenate([edge, np.nextafter(edge, dt(-1)), np.nextafter(edge, dt(1e9)),
+                               (rng.random(3000) * tops[a]).astype(dt)])
+        vals = vals[(vals >= 0) & (vals <= dt(tops[a]))]
+        cols.append(vals)
+    m = max(len(c) for c in cols)
+    parts = np.zeros((m, 5), dtype=dt)
+    for a in range(5):
+        parts[:, a] = np.resize(cols[a], m)
+    got = ss.debug_coord_to_grid(p, parts.astype(np.float64))
+    if precision == ss.FP32:
+        exp = np.array([oracle.coord_to_grid_f32(p, q) for q in parts])
+    else:
+        exp = np.array([oracle.coord_to_grid(p, q) for q in parts])
+    np.testing.assert_array_equal(got, exp)
+
+
 # ---------------------------------------------------------------------------
 # Particle update (modifiers_test.rs:39-97, mod_test.rs) and random cases
 # ---------------------------------------------------------------------------
