@@ -83,8 +83,9 @@ def time_gpu(prec, parts, steps, warmup, profile_steps, tune=False, **extra):
 def state_error_study(parts, steps):
     """The FP32 tiled step against the FP64 plain step (the GPU path the
     parity tests tie to the FP64 oracle within 1e-10) after ``steps`` steps
-    from the same initial condition with the same Philox noise (equal to
-    2^-21 in both precisions): particle positions (periodic distance, in
+    from the same initial condition with the same Philox noise (the same
+    52-bit uniforms in both precisions, FP32 rounding them to binary32):
+    particle positions (periodic distance, in
     units of the box length), orientations (|n32 - n64| of the unit
     vectors), and u, grad u of the last step's fields (max |a - b| / max |b|).
     Differences in the binning near cell edges are part of what is
@@ -121,9 +122,10 @@ def state_error_study(parts, steps):
                             "percentiles": dict(zip(map(str, q), map(float, np.percentile(dn, q))))},
             "u_rel_err": float(np.abs(ua - ub).max() / np.abs(ub).max()),
             "grad_u_rel_err": float(np.abs(ga - gb).max() / np.abs(gb).max()),
-            "note": "trajectory divergence after the given steps: the two runs bin slightly different "
-                    "particle sets near cell edges, and FP32 noise cuts the Gaussian tail at 5.5 sigma; "
-                    "the arithmetic error of one solve on the same distribution is fft_variants.*_rel_err"}
+            "note": "trajectory divergence after the given steps: both runs draw the same increments "
+                    "(to FP32 rounding), and their differences are arithmetic, amplified where the two runs "
+                    "bin a particle near a cell edge differently; the arithmetic error of one solve on the "
+                    "same distribution is fft_variants.*_rel_err"}
 
 
 def fft_variant_study(parts, steps):

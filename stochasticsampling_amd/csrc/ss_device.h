@@ -141,14 +141,17 @@ struct Noise {
 // Philox layout, per particle and step: counter (j, id, step lo, step hi),
 // key (seed lo, seed hi).  Call j = 0 gives four words w0..w3; six 21-bit
 // fields f0..f5 are the top 21 bits of w0..w3 and, for f4 and f5, the low 11
-// bits of w0|w1 and w2|w3 (11 + 10 bits).  FP32 uses u_k = (f_k + 1/2) 2^-21.
-// FP64 appends 31 bits from the top of the words of calls j = 1, 2 (in
-// order) to each f_k and uses u_k = (m_k + 1/2) 2^-52.  Both are exactly
-// representable and strictly inside (0, 1); the FP32 uniforms are the FP64
-// ones rounded to 21 bits, so both precisions see the same increments.
-// Transform: Box-Muller normals x, y from (u0, u1), z from (u2, u3), axis
-// angle 2 pi u4, rotate angle dr sqrt(-2 ln(1 - u5)) -- the RandomVector of
-// src/bin/simulation/mod.rs:51-57,259-286,383-385.
+// bits of w0|w1 and w2|w3 (11 + 10 bits).  31 more bits from the top of the
+// words of calls j = 1, 2 (in order) extend each f_k to a 52-bit integer m_k,
+// and u_k = (m_k + 1/2) 2^-52, strictly inside (0, 1) (smallest 2^-53, so the
+// normals reach 8.6 sigma).  Both precisions draw these same uniforms: FP64
+// exactly, FP32 rounded to binary32 (within one ulp); 1 - u5 is the uniform
+// of the complemented integer, exact in both.  The transforms differ only in
+// the transcendentals (libm in FP64, the hardware's FP32 log2, sin and cos of
+// turns in FP32), so the FP32 increments equal the FP64 ones to about 1e-6
+// relative.  Transform: Box-Muller normals x, y from (u0, u1), z from (u2,
+// u3), axis angle 2 pi u4, rotate angle dr sqrt(-2 ln(1 - u5)) -- the
+// RandomVector of src/bin/simulation/mod.rs:51-57,259-286,383-385.
 __host__ __device__ inline void philox_fields(const uint32_t w[4], uint32_t f[6]) {
     f[0] = w[0] >> 11;
     f[1] = w[1] >> 11;
@@ -183,14 +186,27 @@ __device__ inline void philox_noise(uint64_t seed, uint64_t step, uint32_t id, d
     nz.rot = dr * ::sqrt(-2.0 * ::log(1.0 - u[5]));
 }
 
+// (m + 1/2) 2^-52 for m = f 2^31 + t (f: 21 bits, t: 31 bits) in binary32:
+// f 2^-21 is exact, the tail is rounded once before the fused sum, so the
+// result is within one ulp of the exact value.
+__device__ inline float uniform52_f32(uint32_t f, uint32_t t) {
+    return __builtin_fmaf((float)t + 0.5f, 2.220446049250313080847e-16f, (float)f * 4.76837158203125e-07f);
+}
+
 __device__ inline void philox_noise(uint64_t seed, uint64_t step, uint32_t id, float dr, Noise<float>& nz) {
-    uint32_t w[4];
-    philox4x32_10(0u, id, (uint32_t)step, (uint32_t)(step >> 32), (uint32_t)seed, (uint32_t)(seed >> 32), w);
+    uint32_t w[12];
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    const uint32_t s0 = (uint32_t)step, s1 = (uint32_t)(step >> 32);
+    philox4x32_10(0u, id, s0, s1, k0, k1, w);
+    philox4x32_10(1u, id, s0, s1, k0, k1, w + 4);
+    philox4x32_10(2u, id, s0, s1, k0, k1, w + 8);
     uint32_t f[6];
     philox_fields(w, f);
-    float u[6];
+    float u[5];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) u[k] = ((float)f[k] + 0.5f) * (1.0f / 2097152.0f);
+    for (int k = 0; k < 5; ++k) u[k] = uniform52_f32(f[k], w[4 + k] >> 1);
+    // 1 - u5: the complement of the 52-bit integer
+    const float u5c = uniform52_f32(~f[5] & 0x1FFFFFu, ~(w[9] >> 1) & 0x7FFFFFFFu);
     // hardware transcendentals: v_log_f32 is log2, v_sin/v_cos take turns
     const float m2ln2 = -1.38629436111989061883f;  // -2 ln 2
     const float r0 = __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(u[0]));
@@ -200,7 +216,7 @@ __device__ inline void philox_noise(uint64_t seed, uint64_t step, uint32_t id, f
     nz.z = r1 * __builtin_amdgcn_cosf(u[3]);
     nz.sa = __builtin_amdgcn_sinf(u[4]);
     nz.ca = __builtin_amdgcn_cosf(u[4]);
-    nz.rot = dr * __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(1.0f - u[5]));
+    nz.rot = dr * __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(u5c));
 }
 
 // Caller-supplied RandomVector [x, y, z, axis_angle, rotate_angle].
