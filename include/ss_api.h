@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define SS_API_VERSION 2
+#define SS_API_VERSION 3
 
 /* Return codes. */
 #define SS_OK 0
@@ -228,6 +228,18 @@ int ss_profile_steps(ss_sim* sim, uint64_t nsteps, double* ms_per_step, char* na
    SS_ERR_STATE for a member of an ensemble. */
 int ss_resort(ss_sim* sim);
 
+/* Keep (on != 0) or stop keeping the full count grid of every step's
+   binning, as SS_FLAG_KEEP_DISTRIBUTION does from ss_create, so that
+   ss_get_distribution (Simulation::get_distribution, mod.rs:231-233) can be
+   read after every step on the tiled path too.  Turning it on allocates the
+   two count grids (kept until ss_destroy) and bins the current particles at
+   once, so ss_get_distribution is valid immediately (the distribution the
+   next step samples) and after every following step; while it is on, steps
+   are enqueued eagerly (no captured graphs) and the update adds the count
+   atomics.  No effect on the plain path, which always keeps the grid.
+   SS_ERR_UNSUPPORTED for a member of an ensemble. */
+int ss_keep_distribution(ss_sim* sim, int on);
+
 /* Global diagnostics of the current particles, reduced in FP64 (exact
    64-bit fixed-point sums, independent of the particle order), 15 doubles:
    out[0..2]  mean orientation vector n over the particles;
@@ -365,6 +377,29 @@ int ss_fft_plan_destroy(ss_fft_plan* plan);
 /* ---------------------------------------------------------------------------
  * Solver entry points for unit-level parity.
  * ------------------------------------------------------------------------- */
+
+/* Reusable solver handles -- SpectralSolver::new (spectral_solver.rs:33-68)
+   and MagneticSolver::new (magnetic_solver.rs:32-62): plan and allocate once
+   for params' grid, box, stress prefactors, shape and precision; every solve
+   then reuses them (no allocation per call).  One handle per host thread. */
+typedef struct ss_solver ss_solver;
+#define SS_SOLVER_FLOW 0
+#define SS_SOLVER_MAGNETIC 1
+int ss_solver_create(const ss_params* params, int kind, ss_solver** out);
+int ss_solver_destroy(ss_solver* solver);
+
+/* SpectralSolver::mean_flow_field(hydro_screening, dist) + get_real_flow_field
+   (spectral_solver.rs:233-258) on a distribution [nx][ny][nz][phi][theta]
+   (already multiplied by the box volume, like the caller in
+   src/bin/simulation/mod.rs:253-257): real parts of u [3][..] and grad_u
+   [3][3][..] (may be NULL).  SS_SOLVER_FLOW handles only. */
+int ss_solver_mean_flow_field(ss_solver* solver, double hydro_screening, const double* dist, double* u,
+                              double* grad_u);
+
+/* MagneticSolver::mean_magnetic_field (magnetic_solver.rs:139-161): real parts
+   of B [3][..] and grad B [3][3][..] (may be NULL).  SS_SOLVER_MAGNETIC
+   handles only. */
+int ss_solver_mean_magnetic_field(ss_solver* solver, const double* dist, double* b, double* grad_b);
 
 /* SpectralSolver::new + mean_flow_field (spectral_solver.rs:33-68,233-254) on a
    given distribution dist [nx][ny][nz][phi][theta] (already multiplied by the

@@ -1411,7 +1411,8 @@ int ss_get_distribution(ss_sim* s, uint32_t* counts, double* dist) {
     if (!s) return fail(SS_ERR_INVALID, "NULL handle");
     if (!s->initialised) return fail(SS_ERR_STATE, "not initialised");
     if (!s->have_hist || !s->hist)
-        return fail(SS_ERR_STATE, "no count grid kept: the sorted path needs SS_FLAG_KEEP_DISTRIBUTION");
+        return fail(SS_ERR_STATE,
+                    "no count grid kept: the sorted path needs SS_FLAG_KEEP_DISTRIBUTION or ss_keep_distribution");
     SS_HIP_CHECK(hipSetDevice(s->p.device));
     const size_t nb = (size_t)s->ns * s->na;
     std::vector<uint32_t> tmp;
@@ -1599,6 +1600,43 @@ int ss_synchronize(ss_sim* s) {
     return SS_OK;
 }
 
+int ss_keep_distribution(ss_sim* s, int on) {
+    if (!s) return fail(SS_ERR_INVALID, "NULL handle");
+    if (s->in_ensemble) return fail(SS_ERR_UNSUPPORTED, "ensemble members do not keep the distribution");
+    if (!s->sorted) return SS_OK;
+    SS_HIP_CHECK(hipSetDevice(s->p.device));
+    if (!on) {
+        s->p.flags &= ~SS_FLAG_KEEP_DISTRIBUTION;
+        s->use_graph = !(s->p.flags & SS_FLAG_NO_GRAPH);
+        s->have_hist = false;
+        s->swap_hist = false;
+        return SS_OK;
+    }
+    const size_t nb = (size_t)s->ns * s->na;
+    int rc;
+    if (!s->hist && (rc = alloc(s, (void**)&s->hist, nb * 4)) != SS_OK) return rc;
+    if (!s->hist_next && (rc = alloc(s, (void**)&s->hist_next, nb * 4)) != SS_OK) return rc;
+    s->p.flags |= SS_FLAG_KEEP_DISTRIBUTION;
+    drop_graphs(s);
+    s->use_graph = false;
+    if (s->initialised) {
+        // the counts of the current particles: the distribution the next step samples
+        fill_u32(s, s->hist, nb, 0u);
+        if (s->prec == SS_FP64)
+            hipLaunchKernelGGL(k_bin_hist<double>, dim3(blocks_for(s->n)), dim3(256), 0, s->stream, (const double*)s->x,
+                               (const double*)s->y, (const double*)s->z, (const double*)s->phi,
+                               (const double*)s->theta, s->n, geo<double>(s), s->hist);
+        else
+            hipLaunchKernelGGL(k_bin_hist<float>, dim3(blocks_for(s->n)), dim3(256), 0, s->stream, (const float*)s->x,
+                               (const float*)s->y, (const float*)s->z, (const float*)s->phi, (const float*)s->theta,
+                               s->n, geo<float>(s), s->hist);
+        SS_HIP_CHECK(hipGetLastError());
+        s->have_hist = true;
+    }
+    s->swap_hist = false;
+    return SS_OK;
+}
+
 int ss_resort(ss_sim* s) {
     if (!s) return fail(SS_ERR_INVALID, "NULL handle");
     if (!s->initialised) return fail(SS_ERR_STATE, "ss_resort before ss_init");
@@ -1654,67 +1692,120 @@ int ss_profile_steps(ss_sim* s, uint64_t nsteps, double* ms_per_step, char* name
 // ---------------------------------------------------------------------------
 // Solver entry points (unit-level parity).
 // ---------------------------------------------------------------------------
+
+// A planned solve on given distributions: a one-particle system on the plain
+// path owns the transforms, tables and spectra, plus a device copy of the
+// distribution.  Created once, reused for every solve.
+struct ss_solver {
+    ss_sim* s = nullptr;
+    double* d = nullptr;  // [Ns][Na] distribution
+    int kind = SS_SOLVER_FLOW;
+};
+
+
 template <typename T>
-static int solve_dense(const ss_params* params, const double* dist, bool magnetic, double* o1, double* o2) {
+static int solver_run(ss_solver* v, const double* dist, double* o1, double* o2) {
+    ss_sim* s = v->s;
+    const size_t nb = (size_t)s->ns * s->na;
+    SS_HIP_CHECK(hipMemcpyAsync(v->d, dist, nb * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    Geo<T> g = geo<T>(s);
+    GW gw = grid_width(s->p);
+    const T measure = (T)(gw.phi * gw.theta);
+    const unsigned nbk = (unsigned)((s->ns + 3) / 4);
+    if (s->nm == 6)
+        hipLaunchKernelGGL((k_contract_dense<T, 6>), dim3(nbk), dim3(256), 0, s->stream, v->d, g, (const T*)s->kern,
+                           measure, (cplx<T>*)s->spec_in);
+    else if (s->nm == 9)
+        hipLaunchKernelGGL((k_contract_dense<T, 9>), dim3(nbk), dim3(256), 0, s->stream, v->d, g, (const T*)s->kern,
+                           measure, (cplx<T>*)s->spec_in);
+    else
+        hipLaunchKernelGGL((k_contract_dense<T, 12>), dim3(nbk), dim3(256), 0, s->stream, v->d, g,
+                           (const T*)s->kern, measure, (cplx<T>*)s->spec_in);
+    SS_HIP_CHECK(hipGetLastError());
+    int rc = s->fft.run(s->spec_in, s->nm, -1, s->stream);
+    if (rc == SS_OK) rc = full_fields<T>(s);
+    if (rc == SS_OK) {
+        const int c0 = v->kind == SS_SOLVER_MAGNETIC ? 12 : 0;
+        rc = copy_real<T>(s, s->spec_full, c0, 3, o1);
+        if (rc == SS_OK && o2) rc = copy_real<T>(s, s->spec_full, c0 + 3, 9, o2);
+    }
+    return rc;
+}
+
+
+int ss_solver_create(const ss_params* params, int kind, ss_solver** out) {
+    if (!params || !out) return fail(SS_ERR_INVALID, "NULL argument");
+    *out = nullptr;
+    if (kind != SS_SOLVER_FLOW && kind != SS_SOLVER_MAGNETIC) return fail(SS_ERR_INVALID, "bad solver kind");
     ss_params p = *params;
     p.number_of_particles = 1;
     p.noise_mode = SS_NOISE_PHILOX;
     p.binning = SS_BIN_HISTOGRAM;
-    if (magnetic) p.flags |= SS_FLAG_MAGNETIC_ALWAYS;
+    if (kind == SS_SOLVER_MAGNETIC) p.flags |= SS_FLAG_MAGNETIC_ALWAYS;
     p.flags &= ~SS_FLAG_QUASI2D;
-    ss_sim* s = nullptr;
-    int rc = ss_create(&p, &s);
-    if (rc != SS_OK) return rc;
-    const size_t nb = (size_t)s->ns * s->na;
-    double* d = nullptr;
-    if (hipMalloc(&d, nb * sizeof(double)) != hipSuccess) {
-        ss_destroy(s);
+    ss_solver* v = new ss_solver();
+    v->kind = kind;
+    int rc = ss_create(&p, &v->s);
+    if (rc != SS_OK) {
+        delete v;
+        return rc;
+    }
+    const size_t nb = (size_t)v->s->ns * v->s->na;
+    if (hipMalloc(&v->d, nb * sizeof(double)) != hipSuccess) {
+        ss_destroy(v->s);
+        delete v;
         return fail(SS_ERR_ALLOC, "hipMalloc");
     }
-    rc = SS_OK;
-    if (hipMemcpy(d, dist, nb * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
-        rc = fail(SS_ERR_DEVICE, "copy in");
-    if (rc == SS_OK) {
-        Geo<T> g = geo<T>(s);
-        GW gw = grid_width(s->p);
-        const T measure = (T)(gw.phi * gw.theta);
-        const unsigned nbk = (unsigned)((s->ns + 3) / 4);
-        if (s->nm == 6)
-            hipLaunchKernelGGL((k_contract_dense<T, 6>), dim3(nbk), dim3(256), 0, s->stream, d, g,
-                               (const T*)s->kern, measure, (cplx<T>*)s->spec_in);
-        else if (s->nm == 9)
-            hipLaunchKernelGGL((k_contract_dense<T, 9>), dim3(nbk), dim3(256), 0, s->stream, d, g,
-                               (const T*)s->kern, measure, (cplx<T>*)s->spec_in);
-        else
-            hipLaunchKernelGGL((k_contract_dense<T, 12>), dim3(nbk), dim3(256), 0, s->stream, d, g,
-                               (const T*)s->kern, measure, (cplx<T>*)s->spec_in);
-        if (hipGetLastError() != hipSuccess) rc = fail(SS_ERR_DEVICE, "contract launch");
-    }
-    if (rc == SS_OK) rc = s->fft.run(s->spec_in, s->nm, -1, s->stream);
-    if (rc == SS_OK) rc = full_fields<T>(s);
-    if (rc == SS_OK) {
-        const int c0 = magnetic ? 12 : 0;
-        rc = copy_real<T>(s, s->spec_full, c0, 3, o1);
-        if (rc == SS_OK && o2) rc = copy_real<T>(s, s->spec_full, c0 + 3, 9, o2);
-    }
-    (void)hipFree(d);
+    *out = v;
+    return SS_OK;
+}
+
+int ss_solver_destroy(ss_solver* v) {
+    if (!v) return SS_OK;
+    if (v->s) (void)hipSetDevice(v->s->p.device);
+    if (v->d) (void)hipFree(v->d);
+    ss_destroy(v->s);
+    delete v;
+    return SS_OK;
+}
+
+int ss_solver_mean_flow_field(ss_solver* v, double hydro_screening, const double* dist, double* u, double* grad_u) {
+    if (!v || !dist || !u) return fail(SS_ERR_INVALID, "NULL argument");
+    if (v->kind != SS_SOLVER_FLOW) return fail(SS_ERR_INVALID, "not a flow solver");
+    SS_HIP_CHECK(hipSetDevice(v->s->p.device));
+    v->s->p.hydro_screening = hydro_screening;  // read by the solve at launch
+    return v->s->prec == SS_FP64 ? solver_run<double>(v, dist, u, grad_u) : solver_run<float>(v, dist, u, grad_u);
+}
+
+int ss_solver_mean_magnetic_field(ss_solver* v, const double* dist, double* b, double* grad_b) {
+    if (!v || !dist || !b) return fail(SS_ERR_INVALID, "NULL argument");
+    if (v->kind != SS_SOLVER_MAGNETIC) return fail(SS_ERR_INVALID, "not a magnetic solver");
+    SS_HIP_CHECK(hipSetDevice(v->s->p.device));
+    return v->s->prec == SS_FP64 ? solver_run<double>(v, dist, b, grad_b) : solver_run<float>(v, dist, b, grad_b);
+}
+
+static int solve_once(const ss_params* params, int kind, const double* dist, double* o1, double* o2) {
+    ss_solver* v = nullptr;
+    int rc = ss_solver_create(params, kind, &v);
+    if (rc != SS_OK) return rc;
+    rc = kind == SS_SOLVER_FLOW ? ss_solver_mean_flow_field(v, params->hydro_screening, dist, o1, o2)
+                                : ss_solver_mean_magnetic_field(v, dist, o1, o2);
     std::string msg = ss_last_error();
-    ss_destroy(s);
+    ss_solver_destroy(v);
     if (rc != SS_OK) set_error(msg);
     return rc;
 }
 
 int ss_flow_solve(const ss_params* params, const double* dist, double* u, double* grad_u) {
     if (!params || !dist || !u) return fail(SS_ERR_INVALID, "NULL argument");
-    return params->precision == SS_FP64 ? solve_dense<double>(params, dist, false, u, grad_u)
-                                        : solve_dense<float>(params, dist, false, u, grad_u);
+    return solve_once(params, SS_SOLVER_FLOW, dist, u, grad_u);
 }
 
 int ss_magnetic_solve(const ss_params* params, const double* dist, double* b, double* grad_b) {
     if (!params || !dist || !b) return fail(SS_ERR_INVALID, "NULL argument");
-    return params->precision == SS_FP64 ? solve_dense<double>(params, dist, true, b, grad_b)
-                                        : solve_dense<float>(params, dist, true, b, grad_b);
+    return solve_once(params, SS_SOLVER_MAGNETIC, dist, b, grad_b);
 }
+
 
 // DensityGradient::get_real_gradient (distribution/density_gradient.rs).
 template <typename T>

@@ -70,6 +70,11 @@ def _declare(L):
         "ss_fft_execute_host_noalloc": (C.c_int, [vp, vp]),
         "ss_fft_plan_destroy": (C.c_int, [vp]),
         "ss_flow_solve": (C.c_int, [pp, _d, _d, _d]),
+        "ss_solver_create": (C.c_int, [pp, C.c_int, _P(vp)]),
+        "ss_solver_destroy": (C.c_int, [vp]),
+        "ss_solver_mean_flow_field": (C.c_int, [vp, C.c_double, _d, _d, _d]),
+        "ss_solver_mean_magnetic_field": (C.c_int, [vp, _d, _d, _d]),
+        "ss_keep_distribution": (C.c_int, [vp, C.c_int]),
         "ss_magnetic_solve": (C.c_int, [pp, _d, _d, _d]),
         "ss_density_gradient": (C.c_int, [pp, _d, _d]),
         "ss_interpolate_vector_field": (C.c_int, [pp, _d, _d, C.c_uint64, _d]),

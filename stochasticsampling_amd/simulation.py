@@ -140,6 +140,11 @@ class Simulation:
     def synchronize(self):
         _l.check(self.L.ss_synchronize(self._h))
 
+    def keep_distribution(self, on=True):
+        """Keep (or stop keeping) the count grid of every step on the tiled
+        path (ss_keep_distribution); get_distribution is valid at once."""
+        _l.check(self.L.ss_keep_distribution(self._h, 1 if on else 0))
+
     def resort(self):
         """Start a new re-sort period at the next step (ss_resort)."""
         _l.check(self.L.ss_resort(self._h))
@@ -363,6 +368,60 @@ def mean_magnetic_field(params, dist):
     g = np.zeros((3, 3) + params.grid_shape)
     _l.check(L.ss_magnetic_solve(C.byref(params), _l.dptr(d), _l.dptr(b), _l.dptr(g)))
     return b, g
+
+
+class _Solver:
+    KIND = 0
+
+    def __init__(self, params):
+        self.L = _l.load()
+        self.params = params
+        h = C.c_void_p()
+        _l.check(self.L.ss_solver_create(C.byref(params), self.KIND, C.byref(h)))
+        self._h = h
+
+    def _out(self, with_gradient):
+        p = self.params
+        a = np.zeros((3,) + p.grid_shape)
+        g = np.zeros((3, 3) + p.grid_shape) if with_gradient else None
+        return a, g
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.L.ss_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class SpectralSolver(_Solver):
+    """SpectralSolver (spectral_solver.rs:20-258): planned once for the
+    params' grid, box, stress and precision, then mean_flow_field(screening,
+    dist) -> (u, grad_u) real parts for any number of distributions."""
+
+    KIND = 0
+
+    def mean_flow_field(self, hydro_screening, dist, with_gradient=True):
+        d = _f64(dist, self.params.dist_shape)
+        u, g = self._out(with_gradient)
+        _l.check(self.L.ss_solver_mean_flow_field(self._h, float(hydro_screening), _l.dptr(d), _l.dptr(u),
+                                                  None if g is None else _l.dptr(g)))
+        return (u, g) if with_gradient else u
+
+
+class MagneticSolver(_Solver):
+    """MagneticSolver (magnetic_solver.rs:20-169): planned once, then
+    mean_magnetic_field(dist) -> (B, grad_B) real parts."""
+
+    KIND = 1
+
+    def mean_magnetic_field(self, dist, with_gradient=True):
+        d = _f64(dist, self.params.dist_shape)
+        b, g = self._out(with_gradient)
+        _l.check(self.L.ss_solver_mean_magnetic_field(self._h, _l.dptr(d), _l.dptr(b),
+                                                      None if g is None else _l.dptr(g)))
+        return (b, g) if with_gradient else b
 
 
 class FFTPlan:

@@ -622,3 +622,70 @@ def test_full_size_fp32_invariants():
     assert np.all(parts2[:, :3] >= 0) and np.all(parts2[:, :3] <= 50.0)
     assert np.all(parts2[:, 3] >= 0) and np.all(parts2[:, 3] <= 2 * np.pi)
     assert np.all(parts2[:, 4] >= 0) and np.all(parts2[:, 4] <= np.pi)
+
+
+# ---------------------------------------------------------------------------
+# Reusable solver handles (SpectralSolver / MagneticSolver) and the
+# run-time switch of the kept distribution
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("precision", [ss.FP64, ss.FP32])
+def test_solver_handles_reused_match_oracle(oracle, precision):
+    """One SpectralSolver and one MagneticSolver (spectral_solver.rs:33-68,
+    magnetic_solver.rs:32-62) serve three distributions and screenings in a
+    row: each result equals the one-shot solve bit for bit and the oracle
+    within the precision's bar."""
+    p = ss.make_params(box=(10.0, 12.0, 9.0), grid=(16, 12, 10, 6, 5), stress_active=-1.3, stress_magnetic=0.8,
+                       shape=0.4, precision=precision)
+    tol = FP64_TOL if precision == ss.FP64 else FP32_FIELD_TOL
+    fs = ss.SpectralSolver(p)
+    bs = ss.MagneticSolver(p)
+    rng = np.random.default_rng(7)
+    for screening in (0.0, 0.25, 1.5):
+        dist = rng.random(p.dist_shape) * 3.0
+        u, g = fs.mean_flow_field(screening, dist)
+        ue, ge = oracle.mean_flow_field(p, screening, dist)
+        assert rel_err(u, ue) < tol and rel_err(g, ge) < tol
+        p.hydro_screening = screening
+        u1, g1 = ss.mean_flow_field(p, dist)
+        np.testing.assert_array_equal(u, u1)
+        np.testing.assert_array_equal(g, g1)
+        b, gb = bs.mean_magnetic_field(dist)
+        be, gbe = oracle.mean_magnetic_field(p, dist)
+        assert rel_err(b, be) < tol and rel_err(gb, gbe) < tol
+    fs.close()
+    bs.close()
+
+
+def test_keep_distribution_switched_at_run_time(oracle):
+    """The FP32 tiled path without SS_FLAG_KEEP_DISTRIBUTION has no count grid;
+    ss_keep_distribution turns it on between steps: at once the counts of
+    the current particles (the reference's binning rule in binary32), then
+    after every step the same counts and particles as a run keeping the
+    grid from its creation; turned off, the steps go on unchanged."""
+    kw = dict(grid=(16, 16, 16, 8, 8), n=1 << 14, precision=ss.FP32, resort_interval=4)
+    parts = ss.initial_condition(small_params(**kw), amplitude=0.05, seed=14)
+    a = ss.Simulation(small_params(**kw))
+    b = ss.Simulation(small_params(flags=ss.FLAG_KEEP_DISTRIBUTION, **kw))
+    for sim in (a, b):
+        sim.init(parts)
+        sim.step(5)
+    with pytest.raises(ss.SSError, match="ss_keep_distribution"):
+        a.get_counts()
+    a.keep_distribution(True)
+    p = small_params(**kw)
+    cur = a.get_particles().astype(np.float32)
+    exp = np.zeros(p.dist_shape, dtype=np.uint32)
+    for q in cur:
+        exp[tuple(oracle.coord_to_grid_f32(p, q))] += 1
+    np.testing.assert_array_equal(a.get_counts(), exp)
+    for _ in range(3):
+        a.step(1)
+        b.step(1)
+        np.testing.assert_array_equal(a.get_counts(), b.get_counts())
+        np.testing.assert_array_equal(a.get_particles(), b.get_particles())
+    a.keep_distribution(False)
+    with pytest.raises(ss.SSError):
+        a.get_counts()
+    a.step(9)
+    b.step(9)
+    np.testing.assert_array_equal(a.get_particles(), b.get_particles())
