@@ -28,7 +28,7 @@ import time
 import numpy as np
 
 from . import output as out
-from .params import FLAG_KEEP_DISTRIBUTION, FP32, NOISE_REFERENCE
+from .params import FP32, NOISE_REFERENCE
 from .simulation import Simulation, Snapshot
 from .simulation import create_homogeneous as _gpu_homogeneous
 from .simulation import create_isotropic as _gpu_isotropic
@@ -144,9 +144,12 @@ def run_simulation(settings, sim, writer):
     box = (s.box_size.x, s.box_size.y, s.box_size.z)
     grid = (s.grid_size.x, s.grid_size.y, s.grid_size.z, s.grid_size.phi, s.grid_size.theta)
     if o.initial_condition:
+        # the count grid only while a distribution is written (tiled path)
+        sim.keep_distribution(True)
         writer.append(out.entry_value(sim.get_timestep(),
                                       distribution=out.distribution_value(sim.get_distribution(), box, grid),
                                       particles=_particles(sim, o) if o.particles is not None else None))
+        sim.keep_distribution(False)
     start = sim.get_timestep() + 1
     last = s.number_of_timesteps + start
     every = [x for x in (o.distribution, o.flowfield, o.magneticfield, o.particles, o.snapshot) if x]
@@ -161,13 +164,20 @@ def run_simulation(settings, sim, writer):
         if nxt > t:
             sim.step(nxt - t)
             t = nxt
-        sim.step(1)
         ts = t
 
         def due(x):
             return x is not None and ts % x == 0
 
-        dist = out.distribution_value(sim.get_distribution(), box, grid) if due(o.distribution) else None
+        # the distribution of an output step is the one it samples: keep the
+        # count grid from just before that step (it bins the particles then)
+        if due(o.distribution):
+            sim.keep_distribution(True)
+        sim.step(1)
+        dist = None
+        if due(o.distribution):
+            dist = out.distribution_value(sim.get_distribution(), box, grid)
+            sim.keep_distribution(False)
         ff = sim.get_flow_field() if due(o.flowfield) else None
         mf = sim.get_magnetic_field() if due(o.magneticfield) else None
         parts = _particles(sim, o) if due(o.particles) else None
@@ -188,10 +198,7 @@ def run(settings, out_dir, precision=FP32, init_type="distribution", noise="phil
     """A whole run like the reference binary's ``run`` (main.rs:66-132):
     output directory, the settings saved as TOML, the header, the loop.
     Returns the output file path."""
-    o = settings.simulation.output_at_timestep
     flags = overrides.pop("flags", 0)
-    if o.distribution is not None or o.initial_condition:
-        flags |= FLAG_KEEP_DISTRIBUTION
     if noise == "reference":
         if noise_streams is None:
             env = os.environ.get("RAYON_NUM_THREADS")

@@ -273,6 +273,8 @@ def test_driver_run_and_resume(tmp_path):
     sim = ss.Simulation(p)
     box = (4.0, 4.0, 4.0)
     sim.init(driver.create_isotropic(3000, box, 1))
+    # the driver keeps the count grid only around its distribution outputs
+    np.testing.assert_array_equal(sim.get_distribution(), r[0]["distribution"]["dist"])
     sim.step(6)
     assert r[4]["timestep"] == 6
     np.testing.assert_array_equal(sim.get_particles(), r[4]["particles"])
