@@ -26,9 +26,17 @@ HBM_PEAK_GBS = 8000.0
 # Philox: read x,y,z,phi,theta + 3 compensation words, write the same) and of
 # the FP64 plain update (read 40 + write 40; noise is generated in-kernel).
 UPDATE_BYTES = {"f32": 64.0, "f64": 80.0}
-# The plain path's binning kernel reads the five FP64 coordinates (40 B); on
-# the tiled path the binning of the new state is fused into the update.
+# The plain path's bin stage reads the five FP64 coordinates (40 B per
+# particle) and writes the count grid (4 B per bin) and the moments (16 B per
+# cell and component); on the tiled path the binning of the new state is fused
+# into the update.
 BIN_BYTES = {"f64": 40.0}
+
+
+def bin_stage_bytes(n, grid=(64, 64, 64, 16, 16), nm=6):
+    """Algorithmic bytes of one plain-path bin stage."""
+    ns, na = grid[0] * grid[1] * grid[2], grid[3] * grid[4]
+    return BIN_BYTES["f64"] * n + 4.0 * ns * na + 16.0 * nm * ns
 
 
 def hbm_frac(nbytes, n, ms):
@@ -95,7 +103,7 @@ def state_error_study(parts, steps):
 
     n = parts.shape[0]
     p32 = autotune.tuned(make(ss.FP32, n))
-    sims = [ss.Simulation(p32), ss.Simulation(make(ss.FP64, n, binning=ss.BIN_HISTOGRAM))]
+    sims = [ss.Simulation(p32), ss.Simulation(make(ss.FP64, n, binning=ss.BIN_PLAIN_TILED))]
     out = []
     for sim in sims:
         sim.init(parts)
@@ -216,7 +224,7 @@ def small_case(steps, threads):
     cpu_s = time.perf_counter() - t0
     return {"workload": "FP64, L=10, 16^3x8x8, 2^16 particles, dt=1e-3, D_t=0.1, D_r=0.5, alpha=-1, seed 1",
             "steps": steps, "gpu": {"value": n * steps / gpu_s, "ms_per_step": gpu_s * 1e3 / steps,
-                                    "path": "FP64 plain (histogram binning)"},
+                                    "path": "FP64 plain (tile-ordered count rows)"},
             "cpu": {"value": n * steps / cpu_s, "ms_per_step": cpu_s * 1e3 / steps, "cores": threads,
                     "kind": "port"},
             "gpu_over_cpu": cpu_s / gpu_s, "unit": "particle-steps/s"}
@@ -415,14 +423,26 @@ def main():
     fp64 = None
     if not args.no_fp64:
         print("[bench] FP64 plain path", file=sys.stderr, flush=True)
+        # profiled over one re-sort period (16 steps), so the bin stage holds
+        # its share of the re-sort
         ms64, st64 = time_gpu(ss.FP64, parts, max(1, args.steps // 4), min(args.warmup, 5),
-                              min(args.profile_steps, 10), binning=ss.BIN_HISTOGRAM)
+                              16 if args.profile_steps else 0, binning=ss.BIN_PLAIN_TILED)
+        bb = bin_stage_bytes(n)
         fp64 = {"value": n / (ms64 * 1e-3), "ms_per_step": ms64, "steps": max(1, args.steps // 4),
-                "dtype": "f64", "binning": "histogram", "stages_ms": st64,
-                # the bin stage: clear, histogram and contraction kernels
-                "hbm_frac": {"bin_stage": hbm_frac(BIN_BYTES["f64"], n, st64.get("bin")),
+                "dtype": "f64", "binning": "plain_tiled", "stages_ms": st64,
+                # the bin stage: per-tile count rows, drifted particles,
+                # contraction (and every 16th step the re-sort)
+                "hbm_frac": {"bin_stage": bb / (st64["bin"] * 1e-3) / 1e9 / HBM_PEAK_GBS if st64.get("bin") else None,
                              "update": hbm_frac(UPDATE_BYTES["f64"], n, st64.get("update")),
-                             "bytes_per_particle": {"bin": BIN_BYTES["f64"], "update": UPDATE_BYTES["f64"]}}}
+                             "bytes": {"bin_stage": bb, "update_per_particle": UPDATE_BYTES["f64"]}}}
+        # the same plain step with the count grid built by global atomics
+        print("[bench] FP64 plain path, atomic histogram", file=sys.stderr, flush=True)
+        ms64h, st64h = time_gpu(ss.FP64, parts, max(1, args.steps // 4), min(args.warmup, 5),
+                                min(args.profile_steps, 10), binning=ss.BIN_HISTOGRAM)
+        fp64["histogram"] = {"value": n / (ms64h * 1e-3), "ms_per_step": ms64h, "binning": "histogram",
+                             "stages_ms": st64h,
+                             "hbm_frac": {"bin_stage": bb / (st64h["bin"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+                                          if st64h.get("bin") else None}}
         # the same FP64 step on the tiled path (binning fused into the update)
         print("[bench] FP64 tiled path", file=sys.stderr, flush=True)
         ms64t, st64t = time_gpu(ss.FP64, parts, max(1, args.steps // 4), min(args.warmup, 5),

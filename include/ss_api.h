@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define SS_API_VERSION 3
+#define SS_API_VERSION 4
 
 /* Return codes. */
 #define SS_OK 0
@@ -44,10 +44,15 @@ extern "C" {
                                 259-286 */
 
 /* Binning strategy for particle -> (x,y,z,phi,theta) histogram. */
-#define SS_BIN_AUTO 0      /* SORTED for FP32, HISTOGRAM for FP64                        */
+#define SS_BIN_AUTO 0      /* SORTED for FP32, PLAIN_TILED for FP64 (each falls back to
+                              HISTOGRAM where it does not fit)                        */
 #define SS_BIN_HISTOGRAM 1 /* plain: full 5-D u32 histogram by global atomics           */
 #define SS_BIN_SORTED 2    /* particles grouped by spatial tile; binning fused into the
                               update with LDS-privatised fixed-point moments          */
+#define SS_BIN_PLAIN_TILED 3 /* plain path (separate binning, solve and update) with the
+                              particles kept in tile order: per-tile LDS counts stored
+                              with plain stores, a counting sort every resort_interval
+                              steps after the update                                  */
 
 /* FFT variant used by the spectral solve. */
 #define SS_FFT_AUTO 0  /* autotuned among the variants that pass the accuracy gate */
@@ -112,7 +117,7 @@ typedef struct ss_params {
     int32_t device;      /* HIP device ordinal          */
     /* Tiled path (SS_BIN_SORTED): tile extent in cells per axis (0: chosen
        automatically) and steps between re-sorts of the particles by tile
-       (0: 16). */
+       (0: 16).  SS_BIN_PLAIN_TILED uses resort_interval alone. */
     int32_t tile[3];
     int32_t resort_interval;
     /* SS_NOISE_REFERENCE: number of generator streams, the reference run's
@@ -222,9 +227,10 @@ int ss_profile_steps(ss_sim* sim, uint64_t nsteps, double* ms_per_step, char* na
 
 /* Starts a new re-sort period: the next step's update writes the particles
    in tile order (offsets from the counts of the last binning), as every
-   resort_interval-th step does.  No effect on the plain path or when a
-   re-sort is already due; results do not depend on it (the tiled binning is
-   exact).  No reference counterpart (the reference keeps particles unsorted).
+   resort_interval-th step does.  SS_BIN_PLAIN_TILED sorts the particles
+   at once (no effect when nothing moved since the last sort).  No effect on
+   SS_BIN_HISTOGRAM or when a re-sort is already due; results do not depend on
+   it (both tiled binnings are exact).  No reference counterpart (the reference keeps particles unsorted).
    SS_ERR_STATE for a member of an ensemble. */
 int ss_resort(ss_sim* sim);
 
@@ -266,6 +272,9 @@ typedef struct ss_info {
     int32_t pencils;         /* (y, z) pencils per workgroup (fused spectral)  */
     int32_t graphs;          /* whole periods replayed from captured hipGraphs */
     int32_t update_groups;   /* workgroups of the tile update (one per tile)   */
+    int32_t plain_tiled;     /* plain path with tile-ordered particles
+                                (SS_BIN_PLAIN_TILED); tile, ntiles and
+                                resort_interval describe its tiles       */
 } ss_info;
 int ss_get_info(const ss_sim* sim, ss_info* out);
 

@@ -87,11 +87,15 @@ __global__ void __launch_bounds__(256)
 // Contraction of the angular counts of each cell against the moment kernels
 // (average_stress, stress.rs:60-96; DirectorField, director.rs:36-61; density,
 // src/bin/simulation/mod.rs:312-316).  kern: [nm][na]; out: moments m (times
-// scale) as complex with zero imaginary part.  One thread per cell: the
-// cell's counts are read along its row (16-byte loads when the row length
-// allows), the kernels are the same for every thread (LDS broadcast).  A
-// wavefront per cell spent most of its time in the cross-lane reduction of
-// seven sums.
+// scale) as complex with zero imaginary part.  CONTRACT_LANES lanes per cell
+// read the cell's row in 16-byte pieces side by side (a wavefront reads eight
+// rows' 128-byte stretches, whole cache lines, where one thread per cell
+// touched 64 lines per load), sum their pieces in order and combine by a
+// fixed butterfly, so the result does not depend on the launch.  The kernels
+// sit in LDS.
+constexpr int CONTRACT_LANES = 8;
+constexpr int CONTRACT_GROUPS = 4;
+
 template <typename T, int NM>
 __global__ void __launch_bounds__(256)
     k_contract_hist(const uint32_t* __restrict__ hist, Geo<T> g, const T* __restrict__ kern_g, T scale,
@@ -101,34 +105,48 @@ __global__ void __launch_bounds__(256)
     const int na = g.na;
     for (int t = threadIdx.x; t < NM * na; t += blockDim.x) kern[t] = kern_g[t];
     __syncthreads();
-    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cell >= g.ns) return;
-    const uint32_t* cnt = hist + (size_t)cell * na;
-    T acc[NM];
-    T tot = T(0);
+    const int part = threadIdx.x % CONTRACT_LANES;
+    // CONTRACT_GROUPS groups of 256 / CONTRACT_LANES cells per workgroup (the
+    // kernels are loaded once per workgroup)
+    for (int grp = 0; grp < CONTRACT_GROUPS; ++grp) {
+        const int cell = ((blockIdx.x * CONTRACT_GROUPS + grp) * (int)blockDim.x + (int)threadIdx.x) / CONTRACT_LANES;
+        const bool live = cell < g.ns;
+        T acc[NM];
+        uint32_t tot = 0;
 #pragma unroll
-    for (int q = 0; q < NM; ++q) acc[q] = T(0);
-    auto add = [&](int a, uint32_t n) {
-        const T c = (T)n;
-        tot += c;
+        for (int q = 0; q < NM; ++q) acc[q] = T(0);
+        auto add = [&](int a, uint32_t n) {
+            const T c = (T)n;
+            tot += n;
 #pragma unroll
-        for (int q = 0; q < NM; ++q) acc[q] += kern[q * na + a] * c;
-    };
-    if ((na & 3) == 0) {
-        const uint4* c4 = reinterpret_cast<const uint4*>(cnt);
-        for (int a = 0; a < na; a += 4) {
-            const uint4 v = c4[a >> 2];
-            add(a, v.x);
-            add(a + 1, v.y);
-            add(a + 2, v.z);
-            add(a + 3, v.w);
+            for (int q = 0; q < NM; ++q) acc[q] += kern[q * na + a] * c;
+        };
+        if (live) {
+            const uint32_t* cnt = hist + (size_t)cell * na;
+            if ((na & 3) == 0) {
+                const uint4* c4 = reinterpret_cast<const uint4*>(cnt);
+                for (int a = 4 * part; a < na; a += 4 * CONTRACT_LANES) {
+                    const uint4 v = c4[a >> 2];
+                    add(a, v.x);
+                    add(a + 1, v.y);
+                    add(a + 2, v.z);
+                    add(a + 3, v.w);
+                }
+            } else {
+                for (int a = part; a < na; a += CONTRACT_LANES) add(a, cnt[a]);
+            }
         }
-    } else {
-        for (int a = 0; a < na; ++a) add(a, cnt[a]);
-    }
+        for (int off = CONTRACT_LANES / 2; off > 0; off >>= 1) {
+            tot += __shfl_xor(tot, off);
 #pragma unroll
-    for (int q = 0; q < NM; ++q) spec[(size_t)q * g.ns + cell] = cplx<T>{acc[q] * scale, T(0)};
-    if (rho) rho[cell] = tot * rho_scale;
+            for (int q = 0; q < NM; ++q) acc[q] += __shfl_xor(acc[q], off);
+        }
+        if (live && part == 0) {
+#pragma unroll
+            for (int q = 0; q < NM; ++q) spec[(size_t)q * g.ns + cell] = cplx<T>{acc[q] * scale, T(0)};
+            if (rho) rho[cell] = (T)tot * rho_scale;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -490,7 +508,10 @@ __global__ void __launch_bounds__(256)
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Noise<T> rv;
-    draw_noise<T, NZ>(P, seed, *step + step_off, (uint32_t)i, n, noise, rv);
+    // Philox and caller-supplied noise belong to the particle (its index at
+    // ss_init), the reference stream's records to the storage slot
+    const uint32_t key = (NZ == NZ_REC || !s.id) ? (uint32_t)i : s.id[i];
+    draw_noise<T, NZ>(P, seed, *step + step_off, key, n, noise, rv);
     T nx, ny, nz, ophi, otheta;
     move_particle<T, FULL>(s, i, g, P, rec, rs, rv, nx, ny, nz, ophi, otheta);
 }

@@ -148,6 +148,12 @@ struct ss_sim {
     ss_params p;
     int prec = SS_FP64;
     bool sorted = false;
+    // plain path with the particles kept in tile order (SS_BIN_PLAIN_TILED):
+    // per-tile LDS counts stored with plain stores, drifted particles through
+    // a list, a re-sort in the binning of every `resort`-th step
+    bool psort = false;
+    int psort_lds = 0;
+    uint32_t *strays = nullptr, *scnt = nullptr;  // drifted particles' keys per tile stretch, and their number
     // spectral stage as three fused kernels (tiled path, no magnetic solve,
     // radices 2-5); spec_in then holds moments transformed along z and y only
     bool fused = false;
@@ -280,6 +286,19 @@ static void swap_ab(ss_sim* s) {
     std::swap(s->id, s->id2);
 }
 
+static void swap_sorted(ss_sim* s) {
+    swap_ab(s);
+    std::swap(s->toff, s->toff2);
+    s->pending = false;
+}
+
+// inv[id[slot]] = slot for the current state (after every change of order).
+static void launch_inverse_perm(ss_sim* s) {
+    if (!s->inv) return;
+    hipLaunchKernelGGL(k_inverse_perm, dim3(blocks_for(s->n)), dim3(256), 0, s->stream, (const uint32_t*)s->id, s->n,
+                       s->inv);
+}
+
 template <typename T>
 static UpdateParams<T> update_params(const ss_sim* s) {
     const ss_params& p = s->p;
@@ -331,28 +350,65 @@ static void fill_u32(ss_sim* s, uint32_t* p, size_t n, uint32_t v) {
     hipLaunchKernelGGL(k_fill_u32, dim3(std::max(1u, grid)), dim3(256), 0, s->stream, p, n, v);
 }
 
+// Moments and density from the count grid (plain path).
+template <typename T>
+static void contract_hist(ss_sim* s, const Geo<T>& g, double ms, double ds, T* rho) {
+    const size_t per_wg = (size_t)CONTRACT_GROUPS * 256 / CONTRACT_LANES;  // cells per workgroup
+    const unsigned nb = (unsigned)((s->ns + per_wg - 1) / per_wg);
+    const size_t lds = (size_t)s->nm * s->na * sizeof(T);
+    if (s->nm == 6)
+        hipLaunchKernelGGL((k_contract_hist<T, 6>), dim3(nb), dim3(256), lds, s->stream, s->hist, g,
+                           (const T*)s->kern, (T)ms, (T)ds, (cplx<T>*)s->spec_in, rho);
+    else if (s->nm == 9)
+        hipLaunchKernelGGL((k_contract_hist<T, 9>), dim3(nb), dim3(256), lds, s->stream, s->hist, g,
+                           (const T*)s->kern, (T)ms, (T)ds, (cplx<T>*)s->spec_in, rho);
+    else
+        hipLaunchKernelGGL((k_contract_hist<T, 12>), dim3(nb), dim3(256), lds, s->stream, s->hist, g,
+                           (const T*)s->kern, (T)ms, (T)ds, (cplx<T>*)s->spec_in, rho);
+}
+
 template <typename T>
 static hipError_t launch_bin(ss_sim* s) {
     Geo<T> g = geo<T>(s);
     double ms, ds;
     scales(s, ms, ds);
     T* rho = s->use_rho ? (T*)s->rho : nullptr;
-    if (!s->sorted) {
+    if (s->psort) {
+        // a re-sort is due: count the particles per tile while binning, then
+        // move them to the new tile order before the update
+        const bool resort = s->steps_since_sort >= (uint64_t)s->resort;
+        State<T> st = state_a<T>(s);
+        const dim3 grid((unsigned)s->tg.ntiles);
+        const size_t lds = (size_t)s->psort_lds;
+        auto bin = [&](auto cc) {
+            constexpr bool C = decltype(cc)::value;
+            hipLaunchKernelGGL((k_bin_tile<T, C>), grid, dim3(256), lds, s->stream, st, (const uint32_t*)s->toff, g,
+                               s->tg, s->hist, s->strays, s->scnt, s->tcnt);
+            hipLaunchKernelGGL((k_bin_strays<C>), grid, dim3(64), 0, s->stream, (const uint32_t*)s->strays,
+                               (const uint32_t*)s->toff, (const uint32_t*)s->scnt, s->ny, s->nz, s->na, s->tg,
+                               s->hist, s->tcnt);
+        };
+        if (!resort) {
+            bin(std::false_type());
+        } else {
+            bin(std::true_type());
+            const int nt = s->tg.ntiles;
+            hipLaunchKernelGGL(k_scan_reduce, dim3(s->nparts), dim3(256), 0, s->stream, s->tcnt, nt, s->part);
+            hipLaunchKernelGGL(k_scan_apply, dim3(s->nparts), dim3(256), 0, s->stream, s->tcnt, nt, s->part,
+                               s->nparts, s->toff2, s->cursor);
+            hipLaunchKernelGGL(k_psort_permute<T>, grid, dim3(256), 0, s->stream, st, state_b<T>(s),
+                               (const uint32_t*)s->toff, (const uint32_t*)s->toff2, g, s->tg, s->cursor);
+            swap_sorted(s);
+            launch_inverse_perm(s);
+            s->steps_since_sort = 0;
+        }
+        contract_hist<T>(s, g, ms, ds, rho);
+    } else if (!s->sorted) {
         fill_u32(s, s->hist, (size_t)s->ns * s->na, 0u);
         hipLaunchKernelGGL(k_bin_hist<T>, dim3(blocks_for(s->n)), dim3(256), 0, s->stream, (const T*)s->x,
                            (const T*)s->y, (const T*)s->z, (const T*)s->phi, (const T*)s->theta, s->n, g,
                            s->hist);
-        const unsigned nb = (unsigned)((s->ns + 255) / 256);
-        const size_t lds = (size_t)s->nm * s->na * sizeof(T);
-        if (s->nm == 6)
-            hipLaunchKernelGGL((k_contract_hist<T, 6>), dim3(nb), dim3(256), lds, s->stream, s->hist, g,
-                               (const T*)s->kern, (T)ms, (T)ds, (cplx<T>*)s->spec_in, rho);
-        else if (s->nm == 9)
-            hipLaunchKernelGGL((k_contract_hist<T, 9>), dim3(nb), dim3(256), lds, s->stream, s->hist, g,
-                               (const T*)s->kern, (T)ms, (T)ds, (cplx<T>*)s->spec_in, rho);
-        else
-            hipLaunchKernelGGL((k_contract_hist<T, 12>), dim3(nb), dim3(256), lds, s->stream, s->hist, g,
-                               (const T*)s->kern, (T)ms, (T)ds, (cplx<T>*)s->spec_in, rho);
+        contract_hist<T>(s, g, ms, ds, rho);
     } else {
         // the moments were accumulated by the previous update (or the initial
         // binning); convert them
@@ -588,19 +644,6 @@ static TileArgs<T, NM> tile_args(ss_sim* s) {
     return a;
 }
 
-static void swap_sorted(ss_sim* s) {
-    swap_ab(s);
-    std::swap(s->toff, s->toff2);
-    s->pending = false;
-}
-
-// inv[id[slot]] = slot for the current state (after every change of order).
-static void launch_inverse_perm(ss_sim* s) {
-    if (!s->inv) return;
-    hipLaunchKernelGGL(k_inverse_perm, dim3(blocks_for(s->n)), dim3(256), 0, s->stream, (const uint32_t*)s->id, s->n,
-                       s->inv);
-}
-
 // Tiled path: move every particle and bin the new state (MOVE), or only bin
 // the current state (initial sampling).  A pending re-sort makes the move
 // write state B in the new order, which then becomes current.
@@ -768,6 +811,8 @@ static int step_once(ss_sim* s, hipEvent_t* ev) {
                 return fail(SS_ERR_DEVICE, std::string("sort: ") + hipGetErrorString(e));
         }
         s->swap_hist = s->hist_next != nullptr;
+    } else if (s->psort) {
+        s->steps_since_sort += 1;  // the binning of a later step re-sorts
     }
     mark(ST_SORT);
     if (s->capturing) {
@@ -794,6 +839,8 @@ static int initial_binning(ss_sim* s) {
         if (s->hist_next) std::swap(s->hist, s->hist_next);
         s->swap_hist = false;
     } else {
+        if (s->psort && (e = launch_initial_sort<T>(s)) != hipSuccess)
+            return fail(SS_ERR_DEVICE, hipGetErrorString(e));
         if ((e = launch_bin<T>(s)) != hipSuccess) return fail(SS_ERR_DEVICE, hipGetErrorString(e));
     }
     s->have_hist = !s->sorted || (s->p.flags & SS_FLAG_KEEP_DISTRIBUTION);
@@ -1049,7 +1096,7 @@ static int create_impl(ss_sim* s) {
         A(s->ly, n * 4);
         A(s->lz, n * 4);
     }
-    if (s->sorted) {
+    if (s->sorted || s->psort) {
         A(s->x2, n * sizeof(T));
         A(s->y2, n * sizeof(T));
         A(s->z2, n * sizeof(T));
@@ -1067,9 +1114,15 @@ static int create_impl(ss_sim* s) {
         A(s->toff, ((size_t)nt + 1) * 4);
         A(s->toff2, ((size_t)nt + 1) * 4);
         A(s->cursor, (size_t)nt * 4);
-        A(s->home_ctr, (size_t)nt * 4);
         s->nparts = (nt + SCAN_CHUNK - 1) / SCAN_CHUNK;
         A(s->part, (size_t)s->nparts * 4);
+    }
+    if (s->psort) {
+        A(s->strays, n * 4);
+        A(s->scnt, (size_t)s->tg.ntiles * 4);
+    }
+    if (s->sorted) {
+        A(s->home_ctr, (size_t)s->tg.ntiles * 4);
         A(s->mom_main, (size_t)s->ns * (s->nm + 1) * 8);
         A(s->mom_inflow, (size_t)s->ns * (s->nm + 1) * 8);
         SS_HIP_CHECK(hipMemset(s->mom_inflow, 0, (size_t)s->ns * (s->nm + 1) * 8));
@@ -1090,7 +1143,7 @@ static int create_impl(ss_sim* s) {
     SS_HIP_CHECK(hipMemset(s->spec_in, 0, cs * s->nm));
     if (p.noise_mode == SS_NOISE_EXTERNAL) A(s->noise, 5 * n * sizeof(T));
     if (p.noise_mode == SS_NOISE_REFERENCE) A(s->noise, NOISE_REC * n * sizeof(T));
-    if (p.noise_mode == SS_NOISE_REFERENCE && s->sorted) A(s->inv, n * sizeof(uint32_t));
+    if (p.noise_mode == SS_NOISE_REFERENCE && (s->sorted || s->psort)) A(s->inv, n * sizeof(uint32_t));
     const size_t stage_doubles = std::max<size_t>(5 * n, (size_t)24 * s->ns);
     A(s->staging, stage_doubles * sizeof(double));
     A(s->staging2, 3 * n * sizeof(double));
@@ -1204,6 +1257,39 @@ static int choose_tiles(ss_sim* s) {
     return SS_OK;
 }
 
+// Tile geometry of the plain path with tile-ordered particles: tiles of
+// 4 x 4 x 2 cells (clipped to the grid), halved along their longest axis until
+// the tile's counts ([cells][angles] u32) fit 64 KiB of LDS.  No halo.  False
+// when even one cell does not fit, or when a count-grid index does not fit
+// 32 bits (the drifted particles' list holds them).
+static bool choose_plain_tiles(ss_sim* s) {
+    if ((uint64_t)s->ns * (uint64_t)s->na >= (1ull << 32)) return false;
+    const int n[3] = {s->nx, s->ny, s->nz};
+    int t[3] = {std::min(4, n[0]), std::min(4, n[1]), std::min(2, n[2])};
+    auto lds = [&]() { return (size_t)t[0] * t[1] * t[2] * s->na * 4; };
+    while (lds() > 65536) {
+        int best = -1;
+        for (int a = 0; a < 3; ++a)
+            if (t[a] > 1 && (best < 0 || t[a] > t[best])) best = a;
+        if (best < 0) return false;
+        t[best] /= 2;
+    }
+    TileGeo& tg = s->tg;
+    tg.ntiles = 1;
+    for (int a = 0; a < 3; ++a) {
+        tg.t[a] = t[a];
+        tg.h[a] = 0;
+        tg.w[a] = t[a];
+        tg.nt[a] = (n[a] + t[a] - 1) / t[a];
+        tg.ntiles *= tg.nt[a];
+    }
+    tg.ws = t[0] * t[1] * t[2];
+    tg.nxcd = 8;
+    s->resort = s->p.resort_interval > 0 ? s->p.resort_interval : 16;
+    s->psort_lds = (int)lds();
+    return true;
+}
+
 static int validate(const ss_params* p) {
     if (!p) return fail(SS_ERR_INVALID, "params is NULL");
     const ss_grid& g = p->grid_size;
@@ -1219,7 +1305,7 @@ static int validate(const ss_params* p) {
         return fail(SS_ERR_INVALID, "bad noise_mode");
     if (p->noise_mode == SS_NOISE_REFERENCE && p->noise_streams < 1)  // mod.rs:113-116
         return fail(SS_ERR_INVALID, "No environment variable 'RAYON_NUM_THREADS' set.");
-    if (p->binning < SS_BIN_AUTO || p->binning > SS_BIN_SORTED) return fail(SS_ERR_INVALID, "bad binning");
+    if (p->binning < SS_BIN_AUTO || p->binning > SS_BIN_PLAIN_TILED) return fail(SS_ERR_INVALID, "bad binning");
     if (p->fft_variant != SS_FFT_AUTO && p->fft_variant != SS_FFT_RADIX && p->fft_variant != SS_FFT_DFT)
         return fail(SS_ERR_INVALID, "bad fft_variant");
     if (p->fft_variant == SS_FFT_DFT && p->precision != SS_FP32)
@@ -1314,6 +1400,16 @@ int ss_create(const ss_params* params, ss_sim** out) {
         delete s;
         return fail(SS_ERR_UNSUPPORTED, "too many angle bins for the LDS-resident moment kernels");
     }
+    s->psort = !s->sorted && (p.binning == SS_BIN_PLAIN_TILED || (p.binning == SS_BIN_AUTO && p.precision == SS_FP64));
+    if (s->psort && !choose_plain_tiles(s)) {
+        if (p.binning == SS_BIN_PLAIN_TILED) {
+            delete s;
+            return fail(SS_ERR_UNSUPPORTED, "one cell's angle counts and the moment kernels exceed 64 KiB of LDS");
+        }
+        s->psort = false;
+    }
+    // the end-of-period sort of this path is not captured
+    if (s->psort) s->use_graph = false;
     if (hipStreamCreate(&s->stream) != hipSuccess) {
         delete s;
         return fail(SS_ERR_DEVICE, "hipStreamCreate failed");
@@ -1575,7 +1671,8 @@ int ss_get_info(const ss_sim* s, ss_info* out) {
     std::memset(out, 0, sizeof(*out));
     out->sorted = s->sorted ? 1 : 0;
     out->fused_spectral = s->fused ? (s->paired ? 2 : 1) : 0;
-    if (s->sorted) {
+    out->plain_tiled = s->psort ? 1 : 0;
+    if (s->sorted || s->psort) {
         for (int k = 0; k < 3; ++k) out->tile[k] = s->tg.t[k];
         out->ntiles = s->tg.ntiles;
         out->resort_interval = s->resort;
@@ -1641,6 +1738,12 @@ int ss_resort(ss_sim* s) {
     if (!s) return fail(SS_ERR_INVALID, "NULL handle");
     if (!s->initialised) return fail(SS_ERR_STATE, "ss_resort before ss_init");
     if (s->in_ensemble) return fail(SS_ERR_STATE, "ensemble members re-sort together with the ensemble");
+    if (s->psort) {
+        // the next step's binning re-sorts (already in tile order when
+        // nothing moved since the last sort)
+        if (s->steps_since_sort > 0) s->steps_since_sort = std::max<uint64_t>(s->steps_since_sort, s->resort);
+        return SS_OK;
+    }
     if (!s->sorted || s->pending) return SS_OK;
     SS_HIP_CHECK(hipSetDevice(s->p.device));
     // offsets from the counts binned by the last update (or by ss_init); the

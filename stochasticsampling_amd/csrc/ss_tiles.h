@@ -386,4 +386,140 @@ __global__ void __launch_bounds__(256)
     copy_particle(a, b, i, toff[t] + base + (uint32_t)(__lane_id() - r.head));
 }
 
+// Plain path with the particles kept in tile order (re-sorted every few
+// steps; the tile geometry without halo): one workgroup per tile counts the
+// particles of its segment that lie in its own cells into LDS
+// ([tile cell][angle] u32) and stores those rows of the count grid with plain
+// stores, so the grid needs no clearing and no global atomics for them.  A
+// particle that drifted out of its tile since the last sort is listed in its
+// tile's own stretch of the list (at the tile's segment offset, an LDS counter
+// per tile) for k_bin_strays.  The counts are exact whatever the order.
+// COUNT (steps that re-sort): particles per tile of the current positions
+// into tcnt (own particles here, drifted ones by k_bin_strays).
+// (Distribution::histogram_from, distribution/mod.rs:146-160.)
+constexpr int PSORT_BATCH = 4;  // particles loaded per thread before binning them
+
+template <typename T, bool COUNT>
+__global__ void __launch_bounds__(256)
+    k_bin_tile(State<T> st, const uint32_t* __restrict__ toff, Geo<T> g, TileGeo tg, uint32_t* __restrict__ hist,
+               uint32_t* __restrict__ strays, uint32_t* __restrict__ scnt, uint32_t* __restrict__ tcnt) {
+    extern __shared__ __attribute__((aligned(16))) uint32_t cnt[];  // [t0 t1 t2][na]
+    __shared__ uint32_t s_stray;
+    const int na = g.na;
+    const int t = blockIdx.x;
+    const int t0 = tg.t[0], t1 = tg.t[1], t2 = tg.t[2], tc = t0 * t1 * t2;
+    const int tz = t % tg.nt[2], ty = (t / tg.nt[2]) % tg.nt[1], tx = t / (tg.nt[2] * tg.nt[1]);
+    const int ox = tx * t0, oy = ty * t1, oz = tz * t2;
+    const int ex = min(t0, g.nx - ox), ey = min(t1, g.ny - oy), ez = min(t2, g.nz - oz);
+    const bool v4 = (na & 3) == 0;  // rows in 16-byte pieces
+    if (v4)
+        for (int k = threadIdx.x; k < tc * na / 4; k += blockDim.x) reinterpret_cast<uint4*>(cnt)[k] = uint4{0, 0, 0, 0};
+    else
+        for (int k = threadIdx.x; k < tc * na; k += blockDim.x) cnt[k] = 0u;
+    if (threadIdx.x == 0) s_stray = 0u;
+    __syncthreads();
+    const uint32_t i0 = toff[t], i1 = toff[t + 1];
+    for (uint32_t base = i0; base < i1; base += PSORT_BATCH * blockDim.x) {
+        T q[PSORT_BATCH][5];
+#pragma unroll
+        for (int b = 0; b < PSORT_BATCH; ++b) {
+            const uint32_t i = base + b * blockDim.x + threadIdx.x;
+            if (i < i1) {
+                q[b][0] = at(st.x, i);
+                q[b][1] = at(st.y, i);
+                q[b][2] = at(st.z, i);
+                q[b][3] = at(st.phi, i);
+                q[b][4] = at(st.theta, i);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < PSORT_BATCH; ++b) {
+            if (base + b * blockDim.x + threadIdx.x >= i1) continue;
+            const int ix = bin_x(g, q[b][0]), iy = bin_y(g, q[b][1]), iz = bin_z(g, q[b][2]);
+            const int a = angle_of(g, q[b][3], q[b][4]);
+            const int lx = ix - ox, ly = iy - oy, lz = iz - oz;
+            if (lx >= 0 && lx < ex && ly >= 0 && ly < ey && lz >= 0 && lz < ez) {
+                atomicAdd(cnt + ((lx * t1 + ly) * t2 + lz) * na + a, 1u);
+            } else {
+                const uint32_t k = atomicAdd(&s_stray, 1u);
+                strays[i0 + k] = (uint32_t)((ix * g.ny + iy) * g.nz + iz) * (uint32_t)na + (uint32_t)a;
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        scnt[t] = s_stray;
+        if (COUNT) atomicAdd(tcnt + t, (i1 - i0) - s_stray);
+    }
+    // rows of the own cells: z-runs of ez cells are contiguous in the grid
+    const int w = v4 ? na / 4 : na;  // pieces per row
+    const int rows = ex * ey * ez * w;
+    for (int k = threadIdx.x; k < rows; k += blockDim.x) {
+        const int a = k % w, c = k / w;
+        const int lz = c % ez, ly = (c / ez) % ey, lx = c / (ez * ey);
+        const size_t cell = (size_t)((ox + lx) * g.ny + oy + ly) * g.nz + oz + lz;
+        const int lc = (lx * t1 + ly) * t2 + lz;
+        if (v4)
+            reinterpret_cast<uint4*>(hist)[cell * w + a] = reinterpret_cast<const uint4*>(cnt)[lc * w + a];
+        else
+            hist[cell * na + a] = cnt[lc * na + a];
+    }
+}
+
+// The drifted particles' counts, one wavefront per tile over the tile's
+// stretch of the list, after every tile stored its rows; COUNT: their new
+// tile's particle count.
+template <bool COUNT>
+__global__ void __launch_bounds__(64)
+    k_bin_strays(const uint32_t* __restrict__ strays, const uint32_t* __restrict__ toff,
+                 const uint32_t* __restrict__ scnt, int ny, int nz, int na, TileGeo tg, uint32_t* __restrict__ hist,
+                 uint32_t* __restrict__ tcnt) {
+    const int t = blockIdx.x;
+    const uint32_t i0 = toff[t], m = scnt[t];
+    for (uint32_t j = threadIdx.x; j < m; j += blockDim.x) {
+        const uint32_t key = strays[i0 + j];
+        atomicAdd(hist + key, 1u);
+        if (COUNT) {
+            const uint32_t cell = key / (uint32_t)na;
+            const int iz = (int)(cell % (uint32_t)nz), iy = (int)((cell / (uint32_t)nz) % (uint32_t)ny),
+                      ix = (int)(cell / ((uint32_t)nz * (uint32_t)ny));
+            atomicAdd(tcnt + tile_of(tg, ix, iy, iz), 1u);
+        }
+    }
+}
+
+// Re-sort of the tile-ordered plain path, one workgroup per tile of the
+// current order: particles still in the tile fill its new segment (offsets
+// toff2, from the counts of k_bin_tile / k_bin_strays with COUNT) from the
+// front, one LDS reservation per wavefront; the others fill their new tile's
+// segment from the back (back: zero per tile on entry).
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_psort_permute(State<T> a, State<T> b, const uint32_t* __restrict__ toff, const uint32_t* __restrict__ toff2,
+                    Geo<T> g, TileGeo tg, uint32_t* __restrict__ back) {
+    __shared__ uint32_t s_front;
+    const int t = blockIdx.x;
+    if (threadIdx.x == 0) s_front = 0u;
+    __syncthreads();
+    const uint32_t i0 = toff[t], i1 = toff[t + 1], f0 = toff2[t];
+    const int lane = __lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (uint32_t base = i0; base < i1; base += blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        const bool valid = i < i1;
+        int d = -1;
+        if (valid) d = tile_of(tg, bin_x(g, at(a.x, i)), bin_y(g, at(a.y, i)), bin_z(g, at(a.z, i)));
+        const bool own = valid && d == t;
+        const unsigned long long mask = __ballot(own);
+        uint32_t wb = 0;
+        if (lane == 0 && mask) wb = atomicAdd(&s_front, (uint32_t)__popcll(mask));
+        wb = __shfl(wb, 0);
+        if (own) {
+            copy_particle(a, b, i, f0 + wb + (uint32_t)__popcll(mask & below));
+        } else if (valid) {
+            copy_particle(a, b, i, toff2[d + 1] - 1u - atomicAdd(back + d, 1u));
+        }
+    }
+}
+
 }  // namespace ss

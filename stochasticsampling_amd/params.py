@@ -14,6 +14,7 @@ NOISE_REFERENCE = 2
 BIN_AUTO = 0
 BIN_HISTOGRAM = 1
 BIN_SORTED = 2
+BIN_PLAIN_TILED = 3
 FFT_AUTO = 0
 FFT_RADIX = 1
 FFT_DFT = 2
@@ -119,6 +120,7 @@ class Info(C.Structure):
         ("pencils", C.c_int32),
         ("graphs", C.c_int32),
         ("update_groups", C.c_int32),
+        ("plain_tiled", C.c_int32),
     ]
 
 

@@ -214,6 +214,7 @@ class Simulation:
             "pencils": i.pencils,
             "graphs": bool(i.graphs),
             "update_groups": i.update_groups,
+            "plain_tiled": bool(i.plain_tiled),
         }
 
     def profile_steps(self, nsteps, max_stages=16):

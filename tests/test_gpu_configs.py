@@ -65,7 +65,7 @@ def _run_pair(oracle, p, parts, steps, oracle_params=None):
 # ---------------------------------------------------------------------------
 # 16^3 x 8 x 8, L = 10, 2^16 particles, FP64, 100 steps
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("binning", [ss.BIN_HISTOGRAM, ss.BIN_SORTED])
+@pytest.mark.parametrize("binning", [ss.BIN_HISTOGRAM, ss.BIN_SORTED, ss.BIN_PLAIN_TILED])
 def test_small_case_fp64_100_steps(oracle, binning):
     box = (10.0, 10.0, 10.0)
     p = _params(box, (16, 16, 16, 8, 8), 1 << 16, 1, binning=binning, flags=ss.FLAG_KEEP_DISTRIBUTION)
@@ -87,7 +87,7 @@ BIG_BOX = (50.0, 50.0, 50.0)
 BIG_GRID = (64, 64, 64, 16, 16)
 
 
-@pytest.mark.parametrize("binning", [ss.BIN_HISTOGRAM, ss.BIN_SORTED])
+@pytest.mark.parametrize("binning", [ss.BIN_HISTOGRAM, ss.BIN_SORTED, ss.BIN_PLAIN_TILED])
 def test_large_grid_fp64_matches_oracle(oracle, binning):
     """FP64 at 64^3 x 16 x 16, L = 50, 2^18 particles, 3 steps."""
     n = 1 << 18
@@ -137,23 +137,29 @@ def test_large_grid_fp32_matches_oracle(oracle):
 
 
 def test_full_size_fp64_paths_agree():
-    """2^24 particles on 64^3 x 16 x 16, FP64: the plain histogram path and
-    the tiled path bin the same counts and move the particles alike (the
-    moments differ only in summation order), and the size-independent
+    """2^24 particles on 64^3 x 16 x 16, FP64: the plain histogram path, the
+    tiled path and the plain path with tile-ordered particles bin the same
+    counts and move the particles alike (the moments differ only in summation
+    order; both plain paths contract the same count grid with the same kernel,
+    so they agree bit for bit), and the size-independent
     properties hold: every particle binned, u of zero mean and divergence
     free in k space, states inside the box and the angle ranges."""
     n = 1 << 24
     p = _params(BIG_BOX, BIG_GRID, n, 2, flags=ss.FLAG_KEEP_DISTRIBUTION)
     parts = ss.initial_condition(p, amplitude=0.05, seed=2)
     res = []
-    for binning in (ss.BIN_HISTOGRAM, ss.BIN_SORTED):
+    for binning in (ss.BIN_HISTOGRAM, ss.BIN_SORTED, ss.BIN_PLAIN_TILED):
+        print(f"full size FP64 binning={binning}", flush=True)
         sim = ss.Simulation(_params(BIG_BOX, BIG_GRID, n, 2, binning=binning, flags=ss.FLAG_KEEP_DISTRIBUTION))
         sim.init(parts)
         sim.step(2)
         res.append((sim.get_counts(), sim.get_flow_field(), sim.get_particles()))
         sim.close()
-    (c0, u0, x0), (c1, u1, x1) = res
+    (c0, u0, x0), (c1, u1, x1), (c2, u2, x2) = res
     np.testing.assert_array_equal(c0, c1)
+    np.testing.assert_array_equal(c2, c0)
+    np.testing.assert_array_equal(u2, u0)
+    np.testing.assert_array_equal(x2, x0)
     assert int(c0.sum(dtype=np.uint64)) == n
     # the tiled path sums kernel values rounded to multiples of 2^-37 (the
     # fixed-point scale that keeps 2^24 particle values below 2^62): about
@@ -266,7 +272,7 @@ OPTIONAL = dict(shape=0.6, volume_exclusion=0.3, hydro_screening=0.2, stress_mag
                 magnetic_reorientation=1.5, magnetic_dipole_dipole=0.8, magnetic_drag=0.4)
 
 
-@pytest.mark.parametrize("binning", [ss.BIN_HISTOGRAM, ss.BIN_SORTED])
+@pytest.mark.parametrize("binning", [ss.BIN_HISTOGRAM, ss.BIN_SORTED, ss.BIN_PLAIN_TILED])
 @pytest.mark.parametrize("variant", ["y_field", "z_field", "small_alpha"])
 def test_optional_physics_whole_steps_fp64(oracle, binning, variant):
     """The Langevin chain of mod.rs:323-363 with Jeffery strain, density-

@@ -278,7 +278,7 @@ def _run_pair(oracle, p, parts, steps):
     return sim, osim
 
 
-@pytest.mark.parametrize("binning", [ss.BIN_HISTOGRAM, ss.BIN_SORTED])
+@pytest.mark.parametrize("binning", [ss.BIN_HISTOGRAM, ss.BIN_SORTED, ss.BIN_PLAIN_TILED])
 def test_init_histogram_bit_exact_fp64(oracle, binning):
     p = small_params(binning=binning, flags=ss.FLAG_KEEP_DISTRIBUTION)
     parts = isotropic_particles(p.number_of_particles, (10.0, 10.0, 10.0), 4)
@@ -288,7 +288,7 @@ def test_init_histogram_bit_exact_fp64(oracle, binning):
     np.testing.assert_array_equal(sim.get_particles(), osim.particles())
 
 
-@pytest.mark.parametrize("binning", [ss.BIN_HISTOGRAM, ss.BIN_SORTED])
+@pytest.mark.parametrize("binning", [ss.BIN_HISTOGRAM, ss.BIN_SORTED, ss.BIN_PLAIN_TILED])
 def test_steps_fp64_match_oracle(oracle, binning):
     p = small_params(binning=binning, flags=ss.FLAG_KEEP_DISTRIBUTION, n=4096)
     parts = ss.initial_condition(p, amplitude=0.05, seed=1)
@@ -320,6 +320,40 @@ def test_sorted_and_histogram_paths_identical_fp64():
     # identical per-particle arithmetic; the moments differ only in summation
     # order, so the states agree to rounding
     assert np.max(np.abs(out[0][0] - out[1][0])) < 1e-12
+
+
+@pytest.mark.parametrize("precision", [ss.FP64, ss.FP32])
+def test_plain_tiled_binning_exact_and_independent_of_sort_interval(precision):
+    """The plain path with tile-ordered particles builds the same count grid
+    as the atomic histogram path and contracts it with the same kernel, so
+    counts, fields and states equal that path's bit for bit, whatever the
+    sort interval and however many particles drifted out of their tile
+    between sorts (fast translational diffusion here: about a quarter of a
+    cell per step).  The tiled path bins the same counts."""
+    kw = dict(grid=(16, 16, 16, 8, 8), n=1 << 14, precision=precision, diffusion_translational=5.0)
+    parts = ss.initial_condition(small_params(**kw), amplitude=0.05, seed=6)
+    runs = {}
+    for name, binning, resort in [("pt1", ss.BIN_PLAIN_TILED, 1), ("pt5", ss.BIN_PLAIN_TILED, 5),
+                                  ("pt16", ss.BIN_PLAIN_TILED, 16), ("tiled", ss.BIN_SORTED, 4),
+                                  ("hist", ss.BIN_HISTOGRAM, 0)]:
+        sim = ss.Simulation(small_params(binning=binning, resort_interval=resort, flags=ss.FLAG_KEEP_DISTRIBUTION,
+                                         **kw))
+        if binning == ss.BIN_PLAIN_TILED:
+            assert sim.get_info()["plain_tiled"]
+        sim.init(parts)
+        sim.step(1)
+        first = (sim.get_flow_field(), sim.get_counts())
+        sim.step(20)
+        runs[name] = (*first, sim.get_particles(), sim.get_counts(), sim.get_flow_field())
+        sim.close()
+    ref = runs["hist"]
+    for name in ("pt1", "pt5", "pt16"):
+        for a, b in zip(runs[name], ref):
+            np.testing.assert_array_equal(a, b)
+    # the first step bins the same particles on every path; the tiled path
+    # sums fixed-point moments
+    np.testing.assert_array_equal(runs["tiled"][1], ref[1])
+    assert rel_err(runs["tiled"][0], ref[0]) < (1e-12 if precision == ss.FP64 else 1e-5)
 
 
 @pytest.mark.parametrize("precision", [ss.FP64, ss.FP32])

@@ -59,6 +59,8 @@ def _oracle_states(oracle, seed, t):
         (5, 8, ss.BIN_HISTOGRAM, ss.FP64, 0),
         (50_000, 1, ss.BIN_HISTOGRAM, ss.FP64, 0),
         (20_000, 7, ss.BIN_SORTED, ss.FP64, 0),
+        (20_000, 7, ss.BIN_PLAIN_TILED, ss.FP64, 1),
+        (20_000, 5, ss.BIN_PLAIN_TILED, ss.FP64, 3),
         (20_000, 7, ss.BIN_SORTED, ss.FP32, 1),
         (1 << 20, 16, ss.BIN_SORTED, ss.FP32, 2),
     ],

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
     for name in declared:
         assert hasattr(L, name), name
     assert set(declared) == set(sslib.EXPORTS)
-    assert L.ss_version() == 3
+    assert L.ss_version() == 4
 
 
 def test_params_struct_layout_matches_header(tmp_path):
