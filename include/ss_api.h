@@ -291,8 +291,8 @@ int ss_get_info(const ss_sim* sim, ss_info* out);
 typedef struct ss_ensemble ss_ensemble;
 
 /* params: nsys entries, one per member (1 <= nsys <= 256).  SS_ERR_UNSUPPORTED
- * when the largest member tile window plus 4 (nsys + 1) bytes exceed 64 KiB
- * of LDS. */
+ * when the largest member tile window plus 4 (nsys + 1) bytes exceed the LDS
+ * of a workgroup (160 KiB on gfx950). */
 int ss_ensemble_create(const ss_params* params, int nsys, ss_ensemble** out);
 int ss_ensemble_destroy(ss_ensemble* ens);
 

@@ -41,7 +41,7 @@ TILES = [(4, 4, 4), (2, 4, 4), (4, 4, 2), (4, 4, 8), (4, 8, 4), (2, 2, 4), (8, 4
 # ensembles also try larger tiles (fewer per member)
 ENSEMBLE_TILES = TILES + [(8, 8, 4), (8, 4, 8), (4, 8, 8), (8, 8, 8)]
 RESORTS = [8, 16, 32]
-LDS_LIMIT = 65536  # tile window + moment kernels of the tiled update
+LDS_LIMIT = 163840  # tile window + moment kernels of the tiled update (gfx950: 160 KiB per workgroup)
 
 
 def size_class(p):

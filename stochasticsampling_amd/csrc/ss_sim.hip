@@ -140,6 +140,13 @@ static int upload(void** dst, const std::vector<double>& src) {
 
 static inline unsigned blocks_for(uint64_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }
 
+// Dynamic LDS beyond the default 64 KiB needs the kernel's limit raised
+// before the launch.  It is raised to the device's limit, so the setting is
+// the same whichever simulation asks first.
+static inline void allow_lds(int lds_max, const void* f, size_t bytes) {
+    if (bytes > 65536) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+}
+
 }  // namespace ss
 
 using namespace ss;
@@ -147,6 +154,7 @@ using namespace ss;
 struct ss_sim {
     ss_params p;
     int prec = SS_FP64;
+    int lds_max = 65536;  // dynamic LDS per workgroup the device allows (gfx950: 160 KiB)
     bool sorted = false;
     // plain path with the particles kept in tile order (SS_BIN_PLAIN_TILED):
     // per-tile LDS counts stored with plain stores, drifted particles through
@@ -356,15 +364,17 @@ static void contract_hist(ss_sim* s, const Geo<T>& g, double ms, double ds, T* r
     const size_t per_wg = (size_t)CONTRACT_GROUPS * 256 / CONTRACT_LANES;  // cells per workgroup
     const unsigned nb = (unsigned)((s->ns + per_wg - 1) / per_wg);
     const size_t lds = (size_t)s->nm * s->na * sizeof(T);
+    auto go = [&](auto kern) {
+        allow_lds(s->lds_max, (const void*)kern, lds);
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(256), lds, s->stream, s->hist, g, (const T*)s->kern, (T)ms, (T)ds,
+                           (cplx<T>*)s->spec_in, rho);
+    };
     if (s->nm == 6)
-        hipLaunchKernelGGL((k_contract_hist<T, 6>), dim3(nb), dim3(256), lds, s->stream, s->hist, g,
-                           (const T*)s->kern, (T)ms, (T)ds, (cplx<T>*)s->spec_in, rho);
+        go(k_contract_hist<T, 6>);
     else if (s->nm == 9)
-        hipLaunchKernelGGL((k_contract_hist<T, 9>), dim3(nb), dim3(256), lds, s->stream, s->hist, g,
-                           (const T*)s->kern, (T)ms, (T)ds, (cplx<T>*)s->spec_in, rho);
+        go(k_contract_hist<T, 9>);
     else
-        hipLaunchKernelGGL((k_contract_hist<T, 12>), dim3(nb), dim3(256), lds, s->stream, s->hist, g,
-                           (const T*)s->kern, (T)ms, (T)ds, (cplx<T>*)s->spec_in, rho);
+        go(k_contract_hist<T, 12>);
 }
 
 template <typename T>
@@ -382,6 +392,7 @@ static hipError_t launch_bin(ss_sim* s) {
         const size_t lds = (size_t)s->psort_lds;
         auto bin = [&](auto cc) {
             constexpr bool C = decltype(cc)::value;
+            allow_lds(s->lds_max, (const void*)k_bin_tile<T, C>, lds);
             hipLaunchKernelGGL((k_bin_tile<T, C>), grid, dim3(256), lds, s->stream, st, (const uint32_t*)s->toff, g,
                                s->tg, s->hist, s->strays, s->scnt, s->tcnt);
             hipLaunchKernelGGL((k_bin_strays<C>), grid, dim3(64), 0, s->stream, (const uint32_t*)s->strays,
@@ -584,23 +595,25 @@ static void launch_tile_kernel(ss_sim* s, const TileArgs<T, NM>& a) {
     const int per = (s->tg.ntiles + s->tg.nxcd - 1) / s->tg.nxcd;
     const dim3 grid((unsigned)(per * s->tg.nxcd));
     const size_t lds = tile_lds_bytes(s->tg.ws, NM, s->na);
+    auto go = [&](auto kern) {
+        allow_lds(s->lds_max, (const void*)kern, lds);
+        hipLaunchKernelGGL(kern, grid, dim3(256), lds, s->stream, a);
+    };
     // the default physics (no magnetic, strain, density or quasi-2D terms)
     // with those terms compiled out
     if constexpr (MOVE && NZ == NZ_PHILOX && NM == 6) {
         if (!full_physics(a.P)) {
             if (a.dtoff)
-                hipLaunchKernelGGL((k_tile_update<T, NM, MOVE, NZ, KEEP, true, false>), grid, dim3(256), lds,
-                                   s->stream, a);
+                go(k_tile_update<T, NM, MOVE, NZ, KEEP, true, false>);
             else
-                hipLaunchKernelGGL((k_tile_update<T, NM, MOVE, NZ, KEEP, false, false>), grid, dim3(256), lds,
-                                   s->stream, a);
+                go(k_tile_update<T, NM, MOVE, NZ, KEEP, false, false>);
             return;
         }
     }
     if (MOVE && a.dtoff)
-        hipLaunchKernelGGL((k_tile_update<T, NM, MOVE, NZ, KEEP, MOVE>), grid, dim3(256), lds, s->stream, a);
+        go(k_tile_update<T, NM, MOVE, NZ, KEEP, MOVE>);
     else
-        hipLaunchKernelGGL((k_tile_update<T, NM, MOVE, NZ, KEEP, false>), grid, dim3(256), lds, s->stream, a);
+        go(k_tile_update<T, NM, MOVE, NZ, KEEP, false>);
 }
 
 template <typename T, int NM, bool MOVE>
@@ -1248,8 +1261,7 @@ static int choose_tiles(ss_sim* s) {
     }
     tg.nxcd = 8;
     s->resort = s->p.resort_interval > 0 ? s->p.resort_interval : 16;
-    const size_t lds = (size_t)tg.ws * (s->nm + 1) * 8 + (size_t)s->nm * s->na * 8;
-    if (lds > 65536) {
+    if (tile_lds_bytes(tg.ws, s->nm, s->na) > (size_t)s->lds_max) {
         if (s->p.binning == SS_BIN_SORTED)
             return fail(SS_ERR_UNSUPPORTED, "tile window and angle bins exceed the LDS of the tiled binning");
         s->sorted = false;
@@ -1259,9 +1271,10 @@ static int choose_tiles(ss_sim* s) {
 
 // Tile geometry of the plain path with tile-ordered particles: tiles of
 // 4 x 4 x 2 cells (clipped to the grid), halved along their longest axis until
-// the tile's counts ([cells][angles] u32) fit 64 KiB of LDS.  No halo.  False
-// when even one cell does not fit, or when a count-grid index does not fit
-// 32 bits (the drifted particles' list holds them).
+// the tile's counts ([cells][angles] u32) fit 64 KiB of LDS (one cell may use
+// up to the device's limit).  No halo.  False when even one cell does not
+// fit, or when a count-grid index does not fit 32 bits (the drifted
+// particles' list holds them).
 static bool choose_plain_tiles(ss_sim* s) {
     if ((uint64_t)s->ns * (uint64_t)s->na >= (1ull << 32)) return false;
     const int n[3] = {s->nx, s->ny, s->nz};
@@ -1271,9 +1284,10 @@ static bool choose_plain_tiles(ss_sim* s) {
         int best = -1;
         for (int a = 0; a < 3; ++a)
             if (t[a] > 1 && (best < 0 || t[a] > t[best])) best = a;
-        if (best < 0) return false;
+        if (best < 0) break;
         t[best] /= 2;
     }
+    if (lds() > (size_t)s->lds_max) return false;
     TileGeo& tg = s->tg;
     tg.ntiles = 1;
     for (int a = 0; a < 3; ++a) {
@@ -1360,6 +1374,11 @@ int ss_create(const ss_params* params, ss_sim** out) {
     ss_sim* s = new ss_sim();
     s->p = *params;
     const ss_params& p = s->p;
+    {
+        int lim = 0;
+        if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, p.device) == hipSuccess && lim > 0)
+            s->lds_max = lim;
+    }
     s->prec = p.precision;
     s->esz = s->prec == SS_FP64 ? 8 : 4;
     s->sorted = p.binning == SS_BIN_SORTED || (p.binning == SS_BIN_AUTO && p.precision == SS_FP32);
@@ -1396,7 +1415,7 @@ int ss_create(const ss_params* params, ss_sim** out) {
             return rc2;
         }
     }
-    if ((size_t)s->nm * s->na * s->esz > 65536) {
+    if ((size_t)s->nm * s->na * s->esz > (size_t)s->lds_max) {
         delete s;
         return fail(SS_ERR_UNSUPPORTED, "too many angle bins for the LDS-resident moment kernels");
     }
@@ -1404,7 +1423,8 @@ int ss_create(const ss_params* params, ss_sim** out) {
     if (s->psort && !choose_plain_tiles(s)) {
         if (p.binning == SS_BIN_PLAIN_TILED) {
             delete s;
-            return fail(SS_ERR_UNSUPPORTED, "one cell's angle counts and the moment kernels exceed 64 KiB of LDS");
+            return fail(SS_ERR_UNSUPPORTED, "one cell's angle counts exceed the LDS of a workgroup, or the count "
+                                            "grid has 2^32 bins or more");
         }
         s->psort = false;
     }
@@ -2205,27 +2225,25 @@ static void ens_launch_update(ss_ensemble* e, int slot, bool resort, int k) {
     const size_t lds = e->lds + sizeof(int) * (n + 1);
     const int nj = (int)e->lds;
     const uint32_t ko = (uint32_t)k;
+    auto go = [&](auto kern) {
+        allow_lds(e->m[0]->lds_max, (const void*)kern, lds);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, e->stream, jobs, e->d_start, n, nj, ko);
+    };
     if (e->ext) {
         if (resort)
-            hipLaunchKernelGGL((k_tile_update_batched<T, NM, true, true, true>), dim3(grid), dim3(256), lds,
-                               e->stream, jobs, e->d_start, n, nj, ko);
+            go(k_tile_update_batched<T, NM, true, true, true>);
         else
-            hipLaunchKernelGGL((k_tile_update_batched<T, NM, true, true, false>), dim3(grid), dim3(256), lds,
-                               e->stream, jobs, e->d_start, n, nj, ko);
+            go(k_tile_update_batched<T, NM, true, true, false>);
     } else if (NM == 6 && !e->full) {  // every member on the default physics
         if (resort)
-            hipLaunchKernelGGL((k_tile_update_batched<T, NM, true, NZ_PHILOX, true, false>), dim3(grid), dim3(256),
-                               lds, e->stream, jobs, e->d_start, n, nj, ko);
+            go(k_tile_update_batched<T, NM, true, NZ_PHILOX, true, false>);
         else
-            hipLaunchKernelGGL((k_tile_update_batched<T, NM, true, NZ_PHILOX, false, false>), dim3(grid), dim3(256),
-                               lds, e->stream, jobs, e->d_start, n, nj, ko);
+            go(k_tile_update_batched<T, NM, true, NZ_PHILOX, false, false>);
     } else {
         if (resort)
-            hipLaunchKernelGGL((k_tile_update_batched<T, NM, true, false, true>), dim3(grid), dim3(256), lds,
-                               e->stream, jobs, e->d_start, n, nj, ko);
+            go(k_tile_update_batched<T, NM, true, false, true>);
         else
-            hipLaunchKernelGGL((k_tile_update_batched<T, NM, true, false, false>), dim3(grid), dim3(256), lds,
-                               e->stream, jobs, e->d_start, n, nj, ko);
+            go(k_tile_update_batched<T, NM, true, false, false>);
     }
 }
 
@@ -2601,10 +2619,10 @@ int ss_ensemble_create(const ss_params* params, int nsys, ss_ensemble** out) {
     e->total_blocks = start.back();
     e->lds = (lds + 15) / 16 * 16;
     // the persistent update keeps the member table beside the largest window
-    if (e->lds + sizeof(int) * (nsys + 1) > 65536)
+    if (e->lds + sizeof(int) * (nsys + 1) > (size_t)e->m[0]->lds_max)
         return bail(fail(SS_ERR_UNSUPPORTED,
-                         "ensemble: the largest member tile window plus the member table exceed 64 KiB of LDS; "
-                         "use smaller tiles"));
+                         "ensemble: the largest member tile window plus the member table exceed the LDS of a "
+                         "workgroup; use smaller tiles"));
     e->job_bytes = e->prec == SS_FP32 ? (e->nm == 6 ? sizeof(TileArgs<float, 6>) : e->nm == 9 ? sizeof(TileArgs<float, 9>) : sizeof(TileArgs<float, 12>))
                                       : (e->nm == 6 ? sizeof(TileArgs<double, 6>) : e->nm == 9 ? sizeof(TileArgs<double, 9>) : sizeof(TileArgs<double, 12>));
     if (hipMalloc(&e->d_jobs, e->job_bytes * 4 * nsys) != hipSuccess ||

@@ -8,7 +8,8 @@
 * a batched ensemble over the 32^3 x 8 x 8, 48^3 x 12 x 12 and 64^3 x 16 x 16
   classes (box 0.78 x grid points, alpha and D_r spread) against the oracle;
 * every optional term of the Langevin chain and of the stress in whole FP64
-  steps, on the plain and the tiled path.
+  steps, on the plain and the tiled path;
+* a 32 x 32 angle grid whose kernels need more than 64 KiB of LDS.
 
 Reference: Simulation::do_timestep, src/bin/simulation/mod.rs:251-374.
 Bars: counts bit-exact, FP64 fields <= 1e-10 relative, FP64 states <= 1e-9;
@@ -304,4 +305,34 @@ def test_optional_physics_whole_steps_fp64(oracle, binning, variant):
         be, gbe = osim.magnetic_field()
         assert rel_err(b, be) < FP64_TOL
         assert rel_err(gb, gbe) < FP64_TOL
+    _assert_fp64_states(sim.get_particles(), osim.particles(), box)
+
+
+# ---------------------------------------------------------------------------
+# Angle grids whose LDS needs exceed 64 KiB per workgroup
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("binning", [ss.BIN_HISTOGRAM, ss.BIN_SORTED, ss.BIN_PLAIN_TILED])
+def test_fine_angle_grid_beyond_64k_lds_fp64(oracle, binning):
+    """32 x 32 angle bins with magnetic stress (nine moment components): the
+    tiled update's window and kernels take 91 KiB of LDS, the contraction's
+    kernels 72 KiB, so both run with the raised per-kernel limit (gfx950:
+    160 KiB) instead of being refused or falling back.  Whole FP64 steps
+    against the oracle: counts exact, fields <= 1e-10, states <= 1e-9."""
+    box = (8.0, 8.0, 8.0)
+    grid = (8, 8, 8, 32, 32)
+    n = 4096
+    kw = dict(stress_magnetic=0.7)
+    p = _params(box, grid, n, 9, binning=binning, flags=ss.FLAG_KEEP_DISTRIBUTION, **kw)
+    sim = ss.Simulation(p)
+    info = sim.get_info()
+    assert info["sorted"] == (binning == ss.BIN_SORTED)
+    assert info["plain_tiled"] == (binning == ss.BIN_PLAIN_TILED)
+    sim.close()
+    parts = ss.initial_condition(_params(box, grid, n, 9), amplitude=0.2, seed=9)
+    sim, osim = _run_pair(oracle, p, parts, 3, _params(box, grid, n, 9, **kw))
+    np.testing.assert_array_equal(sim.get_counts(), osim.counts())
+    u, g = sim.get_flow_field(with_gradient=True)
+    ue, ge = osim.flow_field()
+    assert rel_err(u, ue) < FP64_TOL
+    assert rel_err(g, ge) < FP64_TOL
     _assert_fp64_states(sim.get_particles(), osim.particles(), box)
