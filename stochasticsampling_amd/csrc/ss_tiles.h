@@ -238,13 +238,12 @@ __device__ inline void load_kf(const long long* __restrict__ kfix, int na, long 
 }
 
 // One system: workgroup `blockIdx.x` moves and bins the particles of one
-// tile (tile_for_block).  The default-physics variant asks for three waves
-// per SIMD: the FP64 build then fits 168 VGPRs with a few spilled (0.83 ->
-// 0.78 ms per update at 2^24 particles, 64^3 x 16 x 16, against two waves at
-// 183 VGPRs); the FP32 build needs fewer registers and does not change.
+// tile (tile_for_block).  No occupancy request: asking the FP64 build for
+// three waves per SIMD fits it into 168 VGPRs only with 14 spilled to
+// scratch, 2% slower than two waves at 194 VGPRs without spills (2^24
+// particles, 64^3 x 16 x 16); the FP32 build runs five waves either way.
 template <typename T, int NM, bool MOVE, int NZ, bool KEEP, bool SCATTER, bool FULL = true>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? 1 : 3)))
-    k_tile_update(TileArgs<T, NM> a) {
+__global__ void __launch_bounds__(256) k_tile_update(TileArgs<T, NM> a) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     const int tile = tile_for_block(a.tg, blockIdx.x);
     if (tile >= a.tg.ntiles) return;
