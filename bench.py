@@ -139,7 +139,8 @@ def state_error_study(parts, steps):
 def fft_variant_study(parts, steps):
     """The spectral-stage variants of the FP32 step on the same workload:
     fused radix kernels (the default), radix axis passes, and dense DFT
-    products on the matrix cores (3-pass split bf16) in axis passes.  Timing:
+    products on the matrix cores (three-part split bf16, six products) in
+    axis passes.  Timing:
     mean ms per step of each stage over `steps` steps (HIP events).  Accuracy:
     u and grad u of the spectral solve (SpectralSolver::mean_flow_field) in
     FP32 with the variant's transforms against the FP64 solve, both from the

@@ -57,7 +57,7 @@ extern "C" {
 /* FFT variant used by the spectral solve. */
 #define SS_FFT_AUTO 0  /* autotuned among the variants that pass the accuracy gate */
 #define SS_FFT_RADIX 1 /* LDS-staged mixed-radix Stockham                          */
-#define SS_FFT_DFT 2   /* dense DFT matrix product (3-pass split-bf16 on MFMA)     */
+#define SS_FFT_DFT 2   /* dense DFT matrix product on MFMA (3-part split bf16) */
 
 /* Flags. */
 #define SS_FLAG_QUASI2D 1u            /* z = 0, theta = pi/2 after every step (feature "quasi2d") */

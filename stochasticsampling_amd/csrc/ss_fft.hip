@@ -156,14 +156,18 @@ __global__ void __launch_bounds__(256) k_fft_axis(cplx<T>* __restrict__ data, Ax
 // vector x of n points, stored as the real 2n-vector [Re x; Im x], and the
 // transform is the real matrix product [Re X; Im X] = M [Re x; Im x] with
 // M = [[C, -s S], [s S, C]], C = cos(2 pi jk / n), S = sin(2 pi jk / n).  It
-// runs on the matrix cores with v_mfma_f32_32x32x8bf16_1k in three passes,
-// M_hi x_hi + M_hi x_lo + M_lo x_hi, where hi = bf16(v) and lo = bf16(v - hi),
-// accumulated in FP32 (about 16 significant bits per product).  Lane layout
-// of the instruction (checked on gfx950): A[i][k] and B[k][j] hold
-// i (or j) = lane % 32, k = 4 (lane / 32) + t; D[i][j] in register v holds
-// j = lane % 32, i = 8 (v / 4) + 4 (lane / 32) + v % 4.
+// runs on the matrix cores with gfx950's v_mfma_f32_32x32x16_bf16 (16-deep
+// products).  Each factor is split into three bf16 parts, v = h + m + l with
+// h = bf16(v), m = bf16(v - h), l = bf16(v - h - m) (24 significant bits),
+// and the six products whose parts weigh 2^-16 or more (hh, hm, mh, hl, lh,
+// mm) are accumulated in FP32, so each product keeps about 24 bits, the
+// accuracy of the radix FFT; three passes (hh, hm, mh over two-part splits)
+// gave 8e-6 relative in u at 64^3 in the same time, the axis passes being
+// bound by their memory traffic.  Lane layout: A[i][k] and B[k][j] hold
+// i (or j) = lane % 32, k = 8 (lane / 32) + t, t = 0..7; D[i][j] in register v
+// holds j = lane % 32, i = 8 (v / 4) + 4 (lane / 32) + v % 4.
 // ---------------------------------------------------------------------------
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int DFT_LINES = 32;  // lines (matrix columns) per workgroup
@@ -174,11 +178,13 @@ __device__ inline short bf16_rne(float x) {
     return (short)(b >> 16);
 }
 __device__ inline float bf16_value(short h) { return __uint_as_float(((uint32_t)(uint16_t)h) << 16); }
+__device__ inline __bf16 bf16_of_bits(short h) { return __builtin_bit_cast(__bf16, h); }
 
 // One tile of DFT_LINES lines; four wavefronts, wavefront w computes output
 // rows 32 w .. 32 w + 31 (rows <= 128).
 __global__ void __launch_bounds__(256) k_dft_axis(cplx<float>* __restrict__ data, AxisDesc d,
-                                                  const bf16x4* __restrict__ mh, const bf16x4* __restrict__ ml,
+                                                  const bf16x8* __restrict__ mh, const bf16x8* __restrict__ mm,
+                                                  const bf16x8* __restrict__ ml,
                                                   int rows, int kdim) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     float* xs = reinterpret_cast<float*>(smem_raw);  // [DFT_LINES][kdim + 4]
@@ -208,25 +214,34 @@ __global__ void __launch_bounds__(256) k_dft_axis(cplx<float>* __restrict__ data
     }
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int ksteps = kdim / 8;
+    const int ksteps = kdim / 16;
     f32x16 acc = {};
     const bool active = wave * 32 < rows;
     if (active) {
         for (int ks = 0; ks < ksteps; ++ks) {
-            const bf16x4 ah = mh[(wave * ksteps + ks) * 64 + lane];
-            const bf16x4 al = ml[(wave * ksteps + ks) * 64 + lane];
-            const float* xp = xs + (lane & 31) * ldx + ks * 8 + 4 * (lane >> 5);
-            bf16x4 bh, bl;
+            const int fr = (wave * ksteps + ks) * 64 + lane;
+            const bf16x8 ah = mh[fr], am = mm[fr], al = ml[fr];
+            const float* xp = xs + (lane & 31) * ldx + ks * 16 + 8 * (lane >> 5);
+            bf16x8 bh, bm, bl;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
+            for (int t = 0; t < 8; ++t) {
                 const float x = xp[t];
                 const short h = bf16_rne(x);
-                bh[t] = h;
-                bl[t] = bf16_rne(x - bf16_value(h));
+                const float r = x - bf16_value(h);
+                const short m = bf16_rne(r);
+                bh[t] = bf16_of_bits(h);
+                bm[t] = bf16_of_bits(m);
+                bl[t] = bf16_of_bits(bf16_rne(r - bf16_value(m)));
             }
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, bh, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, bl, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(al, bh, acc, 0, 0, 0);
+            // smallest terms first
+            f32x16 lo = {};
+            lo = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am, bm, lo, 0, 0, 0);
+            lo = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, lo, 0, 0, 0);
+            lo = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, lo, 0, 0, 0);
+            lo = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am, bh, lo, 0, 0, 0);
+            lo = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bm, lo, 0, 0, 0);
+            acc += lo;
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
         }
     }
     __syncthreads();  // every wavefront has read its inputs
@@ -267,16 +282,16 @@ static double bf16_double(uint16_t h) {
 }
 
 // DFT matrices of one axis for both signs, split into bf16 high and low parts
-// in fragment order [row tile][k step][lane][4].
+// in fragment order [row tile][k step][lane][8].
 static int build_dft(FftAxis& a) {
     const int n = a.n;
     a.dft_rows = (2 * n + 31) / 32 * 32;
-    a.dft_k = (2 * n + 7) / 8 * 8;
-    const int rt = a.dft_rows / 32, ks = a.dft_k / 8;
-    const size_t cnt = (size_t)rt * ks * 64 * 4;
+    a.dft_k = (2 * n + 15) / 16 * 16;
+    const int rt = a.dft_rows / 32, ks = a.dft_k / 16;
+    const size_t cnt = (size_t)rt * ks * 64 * 8;
     for (int sg = 0; sg < 2; ++sg) {
         const double s = sg == 0 ? -1.0 : 1.0;
-        std::vector<uint16_t> hi(cnt), lo(cnt);
+        std::vector<uint16_t> hi(cnt), mid(cnt), lo(cnt);
         auto m = [&](int i, int k) -> double {
             if (i >= 2 * n || k >= 2 * n) return 0.0;
             const int e = i % n, j = k % n;
@@ -288,17 +303,21 @@ static int build_dft(FftAxis& a) {
         for (int r = 0; r < rt; ++r)
             for (int q = 0; q < ks; ++q)
                 for (int l = 0; l < 64; ++l)
-                    for (int t = 0; t < 4; ++t) {
-                        const double v = m(r * 32 + l % 32, q * 8 + 4 * (l / 32) + t);
+                    for (int t = 0; t < 8; ++t) {
+                        const double v = m(r * 32 + l % 32, q * 16 + 8 * (l / 32) + t);
                         const uint16_t h = bf16_bits_rne((float)v);
-                        const size_t idx = (((size_t)r * ks + q) * 64 + l) * 4 + t;
+                        const double rest = v - bf16_double(h);
+                        const uint16_t md = bf16_bits_rne((float)rest);
+                        const size_t idx = (((size_t)r * ks + q) * 64 + l) * 8 + t;
                         hi[idx] = h;
-                        lo[idx] = bf16_bits_rne((float)(v - bf16_double(h)));
+                        mid[idx] = md;
+                        lo[idx] = bf16_bits_rne((float)(rest - bf16_double(md)));
                     }
-        SS_HIP_CHECK(hipMalloc(&a.dft[sg][0], cnt * 2));
-        SS_HIP_CHECK(hipMalloc(&a.dft[sg][1], cnt * 2));
-        SS_HIP_CHECK(hipMemcpy(a.dft[sg][0], hi.data(), cnt * 2, hipMemcpyHostToDevice));
-        SS_HIP_CHECK(hipMemcpy(a.dft[sg][1], lo.data(), cnt * 2, hipMemcpyHostToDevice));
+        const std::vector<uint16_t>* parts[3] = {&hi, &mid, &lo};
+        for (int h = 0; h < 3; ++h) {
+            SS_HIP_CHECK(hipMalloc(&a.dft[sg][h], cnt * 2));
+            SS_HIP_CHECK(hipMemcpy(a.dft[sg][h], parts[h]->data(), cnt * 2, hipMemcpyHostToDevice));
+        }
     }
     return SS_OK;
 }
@@ -353,7 +372,7 @@ void Fft3d::release() {
             ax[a].tw = nullptr;
         }
         for (int sg = 0; sg < 2; ++sg)
-            for (int h = 0; h < 2; ++h)
+            for (int h = 0; h < 3; ++h)
                 if (ax[a].dft[sg][h]) {
                     (void)hipFree(ax[a].dft[sg][h]);
                     ax[a].dft[sg][h] = nullptr;
@@ -392,7 +411,8 @@ static int run_dft_axis(cplx<float>* data, const FftAxis& fa, int axis, int nx, 
     const int sg = sign < 0 ? 0 : 1;
     const size_t lds = sizeof(float) * DFT_LINES * (fa.dft_k + 4);
     hipLaunchKernelGGL(k_dft_axis, dim3((unsigned)(outers * d.tiles)), dim3(256), lds, stream, data, d,
-                       reinterpret_cast<const bf16x4*>(fa.dft[sg][0]), reinterpret_cast<const bf16x4*>(fa.dft[sg][1]),
+                       reinterpret_cast<const bf16x8*>(fa.dft[sg][0]), reinterpret_cast<const bf16x8*>(fa.dft[sg][1]),
+                       reinterpret_cast<const bf16x8*>(fa.dft[sg][2]),
                        fa.dft_rows, fa.dft_k);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(SS_ERR_DEVICE, std::string("dft launch: ") + hipGetErrorString(e));

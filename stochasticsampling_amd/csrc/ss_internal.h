@@ -33,10 +33,11 @@ struct FftAxis {
     int fac[16] = {0};
     void* tw = nullptr;  // n complex twiddles exp(-2 pi i t / n) in the plan precision
     // dense-DFT variant: the real 2n x 2n DFT matrix of each sign split into
-    // bf16 high and low parts, in MFMA fragment order ([sign][hi/lo])
-    void* dft[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    // three bf16 parts (high, middle, low), in MFMA fragment order
+    // ([sign][part])
+    void* dft[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
     int dft_rows = 0;  // 2n rounded up to 32 (output rows)
-    int dft_k = 0;     // 2n rounded up to 8 (reduction length)
+    int dft_k = 0;     // 2n rounded up to 16 (reduction length)
 };
 
 struct Fft3d {

@@ -42,10 +42,10 @@ def test_fft_plan_matches_numpy(shape, precision, direction):
 @pytest.mark.parametrize("shape", [(16, 16, 16), (64, 64, 64), (32, 48, 6), (8, 12, 10), (96, 16, 7)])
 @pytest.mark.parametrize("direction", [-1, 1])
 def test_fft_plan_dense_dft_variant(shape, direction):
-    """Dense DFT products on the matrix cores (3-pass split bf16, FP32
-    accumulation) for axes of at most 64 points, radix stages for longer axes:
-    the 3-pass split keeps about 16 significant bits per product, so the bound
-    is 1e-4 of the largest output."""
+    """Dense DFT products on the matrix cores (three-part bf16 splits, six
+    products, FP32 accumulation) for axes of at most 64 points, radix stages
+    for longer axes: about 24 significant bits per product, the radix FFT's
+    FP32 bound of 2e-6 of the largest output."""
     rng = np.random.default_rng(2)
     batch = 2
     x = rng.standard_normal((batch,) + shape) + 1j * rng.standard_normal((batch,) + shape)
@@ -54,7 +54,7 @@ def test_fft_plan_dense_dft_variant(shape, direction):
     plan.reexecute3d(a)
     axes = (1, 2, 3)
     ref = np.fft.fftn(x, axes=axes) if direction == -1 else np.fft.ifftn(x, axes=axes) * np.prod(shape)
-    assert rel_err(a, ref) < 1e-4
+    assert rel_err(a, ref) < 2e-6
 
 
 def test_fft_plan_dense_dft_variant_is_fp32_only():
@@ -583,8 +583,7 @@ def test_fp32_fields_within_target_of_fp64(oracle):
 
 def test_fp32_dense_dft_fields_against_fp64(oracle):
     """The FP32 step with the dense-DFT FFT variant against the FP64 oracle:
-    u and grad u within 2e-4 relative (the radix variant meets 1e-5, so the
-    automatic choice keeps the radix stages)."""
+    u and grad u within the 1e-5 FP32 target, as the radix variant."""
     kw = dict(grid=(16, 16, 16, 8, 8), n=1 << 15)
     p32 = small_params(precision=ss.FP32, fft_variant=ss.FFT_DFT, flags=ss.FLAG_KEEP_DISTRIBUTION, **kw)
     p64 = small_params(precision=ss.FP64, **kw)
@@ -600,8 +599,8 @@ def test_fp32_dense_dft_fields_against_fp64(oracle):
     assert moved <= 20, moved
     u, g = sim.get_flow_field(with_gradient=True)
     ue, ge = oracle.mean_flow_field(p64, 0.0, sim.get_distribution())
-    assert rel_err(u, ue) < 2e-4
-    assert rel_err(g, ge) < 2e-4
+    assert rel_err(u, ue) < 1e-5
+    assert rel_err(g, ge) < 1e-5
 
 
 def test_snapshot_resume_is_exact():
