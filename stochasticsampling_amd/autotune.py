@@ -80,7 +80,7 @@ def candidates(p, tiles=TILES, min_tiles=256):
         eff = tuple(a[0] for a in ax)
         ws = ax[0][1] * ax[1][1] * ax[2][1]
         ntiles = ax[0][2] * ax[1][2] * ax[2][2]
-        if eff in seen or ws * (nm + 1) * 8 + nm * na * 8 > LDS_LIMIT or ntiles < min_tiles:
+        if eff in seen or (ws * (nm + 1) * 8 + 15) // 16 * 16 + ((nm + 1) & ~1) * na * 8 > LDS_LIMIT or ntiles < min_tiles:
             continue
         seen.add(eff)
         out.append(eff)

@@ -105,8 +105,25 @@ struct Math<float> {
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
 // ---------------------------------------------------------------------------
-__host__ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
-                                              uint32_t k0, uint32_t k1, uint32_t out[4]) {
+
+// a ^ b ^ k; on the device one gfx950 v_bitop3_b32 with truth table 0x96
+// (odd parity of the three inputs).  k must be uniform across the wavefront
+// (the Philox key: the seed and round constants), so that it can be the
+// instruction's scalar operand; the compiler otherwise splits the three-way
+// xor into two two-way ones.  (The host pass, which draws the initial
+// conditions, uses the plain expression.)
+__host__ __device__ inline uint32_t xor3_key(uint32_t a, uint32_t b, uint32_t k) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t r;
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(a), "v"(b), "s"(k));
+    return r;
+#else
+    return (a ^ b) ^ k;
+#endif
+}
+
+__host__ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                     uint32_t k1, uint32_t out[4]) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         if (r > 0) {
@@ -115,8 +132,8 @@ __host__ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
         }
         uint64_t p0 = (uint64_t)0xD2511F53u * c0;
         uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-        uint32_t n0 = ((uint32_t)(p1 >> 32) ^ c1) ^ k0;
-        uint32_t n2 = ((uint32_t)(p0 >> 32) ^ c3) ^ k1;
+        uint32_t n0 = xor3_key((uint32_t)(p1 >> 32), c1, k0);
+        uint32_t n2 = xor3_key((uint32_t)(p0 >> 32), c3, k1);
         c1 = (uint32_t)p1;
         c3 = (uint32_t)p0;
         c0 = n0;
@@ -242,6 +259,7 @@ __device__ inline T modulo(T f, T m) {
 // modulo() (fmod is exact, and so is f - m for m <= f < 2m).
 template <typename T>
 __device__ inline T modulo_near(T f, T m) {
+    if (f >= T(0) && f < m) return f;
     if (f >= m) {
         const T r = f - m;
         return r < m ? r : modulo(f, m);
@@ -299,10 +317,13 @@ __device__ inline int grid_index_r(T q, T g, T rg, T tol, int n) {
     T f = Math<T>::floor(t);
     const T fr = t - f;
     if (!(fr >= tol && fr <= T(1) - tol)) f = Math<T>::floor(q / g);
-    int i = (int)f;
-    if (f >= (T)n) i = n - 1;
-    if (f < T(0)) i = 0;
-    return i;
+    // clamp to [0, n - 1] before the conversion (an index equal to n folds to
+    // n - 1; NaN gives 0)
+    if constexpr (sizeof(T) == 4)
+        f = __builtin_fminf(__builtin_fmaxf(f, 0.0f), (float)(n - 1));
+    else
+        f = __builtin_fmin(__builtin_fmax(f, 0.0), (double)(n - 1));
+    return (int)f;
 }
 
 // ---------------------------------------------------------------------------

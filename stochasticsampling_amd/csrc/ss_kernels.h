@@ -347,17 +347,19 @@ __device__ inline float add_wrap<float>(float x, float d, float L, float* lo) {
     float t = x + y;
     float bp = t - x;
     float l = (x - (t - bp)) + (y - bp);  // TwoSum: t + l == x + y exactly
-    // wrap on the value t + l, so that it lands in [0, L); t itself may equal L
-    if (t > L || (t == L && l >= 0.0f)) {
-        t = t - L;  // exact (Sterbenz) for t < 2L
-    } else if (t < 0.0f || (t == 0.0f && l < 0.0f)) {
-        float s = t + L;
-        l = l + (t - (s - L));
-        t = s;
-    }
-    if (!(t >= 0.0f && t <= L)) {  // increment of a box length or more
-        t = modulo(t, L);
-        l = 0.0f;
+    if (!(t > 0.0f && t < L)) {  // on or beyond an edge of the box (rare)
+        // wrap on the value t + l, so that it lands in [0, L); t itself may equal L
+        if (t > L || (t == L && l >= 0.0f)) {
+            t = t - L;  // exact (Sterbenz) for t < 2L
+        } else if (t < 0.0f || (t == 0.0f && l < 0.0f)) {
+            float s = t + L;
+            l = l + (t - (s - L));
+            t = s;
+        }
+        if (!(t >= 0.0f && t <= L)) {  // increment of a box length or more
+            t = modulo(t, L);
+            l = 0.0f;
+        }
     }
     *lo = l;
     return t;

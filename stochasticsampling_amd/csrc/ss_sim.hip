@@ -203,6 +203,7 @@ struct ss_sim {
     uint32_t* inv = nullptr;
     bool last_resort = false;  // the last step's update changed the storage order
     bool in_ensemble = false;  // borrowed member of an ensemble: stepped only by it
+    bool no_stage = false;     // tiled update: cell records read from global memory, not staged in LDS
     // tiled path
     TileGeo tg;
     int resort = 16;             // steps between tile sorts
@@ -590,11 +591,20 @@ static hipError_t launch_spec_planes_inv(ss_sim* s) {
     });
 }
 
+// Bytes per window slot of the tiled update's staged cell records, or 0 when
+// the records are read from global memory (full 32-value records, a window
+// that leaves no LDS for them, or switched off for an ensemble).
+static size_t tile_rec_bytes(const ss_sim* s) {
+    if (s->rs != REC_SMALL || s->no_stage) return 0;
+    const size_t b = (size_t)REC_SMALL * (s->prec == SS_FP32 ? sizeof(float) : sizeof(double));
+    return tile_lds_bytes(s->tg.ws, s->nm, s->na, b) <= (size_t)s->lds_max ? b : 0;
+}
+
 template <typename T, int NM, bool MOVE, int NZ, bool KEEP>
 static void launch_tile_kernel(ss_sim* s, const TileArgs<T, NM>& a) {
     const int per = (s->tg.ntiles + s->tg.nxcd - 1) / s->tg.nxcd;
     const dim3 grid((unsigned)(per * s->tg.nxcd));
-    const size_t lds = tile_lds_bytes(s->tg.ws, NM, s->na);
+    const size_t lds = tile_lds_bytes(s->tg.ws, NM, s->na, a.stage ? tile_rec_bytes(s) : 0);
     auto go = [&](auto kern) {
         allow_lds(s->lds_max, (const void*)kern, lds);
         hipLaunchKernelGGL(kern, grid, dim3(256), lds, s->stream, a);
@@ -654,6 +664,7 @@ static TileArgs<T, NM> tile_args(ss_sim* s) {
     a.dtoff = s->pending ? s->toff2 : nullptr;
     a.cross = s->cursor;
     a.home = s->home_ctr;
+    a.stage = tile_rec_bytes(s) > 0 ? 1 : 0;
     return a;
 }
 
@@ -2605,7 +2616,7 @@ int ss_ensemble_create(const ss_params* params, int nsys, ss_ensemble** out) {
         return bail(fail(SS_ERR_ALLOC, "hipMalloc"));
     // members run on the side streams and count steps on the shared counter
     std::vector<int> start(1, 0);
-    size_t lds = 0;
+    size_t lds = 0, lds_staged = 0;
     for (int j = 0; j < nsys; ++j) {
         ss_sim* s = e->m[j];
         (void)hipStreamDestroy(s->stream);
@@ -2615,7 +2626,13 @@ int ss_ensemble_create(const ss_params* params, int nsys, ss_ensemble** out) {
         const int per = (s->tg.ntiles + s->tg.nxcd - 1) / s->tg.nxcd;
         start.push_back(start.back() + per * s->tg.nxcd);
         lds = std::max(lds, tile_lds_bytes(s->tg.ws, s->nm, s->na));
+        lds_staged = std::max(lds_staged, tile_lds_bytes(s->tg.ws, s->nm, s->na, tile_rec_bytes(s)));
     }
+    // staged cell records only when they leave room for the member table
+    if ((lds_staged + 15) / 16 * 16 + sizeof(int) * (nsys + 1) <= (size_t)e->m[0]->lds_max)
+        lds = lds_staged;
+    else
+        for (ss_sim* s : e->m) s->no_stage = true;
     e->total_blocks = start.back();
     e->lds = (lds + 15) / 16 * 16;
     // the persistent update keeps the member table beside the largest window

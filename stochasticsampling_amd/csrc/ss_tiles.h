@@ -79,11 +79,24 @@ struct TileArgs {
     const uint32_t* dtoff;  // [ntiles + 1] or nullptr (update in place)
     uint32_t* cross;        // [ntiles] particles placed from a segment's end
     uint32_t* home;         // [ntiles] particles placed from a segment's front
+    int stage;              // 1: the window's cell records are staged in LDS (8-value records)
 };
 
-// Dynamic LDS of the tile update: window accumulators and moment kernels.
-inline size_t tile_lds_bytes(int ws, int nm, int na) {
-    return ((size_t)ws * (nm + 1) * 8 + (size_t)nm * na * 8 + 15) / 16 * 16;
+// Moment kernels in LDS: one row of KS >= NM values per angle bin (an even
+// count, so that a row is read as 16-byte pairs).
+template <int NM>
+constexpr int kf_stride() {
+    return (NM + 1) & ~1;
+}
+
+// Dynamic LDS of the tile update: window accumulators [ws][nm+1], moment
+// kernels [na][KS], and, when staged, the window's cell records [ws][8] of
+// rec_bytes / 8 bytes per value (0: not staged).  Every region starts at a
+// multiple of 16 bytes.
+inline size_t tile_acc_bytes(int ws, int nm) { return ((size_t)ws * (nm + 1) * 8 + 15) / 16 * 16; }
+inline size_t tile_lds_bytes(int ws, int nm, int na, size_t rec_bytes = 0) {
+    const size_t ks = (size_t)((nm + 1) & ~1);
+    return tile_acc_bytes(ws, nm) + ks * na * 8 + (size_t)ws * rec_bytes;
 }
 
 // The particles in slots [q0, q1) of one tile: move them (MOVE) and bin them
@@ -93,24 +106,60 @@ inline size_t tile_lds_bytes(int ws, int nm, int na) {
 // also runs for a tile without particles); a pass that does not own the tile
 // adds its own cells to the inflow accumulators instead.  kf must hold the
 // fixed-point moment kernels; acc is the window.  Ends with a barrier.
+// Grid cell of window slot `slot` of the tile at (ox, oy, oz), periodic;
+// own: the cell belongs to the tile.
+__device__ inline size_t window_cell(const TileGeo& tg, int nx, int ny, int nz, int ox, int oy, int oz, int slot,
+                                     bool* own) {
+    const int lz = slot % tg.w[2], ly = (slot / tg.w[2]) % tg.w[1], lx = slot / (tg.w[2] * tg.w[1]);
+    const int dx = lx - tg.h[0], dy = ly - tg.h[1], dz = lz - tg.h[2];
+    int cx = ox + dx, cy = oy + dy, cz = oz + dz;
+    cx = cx < 0 ? cx + nx : (cx >= nx ? cx - nx : cx);
+    cy = cy < 0 ? cy + ny : (cy >= ny ? cy - ny : cy);
+    cz = cz < 0 ? cz + nz : (cz >= nz ? cz - nz : cz);
+    const int ex = min(tg.t[0], nx - ox), ey = min(tg.t[1], ny - oy), ez = min(tg.t[2], nz - oz);
+    *own = dx >= 0 && dx < ex && dy >= 0 && dy < ey && dz >= 0 && dz < ez;
+    return (size_t)(cx * ny + cy) * nz + cz;
+}
+
+// The 8-value cell record of a staged window slot, from LDS.
+__device__ inline void staged_record(const float* w, int slot, float r[REC_FULL]) {
+    const float4 u = *reinterpret_cast<const float4*>(w + slot * REC_SMALL);
+    const float2 v = *reinterpret_cast<const float2*>(w + slot * REC_SMALL + 4);
+    r[0] = u.x;
+    r[1] = u.y;
+    r[2] = u.z;
+    r[3] = u.w;
+    r[4] = v.x;
+    r[5] = v.y;
+}
+__device__ inline void staged_record(const double* w, int slot, double r[REC_SMALL]) {
+    const double2 u = *reinterpret_cast<const double2*>(w + slot * REC_SMALL);
+    const double2 v = *reinterpret_cast<const double2*>(w + slot * REC_SMALL + 2);
+    const double2 x = *reinterpret_cast<const double2*>(w + slot * REC_SMALL + 4);
+    r[0] = u.x;
+    r[1] = u.y;
+    r[2] = v.x;
+    r[3] = v.y;
+    r[4] = x.x;
+    r[5] = x.y;
+}
+
 template <typename T, int NM, bool MOVE, int NZ, bool KEEP, bool SCATTER, bool FULL = true>
 __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0, uint32_t q1, bool owner,
-                                 long long* acc, const long long* kf) {
+                                 long long* acc, const long long* kf, T* wrec) {
     constexpr int NC = NM + 1;
+    constexpr int KS = kf_stride<NM>();
     const Geo<T>& g = a.g;
     const TileGeo& tg = a.tg;
     const int tz = tile % tg.nt[2], ty = (tile / tg.nt[2]) % tg.nt[1], tx = tile / (tg.nt[2] * tg.nt[1]);
     const int ox = tx * tg.t[0], oy = ty * tg.t[1], oz = tz * tg.t[2];
-    for (int k = threadIdx.x; k < tg.ws * NC; k += blockDim.x) acc[k] = 0;
-    __syncthreads();
 
     const uint32_t p0 = q0, p1 = q1;
-    const uint64_t step = MOVE ? *a.step + a.step_off : 0;
     constexpr bool scatter = MOVE && SCATTER;
-    const uint32_t hbase = scatter ? a.dtoff[tile] : 0u;
     // Software pipeline: the next particle's state is loaded before the
     // current one is updated, so that each wavefront has two iterations of
-    // loads in flight.
+    // loads in flight.  The first particle's loads are issued before the
+    // window is set up, so that they overlap it.
     uint32_t i = p0 + threadIdx.x;
     Loaded<T> cur{};
     uint32_t cid = 0;
@@ -120,6 +169,30 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
         load_particle(a.s, i, cur);
         if constexpr (NZ == NZ_REC) ncur = load_noise_rec(a.noise, i);
     }
+    for (int k = threadIdx.x; k < tg.ws * NC; k += blockDim.x) acc[k] = 0;
+    // the cell records of the window (8-value records): every particle of the
+    // tile that is still inside the window reads its record from LDS
+    const bool stage = MOVE && a.stage;
+    if (stage) {
+        for (int slot = threadIdx.x; slot < tg.ws; slot += blockDim.x) {
+            bool own;
+            const size_t cell = window_cell(tg, g.nx, g.ny, g.nz, ox, oy, oz, slot, &own);
+            const T* src = a.rec + cell * REC_SMALL;
+            T* dst = wrec + slot * REC_SMALL;
+            if constexpr (sizeof(T) == 4) {
+                reinterpret_cast<float4*>(dst)[0] = reinterpret_cast<const float4*>(src)[0];
+                reinterpret_cast<float4*>(dst)[1] = reinterpret_cast<const float4*>(src)[1];
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    reinterpret_cast<double2*>(dst)[k] = reinterpret_cast<const double2*>(src)[k];
+            }
+        }
+    }
+    __syncthreads();
+
+    const uint64_t step = MOVE ? *a.step + a.step_off : 0;
+    const uint32_t hbase = scatter ? a.dtoff[tile] : 0u;
     for (; i < p1; i += blockDim.x) {
         T nx, ny, nz, ophi, otheta;
         if (MOVE) {
@@ -127,8 +200,24 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
             // waiting for it leaves the next particle's loads in flight
             const int cx = bin_x(g, cur.x), cy = bin_y(g, cur.y), cz = bin_z(g, cur.z);
             const T* recp = a.rec + (size_t)((cx * g.ny + cy) * g.nz + cz) * a.rs;
-            float r[REC_FULL];
-            if constexpr (sizeof(T) == 4) gather_record(recp, a.rs, r);
+            T r[REC_FULL];
+            bool from_lds = false;
+            if (stage) {
+                const int wx = window_coord(cx, ox, tg.t[0], tg.h[0], g.nx);
+                const int wy = window_coord(cy, oy, tg.t[1], tg.h[1], g.ny);
+                const int wz = window_coord(cz, oz, tg.t[2], tg.h[2], g.nz);
+                from_lds = wx >= 0 && wy >= 0 && wz >= 0;
+                staged_record(wrec, from_lds ? (wx * tg.w[1] + wy) * tg.w[2] + wz : 0, r);
+            }
+            // outside the window (drifted since the last re-sort), or not staged
+            if (!from_lds) {
+                if constexpr (sizeof(T) == 4) {
+                    gather_record(recp, a.rs, r);
+                } else if (stage) {
+#pragma unroll
+                    for (int k = 0; k < REC_SMALL; ++k) r[k] = recp[k];
+                }
+            }
             // unconditional (clamped to the segment's last slot): a branch
             // here would make the wait for the record a wait for everything
             const uint32_t j = min(i + blockDim.x, p1 - 1);
@@ -163,8 +252,8 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
                 out = a.d;
                 at(out.id, o) = cid;
             }
-            if constexpr (sizeof(T) == 4)
-                move_loaded<T, FULL>(out, o, cur, a.P, reinterpret_cast<const T*>(r), rv, nx, ny, nz, ophi, otheta);
+            if (sizeof(T) == 4 || stage)
+                move_loaded<T, FULL>(out, o, cur, a.P, r, rv, nx, ny, nz, ophi, otheta);
             else
                 move_loaded<T, FULL>(out, o, cur, a.P, recp, rv, nx, ny, nz, ophi, otheta);
             cur = nxt;
@@ -184,17 +273,21 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
         const int wz = window_coord(iz, oz, tg.t[2], tg.h[2], g.nz);
         if (wx >= 0 && wy >= 0 && wz >= 0) {
             long long* dst = acc + (size_t)((wx * tg.w[1] + wy) * tg.w[2] + wz) * NC;
+            const longlong2* kr = reinterpret_cast<const longlong2*>(kf + (size_t)ang * KS);
 #pragma unroll
-            for (int q = 0; q < NM; ++q)
-                atomicAdd(reinterpret_cast<unsigned long long*>(dst + q), (unsigned long long)kf[q * g.na + ang]);
+            for (int q = 0; q < NM; q += 2) {
+                const longlong2 v = kr[q / 2];
+                atomicAdd(reinterpret_cast<unsigned long long*>(dst + q), (unsigned long long)v.x);
+                if (q + 1 < NM) atomicAdd(reinterpret_cast<unsigned long long*>(dst + q + 1), (unsigned long long)v.y);
+            }
             atomicAdd(reinterpret_cast<unsigned long long*>(dst + NM), 1ull);
         } else {
             const int cell = (ix * g.ny + iy) * g.nz + iz;
             long long* dst = a.inflow + cell;
+            const long long* kr = kf + (size_t)ang * KS;
 #pragma unroll
             for (int q = 0; q < NM; ++q)
-                atomicAdd(reinterpret_cast<unsigned long long*>(dst + (size_t)q * g.ns),
-                          (unsigned long long)kf[q * g.na + ang]);
+                atomicAdd(reinterpret_cast<unsigned long long*>(dst + (size_t)q * g.ns), (unsigned long long)kr[q]);
             atomicAdd(reinterpret_cast<unsigned long long*>(dst + (size_t)NM * g.ns), 1ull);
         }
         if (KEEP) {
@@ -207,17 +300,10 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
     // flush: own cells by plain stores when this pass owns the tile (every
     // cell of the grid is owned by exactly one tile, every tile by exactly
     // one pass), other cells with atomics when they received particles
-    const int ex = min(tg.t[0], g.nx - ox), ey = min(tg.t[1], g.ny - oy), ez = min(tg.t[2], g.nz - oz);
     for (int slot = threadIdx.x; slot < tg.ws; slot += blockDim.x) {
-        const int lz = slot % tg.w[2], ly = (slot / tg.w[2]) % tg.w[1], lx = slot / (tg.w[2] * tg.w[1]);
-        const int dx = lx - tg.h[0], dy = ly - tg.h[1], dz = lz - tg.h[2];
-        int cx = ox + dx, cy = oy + dy, cz = oz + dz;
-        cx = cx < 0 ? cx + g.nx : (cx >= g.nx ? cx - g.nx : cx);
-        cy = cy < 0 ? cy + g.ny : (cy >= g.ny ? cy - g.ny : cy);
-        cz = cz < 0 ? cz + g.nz : (cz >= g.nz ? cz - g.nz : cz);
-        const size_t cell = (size_t)(cx * g.ny + cy) * g.nz + cz;
+        bool own;
+        const size_t cell = window_cell(tg, g.nx, g.ny, g.nz, ox, oy, oz, slot, &own);
         const long long* src = acc + (size_t)slot * NC;
-        const bool own = dx >= 0 && dx < ex && dy >= 0 && dy < ey && dz >= 0 && dz < ez;
         if (own && owner) {
 #pragma unroll
             for (int q = 0; q < NC; ++q) a.main[(size_t)q * g.ns + cell] = src[q];
@@ -231,11 +317,25 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
     __syncthreads();
 }
 
-// Moment kernels into LDS (visible after the first barrier of tile_pass).
+// Moment kernels [NM][na] into LDS rows [na][KS] (visible after the first
+// barrier of tile_pass).
 template <int NM>
 __device__ inline void load_kf(const long long* __restrict__ kfix, int na, long long* kf) {
-    for (int k = threadIdx.x; k < NM * na; k += blockDim.x) kf[k] = kfix[k];
+    for (int k = threadIdx.x; k < NM * na; k += blockDim.x) kf[(k % na) * kf_stride<NM>() + k / na] = kfix[k];
 }
+
+// LDS regions of the tile update (tile_lds_bytes).
+template <typename T, int NM>
+struct TileLds {
+    long long* acc;
+    long long* kf;
+    T* wrec;
+    __device__ TileLds(unsigned char* base, int ws, int na) {
+        acc = reinterpret_cast<long long*>(base);
+        kf = reinterpret_cast<long long*>(base + (((size_t)ws * (NM + 1) * 8 + 15) / 16 * 16));
+        wrec = reinterpret_cast<T*>(kf + (size_t)na * kf_stride<NM>());
+    }
+};
 
 // One system: workgroup `blockIdx.x` moves and bins the particles of one
 // tile (tile_for_block).  No occupancy request: asking the FP64 build for
@@ -247,10 +347,10 @@ __global__ void __launch_bounds__(256) k_tile_update(TileArgs<T, NM> a) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     const int tile = tile_for_block(a.tg, blockIdx.x);
     if (tile >= a.tg.ntiles) return;
-    long long* acc = reinterpret_cast<long long*>(smem_raw);  // [ws][NM+1]
-    long long* kf = acc + (size_t)a.tg.ws * (NM + 1);         // [NM][na]
-    load_kf<NM>(a.kfix, a.g.na, kf);
-    tile_pass<T, NM, MOVE, NZ, KEEP, SCATTER, FULL>(a, tile, a.toff[tile], a.toff[tile + 1], true, acc, kf);
+    const TileLds<T, NM> l(smem_raw, a.tg.ws, a.g.na);
+    load_kf<NM>(a.kfix, a.g.na, l.kf);
+    tile_pass<T, NM, MOVE, NZ, KEEP, SCATTER, FULL>(a, tile, a.toff[tile], a.toff[tile + 1], true, l.acc, l.kf,
+                                                    l.wrec);
 }
 
 // Many systems in one persistent launch: work item w of the concatenated
@@ -284,11 +384,10 @@ __global__ void __launch_bounds__(256)
         a.step_off = step_off;
         const int tile = tile_for_block(a.tg, w - sstart[lo]);
         if (tile < a.tg.ntiles) {
-            long long* acc = reinterpret_cast<long long*>(smem_raw);
-            long long* kf = acc + (size_t)a.tg.ws * (NM + 1);
-            load_kf<NM>(a.kfix, a.g.na, kf);
-            tile_pass<T, NM, MOVE, NZ, false, SCATTER, FULL>(a, tile, a.toff[tile], a.toff[tile + 1], true, acc,
-                                                                     kf);
+            const TileLds<T, NM> l(smem_raw, a.tg.ws, a.g.na);
+            load_kf<NM>(a.kfix, a.g.na, l.kf);
+            tile_pass<T, NM, MOVE, NZ, false, SCATTER, FULL>(a, tile, a.toff[tile], a.toff[tile + 1], true, l.acc,
+                                                             l.kf, l.wrec);
         }
     }
 }
