@@ -121,10 +121,27 @@ __device__ inline size_t window_cell(const TileGeo& tg, int nx, int ny, int nz, 
     return (size_t)(cx * ny + cy) * nz + cz;
 }
 
+// Pointers into LDS typed with its address space, so that the hot loop's
+// LDS accesses stay ds_ instructions: with generic pointers the compiler may
+// merge an LDS access and a global one on the other side of a branch into a
+// single flat access, which waits on both counters.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef long long i64x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) long long lds_i64;
+typedef __attribute__((address_space(3))) i64x2 lds_i64x2;
+typedef __attribute__((address_space(3))) float lds_f32;
+typedef __attribute__((address_space(3))) double lds_f64;
+typedef __attribute__((address_space(3))) f32x4 lds_f32x4;
+typedef __attribute__((address_space(3))) f32x2 lds_f32x2;
+typedef __attribute__((address_space(3))) f64x2 lds_f64x2;
+
 // The 8-value cell record of a staged window slot, from LDS.
 __device__ inline void staged_record(const float* w, int slot, float r[REC_FULL]) {
-    const float4 u = *reinterpret_cast<const float4*>(w + slot * REC_SMALL);
-    const float2 v = *reinterpret_cast<const float2*>(w + slot * REC_SMALL + 4);
+    const lds_f32* b = (const lds_f32*)w + slot * REC_SMALL;
+    const f32x4 u = *(const lds_f32x4*)b;
+    const f32x2 v = *(const lds_f32x2*)(b + 4);
     r[0] = u.x;
     r[1] = u.y;
     r[2] = u.z;
@@ -133,9 +150,10 @@ __device__ inline void staged_record(const float* w, int slot, float r[REC_FULL]
     r[5] = v.y;
 }
 __device__ inline void staged_record(const double* w, int slot, double r[REC_SMALL]) {
-    const double2 u = *reinterpret_cast<const double2*>(w + slot * REC_SMALL);
-    const double2 v = *reinterpret_cast<const double2*>(w + slot * REC_SMALL + 2);
-    const double2 x = *reinterpret_cast<const double2*>(w + slot * REC_SMALL + 4);
+    const lds_f64* b = (const lds_f64*)w + slot * REC_SMALL;
+    const f64x2 u = *(const lds_f64x2*)b;
+    const f64x2 v = *(const lds_f64x2*)(b + 2);
+    const f64x2 x = *(const lds_f64x2*)(b + 4);
     r[0] = u.x;
     r[1] = u.y;
     r[2] = v.x;
@@ -272,19 +290,19 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
         const int wy = window_coord(iy, oy, tg.t[1], tg.h[1], g.ny);
         const int wz = window_coord(iz, oz, tg.t[2], tg.h[2], g.nz);
         if (wx >= 0 && wy >= 0 && wz >= 0) {
-            long long* dst = acc + (size_t)((wx * tg.w[1] + wy) * tg.w[2] + wz) * NC;
-            const longlong2* kr = reinterpret_cast<const longlong2*>(kf + (size_t)ang * KS);
+            lds_i64* dst = (lds_i64*)acc + ((wx * tg.w[1] + wy) * tg.w[2] + wz) * NC;
+            const lds_i64x2* kr = (const lds_i64x2*)((const lds_i64*)kf + ang * KS);
 #pragma unroll
             for (int q = 0; q < NM; q += 2) {
-                const longlong2 v = kr[q / 2];
-                atomicAdd(reinterpret_cast<unsigned long long*>(dst + q), (unsigned long long)v.x);
-                if (q + 1 < NM) atomicAdd(reinterpret_cast<unsigned long long*>(dst + q + 1), (unsigned long long)v.y);
+                const i64x2 v = kr[q / 2];
+                __hip_atomic_fetch_add(dst + q, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (q + 1 < NM) __hip_atomic_fetch_add(dst + q + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
-            atomicAdd(reinterpret_cast<unsigned long long*>(dst + NM), 1ull);
+            __hip_atomic_fetch_add(dst + NM, 1ll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         } else {
             const int cell = (ix * g.ny + iy) * g.nz + iz;
             long long* dst = a.inflow + cell;
-            const long long* kr = kf + (size_t)ang * KS;
+            const lds_i64* kr = (const lds_i64*)kf + ang * KS;
 #pragma unroll
             for (int q = 0; q < NM; ++q)
                 atomicAdd(reinterpret_cast<unsigned long long*>(dst + (size_t)q * g.ns), (unsigned long long)kr[q]);
