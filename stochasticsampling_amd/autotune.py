@@ -67,6 +67,12 @@ def _window(t, n):
     return ta, ta + 2 * h, (n + ta - 1) // ta
 
 
+def _tile_lds(ws, nm, na, naxes):
+    """LDS bytes of the tiled update without staged cell records (the library's
+    tile_lds_bytes): window accumulators, moment-kernel rows, window-slot tables."""
+    return (ws * (nm + 1) * 8 + 15) // 16 * 16 + ((nm + 1) & ~1) * na * 8 + (naxes * 4 + 15) // 16 * 16
+
+
 def candidates(p, tiles=TILES, min_tiles=256):
     """Tile shapes valid for ``p`` (window within the LDS budget, at least
     ``min_tiles`` tiles), without duplicates after clamping to the grid."""
@@ -80,7 +86,7 @@ def candidates(p, tiles=TILES, min_tiles=256):
         eff = tuple(a[0] for a in ax)
         ws = ax[0][1] * ax[1][1] * ax[2][1]
         ntiles = ax[0][2] * ax[1][2] * ax[2][2]
-        if eff in seen or (ws * (nm + 1) * 8 + 15) // 16 * 16 + ((nm + 1) & ~1) * na * 8 > LDS_LIMIT or ntiles < min_tiles:
+        if eff in seen or _tile_lds(ws, nm, na, sum(dims)) > LDS_LIMIT or ntiles < min_tiles:
             continue
         seen.add(eff)
         out.append(eff)

@@ -309,18 +309,25 @@ __device__ inline int grid_index(T q, T g, int n) {
 // The same index without the division: t = q * rg with rg = fl(1/g) is within
 // n (2u + u^2) of q/g for q in [0, n g] (u the unit roundoff), and fl(q/g)
 // within n u of it, so floor(t) = floor(fl(q/g)) unless t lies within
-// tol >= 3 n u of an integer; there the division decides.  t - floor(t) is
-// exact for t >= 0.
+// tol >= 3 n u of an integer; there the division decides.
+// fr = fract(t) = t - floor(t) is exact for t >= 0.  half = fl(1/2 - tol):
+// |fl(fr - 1/2)| < half implies tol <= fr <= 1 - tol (rounding is monotonic
+// and odd, so fr < tol gives fl(fr - 1/2) <= -half and fr > 1 - tol gives
+// fl(fr - 1/2) >= half).
 template <typename T>
-__device__ inline int grid_index_r(T q, T g, T rg, T tol, int n) {
+__device__ inline int grid_index_r(T q, T g, T rg, T half, int n) {
     const T t = q * rg;
     T f = Math<T>::floor(t);
-    const T fr = t - f;
-    if (!(fr >= tol && fr <= T(1) - tol)) f = Math<T>::floor(q / g);
-    // clamp to [0, n - 1] before the conversion (an index equal to n folds to
-    // n - 1; NaN gives 0)
+    T fr;
     if constexpr (sizeof(T) == 4)
-        f = __builtin_fminf(__builtin_fmaxf(f, 0.0f), (float)(n - 1));
+        fr = __builtin_amdgcn_fractf(t);
+    else
+        fr = __builtin_amdgcn_fract(t);
+    if (!(Math<T>::fabs(fr - T(0.5)) < half)) f = Math<T>::floor(q / g);
+    // clamp to [0, n - 1] before the conversion (an index equal to n folds to
+    // n - 1)
+    if constexpr (sizeof(T) == 4)
+        f = __builtin_amdgcn_fmed3f(f, 0.0f, (float)(n - 1));
     else
         f = __builtin_fmin(__builtin_fmax(f, 0.0), (double)(n - 1));
     return (int)f;
@@ -486,11 +493,12 @@ __device__ inline void langevin(const UpdateParams<T>& P, T phi, T theta, const 
         T sh, q0;
         half_angle<T>(rv.rot / T(2), &sh, &q0);
         const T q1 = ax0 * sh, q2 = ax1 * sh, q3 = ax2 * sh;
-        // t = q * (0, n)
-        const T t0 = T(0) * q0 - (q1 * n0 + q2 * n1 + q3 * n2);
-        const T t1 = (n0 * q0 + q1 * T(0)) + (q2 * n2 - q3 * n1);
-        const T t2 = (n1 * q0 + q2 * T(0)) + (q3 * n0 - q1 * n2);
-        const T t3 = (n2 * q0 + q3 * T(0)) + (q1 * n1 - q2 * n0);
+        // t = q * (0, n); the Hamilton product's terms with the zero scalar
+        // part of (0, n) are left out (they add zeros)
+        const T t0 = -(q1 * n0 + q2 * n1 + q3 * n2);
+        const T t1 = n0 * q0 + (q2 * n2 - q3 * n1);
+        const T t2 = n1 * q0 + (q3 * n0 - q1 * n2);
+        const T t3 = n2 * q0 + (q1 * n1 - q2 * n0);
         // r = t * conj(q), vector part
         const T c1 = -q1, c2 = -q2, c3 = -q3;
         const T r1 = (c1 * t0 + t1 * q0) + (t2 * c3 - t3 * c2);

@@ -16,8 +16,8 @@ struct Geo {
     int ns, na;
     T gx, gy, gz, gphi, gtheta;  // GridWidth (grid_width.rs:23-31)
     T box[3];
-    // per axis (x, y, z, phi, theta): fl(1 / width) and the tolerance of
-    // grid_index_r (4 u (n + 1))
+    // per axis (x, y, z, phi, theta): fl(1 / width) and fl(1/2 - tol) with
+    // the tolerance tol = 4 u (n + 1) of grid_index_r
     T rg[5], tol[5];
 };
 

@@ -269,7 +269,8 @@ static Geo<T> geo(const ss_sim* s) {
     const double u = sizeof(T) == 4 ? 0x1p-24 : 0x1p-53;
     for (int a = 0; a < 5; ++a) {
         g.rg[a] = T(1) / w[a];  // correctly rounded in T on the host
-        g.tol[a] = (T)(4.0 * u * ((double)n[a] + 1.0));
+        const T tol = (T)(4.0 * u * ((double)n[a] + 1.0));
+        g.tol[a] = T(0.5) - tol;  // rounded in T, as grid_index_r's test needs
     }
     return g;
 }
@@ -597,14 +598,14 @@ static hipError_t launch_spec_planes_inv(ss_sim* s) {
 static size_t tile_rec_bytes(const ss_sim* s) {
     if (s->rs != REC_SMALL || s->no_stage) return 0;
     const size_t b = (size_t)REC_SMALL * (s->prec == SS_FP32 ? sizeof(float) : sizeof(double));
-    return tile_lds_bytes(s->tg.ws, s->nm, s->na, b) <= (size_t)s->lds_max ? b : 0;
+    return tile_lds_bytes(s->tg.ws, s->nm, s->na, s->nx + s->ny + s->nz, b) <= (size_t)s->lds_max ? b : 0;
 }
 
 template <typename T, int NM, bool MOVE, int NZ, bool KEEP>
 static void launch_tile_kernel(ss_sim* s, const TileArgs<T, NM>& a) {
     const int per = (s->tg.ntiles + s->tg.nxcd - 1) / s->tg.nxcd;
     const dim3 grid((unsigned)(per * s->tg.nxcd));
-    const size_t lds = tile_lds_bytes(s->tg.ws, NM, s->na, a.stage ? tile_rec_bytes(s) : 0);
+    const size_t lds = tile_lds_bytes(s->tg.ws, NM, s->na, s->nx + s->ny + s->nz, a.stage ? tile_rec_bytes(s) : 0);
     auto go = [&](auto kern) {
         allow_lds(s->lds_max, (const void*)kern, lds);
         hipLaunchKernelGGL(kern, grid, dim3(256), lds, s->stream, a);
@@ -1272,7 +1273,7 @@ static int choose_tiles(ss_sim* s) {
     }
     tg.nxcd = 8;
     s->resort = s->p.resort_interval > 0 ? s->p.resort_interval : 16;
-    if (tile_lds_bytes(tg.ws, s->nm, s->na) > (size_t)s->lds_max) {
+    if (tile_lds_bytes(tg.ws, s->nm, s->na, s->nx + s->ny + s->nz) > (size_t)s->lds_max) {
         if (s->p.binning == SS_BIN_SORTED)
             return fail(SS_ERR_UNSUPPORTED, "tile window and angle bins exceed the LDS of the tiled binning");
         s->sorted = false;
@@ -2625,8 +2626,8 @@ int ss_ensemble_create(const ss_params* params, int nsys, ss_ensemble** out) {
         s->d_step = e->d_step;
         const int per = (s->tg.ntiles + s->tg.nxcd - 1) / s->tg.nxcd;
         start.push_back(start.back() + per * s->tg.nxcd);
-        lds = std::max(lds, tile_lds_bytes(s->tg.ws, s->nm, s->na));
-        lds_staged = std::max(lds_staged, tile_lds_bytes(s->tg.ws, s->nm, s->na, tile_rec_bytes(s)));
+        lds = std::max(lds, tile_lds_bytes(s->tg.ws, s->nm, s->na, s->nx + s->ny + s->nz));
+        lds_staged = std::max(lds_staged, tile_lds_bytes(s->tg.ws, s->nm, s->na, s->nx + s->ny + s->nz, tile_rec_bytes(s)));
     }
     // staged cell records only when they leave room for the member table
     if ((lds_staged + 15) / 16 * 16 + sizeof(int) * (nsys + 1) <= (size_t)e->m[0]->lds_max)

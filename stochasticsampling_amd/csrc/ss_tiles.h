@@ -90,14 +90,21 @@ constexpr int kf_stride() {
 }
 
 // Dynamic LDS of the tile update: window accumulators [ws][nm+1], moment
-// kernels [na][KS], and, when staged, the window's cell records [ws][8] of
+// kernels [na][KS], the window-slot tables of the three axes (naxes = nx + ny
+// + nz ints), and, when staged, the window's cell records [ws][8] of
 // rec_bytes / 8 bytes per value (0: not staged).  Every region starts at a
 // multiple of 16 bytes.
 inline size_t tile_acc_bytes(int ws, int nm) { return ((size_t)ws * (nm + 1) * 8 + 15) / 16 * 16; }
-inline size_t tile_lds_bytes(int ws, int nm, int na, size_t rec_bytes = 0) {
+inline size_t tile_lds_bytes(int ws, int nm, int na, int naxes, size_t rec_bytes = 0) {
     const size_t ks = (size_t)((nm + 1) & ~1);
-    return tile_acc_bytes(ws, nm) + ks * na * 8 + (size_t)ws * rec_bytes;
+    return tile_acc_bytes(ws, nm) + ks * na * 8 + ((size_t)naxes * 4 + 15) / 16 * 16 + (size_t)ws * rec_bytes;
 }
+
+// Window-slot tables: entry c of an axis is the window coordinate of cell c
+// times the axis's slot stride, or WT_OUT when the cell lies outside the
+// window; the slot of a cell is the sum of its three entries, negative when
+// outside.
+constexpr int WT_OUT = -(1 << 24);
 
 // The particles in slots [q0, q1) of one tile: move them (MOVE) and bin them
 // into the next step's moments in the LDS window of the tile; MOVE = false
@@ -130,6 +137,7 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 typedef long long i64x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) long long lds_i64;
+typedef __attribute__((address_space(3))) int lds_i32;
 typedef __attribute__((address_space(3))) i64x2 lds_i64x2;
 typedef __attribute__((address_space(3))) float lds_f32;
 typedef __attribute__((address_space(3))) double lds_f64;
@@ -164,7 +172,7 @@ __device__ inline void staged_record(const double* w, int slot, double r[REC_SMA
 
 template <typename T, int NM, bool MOVE, int NZ, bool KEEP, bool SCATTER, bool FULL = true>
 __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0, uint32_t q1, bool owner,
-                                 long long* acc, const long long* kf, T* wrec) {
+                                 long long* acc, const long long* kf, int* wtab, T* wrec) {
     constexpr int NC = NM + 1;
     constexpr int KS = kf_stride<NM>();
     const Geo<T>& g = a.g;
@@ -188,6 +196,18 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
         if constexpr (NZ == NZ_REC) ncur = load_noise_rec(a.noise, i);
     }
     for (int k = threadIdx.x; k < tg.ws * NC; k += blockDim.x) acc[k] = 0;
+    const int naxes = g.nx + g.ny + g.nz;
+    for (int k = threadIdx.x; k < naxes; k += blockDim.x) {
+        const int ax = k < g.nx ? 0 : (k < g.nx + g.ny ? 1 : 2);
+        const int c = ax == 0 ? k : (ax == 1 ? k - g.nx : k - g.nx - g.ny);
+        const int o = ax == 0 ? ox : (ax == 1 ? oy : oz), n = ax == 0 ? g.nx : (ax == 1 ? g.ny : g.nz);
+        const int stride = ax == 0 ? tg.w[1] * tg.w[2] : (ax == 1 ? tg.w[2] : 1);
+        const int w = window_coord(c, o, tg.t[ax], tg.h[ax], n);
+        wtab[k] = w >= 0 ? w * stride : WT_OUT;
+    }
+    const lds_i32* wt = (const lds_i32*)wtab;
+    const lds_i32* wty = wt + g.nx;
+    const lds_i32* wtz = wty + g.ny;
     // the cell records of the window (8-value records): every particle of the
     // tile that is still inside the window reads its record from LDS
     const bool stage = MOVE && a.stage;
@@ -221,11 +241,9 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
             T r[REC_FULL];
             bool from_lds = false;
             if (stage) {
-                const int wx = window_coord(cx, ox, tg.t[0], tg.h[0], g.nx);
-                const int wy = window_coord(cy, oy, tg.t[1], tg.h[1], g.ny);
-                const int wz = window_coord(cz, oz, tg.t[2], tg.h[2], g.nz);
-                from_lds = wx >= 0 && wy >= 0 && wz >= 0;
-                staged_record(wrec, from_lds ? (wx * tg.w[1] + wy) * tg.w[2] + wz : 0, r);
+                const int slot = wt[cx] + wty[cy] + wtz[cz];
+                from_lds = slot >= 0;
+                staged_record(wrec, from_lds ? slot : 0, r);
             }
             // outside the window (drifted since the last re-sort), or not staged
             if (!from_lds) {
@@ -286,11 +304,9 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
         }
         const int ix = bin_x(g, nx), iy = bin_y(g, ny), iz = bin_z(g, nz);
         const int ang = angle_of(g, ophi, otheta);
-        const int wx = window_coord(ix, ox, tg.t[0], tg.h[0], g.nx);
-        const int wy = window_coord(iy, oy, tg.t[1], tg.h[1], g.ny);
-        const int wz = window_coord(iz, oz, tg.t[2], tg.h[2], g.nz);
-        if (wx >= 0 && wy >= 0 && wz >= 0) {
-            lds_i64* dst = (lds_i64*)acc + ((wx * tg.w[1] + wy) * tg.w[2] + wz) * NC;
+        const int wslot = wt[ix] + wty[iy] + wtz[iz];
+        if (wslot >= 0) {
+            lds_i64* dst = (lds_i64*)acc + wslot * NC;
             const lds_i64x2* kr = (const lds_i64x2*)((const lds_i64*)kf + ang * KS);
 #pragma unroll
             for (int q = 0; q < NM; q += 2) {
@@ -347,11 +363,13 @@ template <typename T, int NM>
 struct TileLds {
     long long* acc;
     long long* kf;
+    int* wtab;
     T* wrec;
-    __device__ TileLds(unsigned char* base, int ws, int na) {
+    __device__ TileLds(unsigned char* base, int ws, int na, int naxes) {
         acc = reinterpret_cast<long long*>(base);
         kf = reinterpret_cast<long long*>(base + (((size_t)ws * (NM + 1) * 8 + 15) / 16 * 16));
-        wrec = reinterpret_cast<T*>(kf + (size_t)na * kf_stride<NM>());
+        wtab = reinterpret_cast<int*>(kf + (size_t)na * kf_stride<NM>());
+        wrec = reinterpret_cast<T*>(reinterpret_cast<unsigned char*>(wtab) + ((size_t)naxes * 4 + 15) / 16 * 16);
     }
 };
 
@@ -365,10 +383,10 @@ __global__ void __launch_bounds__(256) k_tile_update(TileArgs<T, NM> a) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     const int tile = tile_for_block(a.tg, blockIdx.x);
     if (tile >= a.tg.ntiles) return;
-    const TileLds<T, NM> l(smem_raw, a.tg.ws, a.g.na);
+    const TileLds<T, NM> l(smem_raw, a.tg.ws, a.g.na, a.g.nx + a.g.ny + a.g.nz);
     load_kf<NM>(a.kfix, a.g.na, l.kf);
     tile_pass<T, NM, MOVE, NZ, KEEP, SCATTER, FULL>(a, tile, a.toff[tile], a.toff[tile + 1], true, l.acc, l.kf,
-                                                    l.wrec);
+                                                    l.wtab, l.wrec);
 }
 
 // Many systems in one persistent launch: work item w of the concatenated
@@ -402,10 +420,10 @@ __global__ void __launch_bounds__(256)
         a.step_off = step_off;
         const int tile = tile_for_block(a.tg, w - sstart[lo]);
         if (tile < a.tg.ntiles) {
-            const TileLds<T, NM> l(smem_raw, a.tg.ws, a.g.na);
+            const TileLds<T, NM> l(smem_raw, a.tg.ws, a.g.na, a.g.nx + a.g.ny + a.g.nz);
             load_kf<NM>(a.kfix, a.g.na, l.kf);
             tile_pass<T, NM, MOVE, NZ, false, SCATTER, FULL>(a, tile, a.toff[tile], a.toff[tile + 1], true, l.acc,
-                                                             l.kf, l.wrec);
+                                                             l.kf, l.wtab, l.wrec);
         }
     }
 }
