@@ -112,7 +112,10 @@ constexpr int WT_OUT = -(1 << 24);
 // tile's own cells (every tile has exactly one owning pass per step, which
 // also runs for a tile without particles); a pass that does not own the tile
 // adds its own cells to the inflow accumulators instead.  kf must hold the
-// fixed-point moment kernels; acc is the window.  Ends with a barrier.
+// fixed-point moment kernels; acc is the window.  REUSE: the workgroup
+// goes on to another tile and reuses the LDS, so the pass ends with a
+// barrier; a workgroup that exits after its one tile does not wait for the
+// flush's atomics to complete.
 // Grid cell of window slot `slot` of the tile at (ox, oy, oz), periodic;
 // own: the cell belongs to the tile.
 __device__ inline size_t window_cell(const TileGeo& tg, int nx, int ny, int nz, int ox, int oy, int oz, int slot,
@@ -170,7 +173,7 @@ __device__ inline void staged_record(const double* w, int slot, double r[REC_SMA
     r[5] = x.y;
 }
 
-template <typename T, int NM, bool MOVE, int NZ, bool KEEP, bool SCATTER, bool FULL = true>
+template <typename T, int NM, bool MOVE, int NZ, bool KEEP, bool SCATTER, bool FULL = true, bool REUSE = true>
 __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0, uint32_t q1, bool owner,
                                  long long* acc, const long long* kf, int* wtab, T* wrec) {
     constexpr int NC = NM + 1;
@@ -231,49 +234,71 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
 
     const uint64_t step = MOVE ? *a.step + a.step_off : 0;
     const uint32_t hbase = scatter ? a.dtoff[tile] : 0u;
+    // Cell record of a particle: from the staged window, or (outside the
+    // window, having drifted since the last re-sort, or records not staged)
+    // from global memory; rp: the record's global address, ts: the tile of
+    // its cell (re-sorting update).
+    auto fetch = [&](const Loaded<T>& q, T r[REC_FULL], const T*& rp, uint32_t& ts) {
+        const int cx = bin_x(g, q.x), cy = bin_y(g, q.y), cz = bin_z(g, q.z);
+        rp = a.rec + (size_t)((cx * g.ny + cy) * g.nz + cz) * a.rs;
+        if constexpr (scatter) ts = (uint32_t)tile_of(tg, cx, cy, cz);
+        bool from_lds = false;
+        if (stage) {
+            const int slot = wt[cx] + wty[cy] + wtz[cz];
+            from_lds = slot >= 0;
+            staged_record(wrec, from_lds ? slot : 0, r);
+        }
+        if (!from_lds) {
+            if constexpr (sizeof(T) == 4) {
+                gather_record(rp, a.rs, r);
+            } else if (stage) {
+#pragma unroll
+                for (int k = 0; k < REC_SMALL; ++k) r[k] = rp[k];
+            }
+        }
+    };
+    // The record of the current particle was fetched at the end of the
+    // previous iteration, before that iteration's LDS atomics, so that no
+    // iteration starts by waiting for the atomics of the one before.
+    T rc[REC_FULL];
+    const T* rpc = a.rec;
+    uint32_t tsc = 0;
+    if (MOVE && i < p1) fetch(cur, rc, rpc, tsc);
     for (; i < p1; i += blockDim.x) {
         T nx, ny, nz, ophi, otheta;
+        Loaded<T> nxt;
+        uint32_t nid = 0;
+        NoiseRec<T> nnxt{};
         if (MOVE) {
-            // the cell record is requested before the next particle, so that
-            // waiting for it leaves the next particle's loads in flight
-            const int cx = bin_x(g, cur.x), cy = bin_y(g, cur.y), cz = bin_z(g, cur.z);
-            const T* recp = a.rec + (size_t)((cx * g.ny + cy) * g.nz + cz) * a.rs;
-            T r[REC_FULL];
-            bool from_lds = false;
-            if (stage) {
-                const int slot = wt[cx] + wty[cy] + wtz[cz];
-                from_lds = slot >= 0;
-                staged_record(wrec, from_lds ? slot : 0, r);
-            }
-            // outside the window (drifted since the last re-sort), or not staged
-            if (!from_lds) {
-                if constexpr (sizeof(T) == 4) {
-                    gather_record(recp, a.rs, r);
-                } else if (stage) {
-#pragma unroll
-                    for (int k = 0; k < REC_SMALL; ++k) r[k] = recp[k];
-                }
-            }
             // unconditional (clamped to the segment's last slot): a branch
             // here would make the wait for the record a wait for everything
             const uint32_t j = min(i + blockDim.x, p1 - 1);
-            Loaded<T> nxt;
-            const uint32_t nid = at(a.s.id, j);
+            nid = at(a.s.id, j);
             load_particle(a.s, j, nxt);
-            NoiseRec<T> nnxt{};
             if constexpr (NZ == NZ_REC) nnxt = load_noise_rec(a.noise, j);
             Noise<T> rv;
-            if constexpr (NZ == NZ_REC)
+            if constexpr (NZ == NZ_REC) {
                 noise_from_vector<T>(ncur.v, rv);
-            else
-                draw_noise<T, NZ>(a.P, a.seed, step, cid, a.n, a.noise, rv);
+            } else {
+                // seed and step opaque to the compiler here, so that the
+                // Philox key schedule derived from them is recomputed by
+                // scalar adds in every iteration instead of being hoisted
+                // into 20 scalar registers held across the loop
+                // (both are uniform: one system per workgroup)
+                uint32_t k0 = __builtin_amdgcn_readfirstlane((uint32_t)a.seed);
+                uint32_t k1 = __builtin_amdgcn_readfirstlane((uint32_t)(a.seed >> 32));
+                uint32_t s0 = __builtin_amdgcn_readfirstlane((uint32_t)step);
+                uint32_t s1 = __builtin_amdgcn_readfirstlane((uint32_t)(step >> 32));
+                asm volatile("" : "+s"(k0), "+s"(k1), "+s"(s0), "+s"(s1));
+                draw_noise<T, NZ>(a.P, ((uint64_t)k1 << 32) | k0, ((uint64_t)s1 << 32) | s0, cid, a.n, a.noise, rv);
+            }
             State<T> out = a.s;
             uint32_t o = i;
             if constexpr (scatter) {
                 // particles still in this tile fill its new segment from the
                 // front (one LDS reservation per wavefront), the others fill
                 // their tile's segment from the back
-                const uint32_t ts = (uint32_t)tile_of(tg, cx, cy, cz);
+                const uint32_t ts = tsc;
                 const bool h = ts == (uint32_t)tile;
                 const uint64_t m = __ballot(h);
                 const int lane = __lane_id();
@@ -289,12 +314,9 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
                 at(out.id, o) = cid;
             }
             if (sizeof(T) == 4 || stage)
-                move_loaded<T, FULL>(out, o, cur, a.P, r, rv, nx, ny, nz, ophi, otheta);
+                move_loaded<T, FULL>(out, o, cur, a.P, rc, rv, nx, ny, nz, ophi, otheta);
             else
-                move_loaded<T, FULL>(out, o, cur, a.P, recp, rv, nx, ny, nz, ophi, otheta);
-            cur = nxt;
-            cid = nid;
-            if constexpr (NZ == NZ_REC) ncur = nnxt;
+                move_loaded<T, FULL>(out, o, cur, a.P, rpc, rv, nx, ny, nz, ophi, otheta);
         } else {
             nx = at(a.s.x, i);
             ny = at(a.s.y, i);
@@ -305,20 +327,28 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
         const int ix = bin_x(g, nx), iy = bin_y(g, ny), iz = bin_z(g, nz);
         const int ang = angle_of(g, ophi, otheta);
         const int wslot = wt[ix] + wty[iy] + wtz[iz];
+        const lds_i64* kr = (const lds_i64*)kf + ang * KS;
+        i64x2 kv[KS / 2];
+        if (wslot >= 0) {
+#pragma unroll
+            for (int q = 0; q < KS / 2; ++q) kv[q] = ((const lds_i64x2*)kr)[q];
+        }
+        if (MOVE) {
+            fetch(nxt, rc, rpc, tsc);
+            cur = nxt;
+            cid = nid;
+            if constexpr (NZ == NZ_REC) ncur = nnxt;
+        }
         if (wslot >= 0) {
             lds_i64* dst = (lds_i64*)acc + wslot * NC;
-            const lds_i64x2* kr = (const lds_i64x2*)((const lds_i64*)kf + ang * KS);
 #pragma unroll
-            for (int q = 0; q < NM; q += 2) {
-                const i64x2 v = kr[q / 2];
-                __hip_atomic_fetch_add(dst + q, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                if (q + 1 < NM) __hip_atomic_fetch_add(dst + q + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            }
+            for (int q = 0; q < NM; ++q)
+                __hip_atomic_fetch_add(dst + q, (q & 1) ? kv[q / 2].y : kv[q / 2].x, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
             __hip_atomic_fetch_add(dst + NM, 1ll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         } else {
             const int cell = (ix * g.ny + iy) * g.nz + iz;
             long long* dst = a.inflow + cell;
-            const lds_i64* kr = (const lds_i64*)kf + ang * KS;
 #pragma unroll
             for (int q = 0; q < NM; ++q)
                 atomicAdd(reinterpret_cast<unsigned long long*>(dst + (size_t)q * g.ns), (unsigned long long)kr[q]);
@@ -348,7 +378,7 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
                           (unsigned long long)src[q]);
         }
     }
-    __syncthreads();
+    if constexpr (REUSE) __syncthreads();
 }
 
 // Moment kernels [NM][na] into LDS rows [na][KS] (visible after the first
@@ -385,7 +415,7 @@ __global__ void __launch_bounds__(256) k_tile_update(TileArgs<T, NM> a) {
     if (tile >= a.tg.ntiles) return;
     const TileLds<T, NM> l(smem_raw, a.tg.ws, a.g.na, a.g.nx + a.g.ny + a.g.nz);
     load_kf<NM>(a.kfix, a.g.na, l.kf);
-    tile_pass<T, NM, MOVE, NZ, KEEP, SCATTER, FULL>(a, tile, a.toff[tile], a.toff[tile + 1], true, l.acc, l.kf,
+    tile_pass<T, NM, MOVE, NZ, KEEP, SCATTER, FULL, false>(a, tile, a.toff[tile], a.toff[tile + 1], true, l.acc, l.kf,
                                                     l.wtab, l.wrec);
 }
 
