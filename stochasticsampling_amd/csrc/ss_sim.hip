@@ -174,7 +174,6 @@ struct ss_sim {
     int npairs = 0;
     int pencils = 8;  // (y, z) pencils per workgroup of the pencil kernel
     int spec_p = 0;   // 16, 32, 48 or 64 when all three axes have that length (stages fixed at compile time)
-    uint32_t* home_ctr = nullptr;  // re-sort: particles placed from a segment's front
     bool mag = false;  // magnetic solve runs
     bool use_e = false, use_rho = false, use_b = false, use_gb = false;
     int nsig = 6;
@@ -664,7 +663,6 @@ static TileArgs<T, NM> tile_args(ss_sim* s) {
     a.d = state_b<T>(s);
     a.dtoff = s->pending ? s->toff2 : nullptr;
     a.cross = s->cursor;
-    a.home = s->home_ctr;
     a.stage = tile_rec_bytes(s) > 0 ? 1 : 0;
     return a;
 }
@@ -730,7 +728,6 @@ static hipError_t launch_count_scan(ss_sim* s) {
     hipLaunchKernelGGL(k_scan_reduce, dim3(s->nparts), dim3(256), 0, s->stream, s->tcnt, nt, s->part);
     hipLaunchKernelGGL(k_scan_apply, dim3(s->nparts), dim3(256), 0, s->stream, s->tcnt, nt, s->part, s->nparts,
                        s->toff2, s->cursor);
-    fill_u32(s, s->home_ctr, (size_t)nt, 0u);
     s->pending = true;
     s->steps_since_sort = 0;
     return hipGetLastError();
@@ -1147,7 +1144,6 @@ static int create_impl(ss_sim* s) {
         A(s->scnt, (size_t)s->tg.ntiles * 4);
     }
     if (s->sorted) {
-        A(s->home_ctr, (size_t)s->tg.ntiles * 4);
         A(s->mom_main, (size_t)s->ns * (s->nm + 1) * 8);
         A(s->mom_inflow, (size_t)s->ns * (s->nm + 1) * 8);
         SS_HIP_CHECK(hipMemset(s->mom_inflow, 0, (size_t)s->ns * (s->nm + 1) * 8));

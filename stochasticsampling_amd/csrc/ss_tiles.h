@@ -78,7 +78,6 @@ struct TileArgs {
     State<T> d;
     const uint32_t* dtoff;  // [ntiles + 1] or nullptr (update in place)
     uint32_t* cross;        // [ntiles] particles placed from a segment's end
-    uint32_t* home;         // [ntiles] particles placed from a segment's front
     int stage;              // 1: the window's cell records are staged in LDS (8-value records)
 };
 
@@ -91,13 +90,13 @@ constexpr int kf_stride() {
 
 // Dynamic LDS of the tile update: window accumulators [ws][nm+1], moment
 // kernels [na][KS], the window-slot tables of the three axes (naxes = nx + ny
-// + nz ints), and, when staged, the window's cell records [ws][8] of
+// + nz ints) followed by the re-sort's front counter (one int), and, when staged, the window's cell records [ws][8] of
 // rec_bytes / 8 bytes per value (0: not staged).  Every region starts at a
 // multiple of 16 bytes.
 inline size_t tile_acc_bytes(int ws, int nm) { return ((size_t)ws * (nm + 1) * 8 + 15) / 16 * 16; }
 inline size_t tile_lds_bytes(int ws, int nm, int na, int naxes, size_t rec_bytes = 0) {
     const size_t ks = (size_t)((nm + 1) & ~1);
-    return tile_acc_bytes(ws, nm) + ks * na * 8 + ((size_t)naxes * 4 + 15) / 16 * 16 + (size_t)ws * rec_bytes;
+    return tile_acc_bytes(ws, nm) + ks * na * 8 + ((size_t)(naxes + 1) * 4 + 15) / 16 * 16 + (size_t)ws * rec_bytes;
 }
 
 // Window-slot tables: entry c of an axis is the window coordinate of cell c
@@ -208,6 +207,10 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
         const int w = window_coord(c, o, tg.t[ax], tg.h[ax], n);
         wtab[k] = w >= 0 ? w * stride : WT_OUT;
     }
+    // front counter of the re-sort: only this workgroup places particles at
+    // the front of this tile's new segment
+    lds_i32* home = (lds_i32*)wtab + naxes;
+    if (scatter && threadIdx.x == 0) *home = 0;
     const lds_i32* wt = (const lds_i32*)wtab;
     const lds_i32* wty = wt + g.nx;
     const lds_i32* wtz = wty + g.ny;
@@ -305,18 +308,22 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
                 if (m != 0) {
                     const int leader = __ffsll((unsigned long long)m) - 1;
                     uint32_t base = 0;
-                    if (lane == leader) base = atomicAdd(a.home + tile, (unsigned)__popcll(m));
+                    if (lane == leader)
+                        base = (uint32_t)__hip_atomic_fetch_add(home, (int)__popcll(m), __ATOMIC_RELAXED,
+                                                                __HIP_MEMORY_SCOPE_WORKGROUP);
                     base = __shfl(base, leader);
                     if (h) o = hbase + base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
                 }
                 if (!h) o = a.dtoff[ts + 1] - 1u - atomicAdd(a.cross + ts, 1u);
                 out = a.d;
-                at(out.id, o) = cid;
             }
+            // (the slot's first use is the stores after the arithmetic, so
+            // the reservation's latency overlaps it)
             if (sizeof(T) == 4 || stage)
                 move_loaded<T, FULL>(out, o, cur, a.P, rc, rv, nx, ny, nz, ophi, otheta);
             else
                 move_loaded<T, FULL>(out, o, cur, a.P, rpc, rv, nx, ny, nz, ophi, otheta);
+            if constexpr (scatter) at(out.id, o) = cid;
         } else {
             nx = at(a.s.x, i);
             ny = at(a.s.y, i);
@@ -399,7 +406,7 @@ struct TileLds {
         acc = reinterpret_cast<long long*>(base);
         kf = reinterpret_cast<long long*>(base + (((size_t)ws * (NM + 1) * 8 + 15) / 16 * 16));
         wtab = reinterpret_cast<int*>(kf + (size_t)na * kf_stride<NM>());
-        wrec = reinterpret_cast<T*>(reinterpret_cast<unsigned char*>(wtab) + ((size_t)naxes * 4 + 15) / 16 * 16);
+        wrec = reinterpret_cast<T*>(reinterpret_cast<unsigned char*>(wtab) + ((size_t)(naxes + 1) * 4 + 15) / 16 * 16);
     }
 };
 
