@@ -70,7 +70,7 @@ def _window(t, n):
 def _tile_lds(ws, nm, na, naxes):
     """LDS bytes of the tiled update without staged cell records (the library's
     tile_lds_bytes): window accumulators, moment-kernel rows, window-slot tables."""
-    return (ws * (nm + 1) * 8 + 15) // 16 * 16 + ((nm + 1) & ~1) * na * 8 + ((naxes + 1) * 4 + 15) // 16 * 16
+    return (ws * (nm + 1) * 8 + 15) // 16 * 16 + ((nm + 1) & ~1) * na * 8 + ((naxes + 56) * 4 + 15) // 16 * 16
 
 
 def candidates(p, tiles=TILES, min_tiles=256):

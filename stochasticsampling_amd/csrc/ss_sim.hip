@@ -211,6 +211,9 @@ struct ss_sim {
     // the order of offsets toff2 (computed from the last binning's counts)
     bool pending = false;
     uint32_t *tcnt = nullptr, *toff = nullptr, *toff2 = nullptr, *cursor = nullptr, *part = nullptr;
+    // tiled path: direction counts of the last pass, the re-sort's block
+    // offsets [ntiles][NDC], far arrivals and far block starts [ntiles]
+    uint32_t *dcnt = nullptr, *boff = nullptr, *farcnt = nullptr, *farbase = nullptr;
     int nparts = 0;
     long long *mom_main = nullptr, *mom_inflow = nullptr, *kfix = nullptr;
     double fixed_scale = 1.0;    // value of one fixed-point unit
@@ -662,7 +665,10 @@ static TileArgs<T, NM> tile_args(ss_sim* s) {
     a.hist = (s->p.flags & SS_FLAG_KEEP_DISTRIBUTION) ? s->hist_next : nullptr;
     a.d = state_b<T>(s);
     a.dtoff = s->pending ? s->toff2 : nullptr;
-    a.cross = s->cursor;
+    a.dcnt = s->dcnt;
+    a.boff = s->boff;
+    a.farbase = s->farbase;
+    a.farctr = s->cursor;
     a.stage = tile_rec_bytes(s) > 0 ? 1 : 0;
     return a;
 }
@@ -710,24 +716,27 @@ static hipError_t launch_update(ss_sim* s) {
     return hipGetLastError();
 }
 
-// Offsets of the next re-sort: particles per tile from the count
-// component of the moments just binned, exclusive scan into toff2, crossing
-// counters cleared.  The next update places the particles (pending).
+// Offsets of the next re-sort from the direction counts of the last pass
+// (k_dir_totals, the tiles' exclusive scan into toff2, k_dir_offsets; far
+// arrivals counted by k_far_count first).  The next update places the
+// particles (pending).
 static hipError_t launch_count_scan(ss_sim* s) {
     const int nt = s->tg.ntiles;
-    const dim3 grid((unsigned)((nt + 3) / 4));
-    if (s->nm == 6)
-        hipLaunchKernelGGL(k_tile_counts<6>, grid, dim3(256), 0, s->stream, s->mom_main, s->mom_inflow, s->nx,
-                           s->ny, s->nz, s->tg, s->tcnt);
-    else if (s->nm == 9)
-        hipLaunchKernelGGL(k_tile_counts<9>, grid, dim3(256), 0, s->stream, s->mom_main, s->mom_inflow, s->nx,
-                           s->ny, s->nz, s->tg, s->tcnt);
+    const dim3 g1((unsigned)((nt + 3) / 4));  // a wavefront per tile
+    fill_u32(s, s->farcnt, (size_t)nt, 0u);
+    if (s->prec == SS_FP32)
+        hipLaunchKernelGGL(k_far_count<float>, dim3(nt), dim3(256), 0, s->stream, state_a<float>(s),
+                           (const uint32_t*)s->toff, geo<float>(s), s->tg, (const uint32_t*)s->dcnt, s->farcnt);
     else
-        hipLaunchKernelGGL(k_tile_counts<12>, grid, dim3(256), 0, s->stream, s->mom_main, s->mom_inflow, s->nx,
-                           s->ny, s->nz, s->tg, s->tcnt);
+        hipLaunchKernelGGL(k_far_count<double>, dim3(nt), dim3(256), 0, s->stream, state_a<double>(s),
+                           (const uint32_t*)s->toff, geo<double>(s), s->tg, (const uint32_t*)s->dcnt, s->farcnt);
+    hipLaunchKernelGGL(k_dir_totals, g1, dim3(256), 0, s->stream, (const uint32_t*)s->dcnt,
+                       (const uint32_t*)s->farcnt, s->tg, s->tcnt);
     hipLaunchKernelGGL(k_scan_reduce, dim3(s->nparts), dim3(256), 0, s->stream, s->tcnt, nt, s->part);
     hipLaunchKernelGGL(k_scan_apply, dim3(s->nparts), dim3(256), 0, s->stream, s->tcnt, nt, s->part, s->nparts,
                        s->toff2, s->cursor);
+    hipLaunchKernelGGL(k_dir_offsets, g1, dim3(256), 0, s->stream, (const uint32_t*)s->dcnt,
+                       (const uint32_t*)s->toff2, s->tg, s->boff, s->farbase, s->cursor);
     s->pending = true;
     s->steps_since_sort = 0;
     return hipGetLastError();
@@ -1136,6 +1145,12 @@ static int create_impl(ss_sim* s) {
         A(s->toff, ((size_t)nt + 1) * 4);
         A(s->toff2, ((size_t)nt + 1) * 4);
         A(s->cursor, (size_t)nt * 4);
+        if (s->sorted) {
+            A(s->dcnt, (size_t)nt * NDC * 4);
+            A(s->boff, (size_t)nt * NDC * 4);
+            A(s->farcnt, (size_t)nt * 4);
+            A(s->farbase, (size_t)nt * 4);
+        }
         s->nparts = (nt + SCAN_CHUNK - 1) / SCAN_CHUNK;
         A(s->part, (size_t)s->nparts * 4);
     }
@@ -1268,7 +1283,9 @@ static int choose_tiles(ss_sim* s) {
         tg.ws *= tg.w[a];
     }
     tg.nxcd = 8;
-    s->resort = s->p.resort_interval > 0 ? s->p.resort_interval : 16;
+    // at least two steps: a re-sort takes its offsets from the direction
+    // counts of a pass that did not re-sort
+    s->resort = std::max(2, s->p.resort_interval > 0 ? s->p.resort_interval : 16);
     if (tile_lds_bytes(tg.ws, s->nm, s->na, s->nx + s->ny + s->nz) > (size_t)s->lds_max) {
         if (s->p.binning == SS_BIN_SORTED)
             return fail(SS_ERR_UNSUPPORTED, "tile window and angle bins exceed the LDS of the tiled binning");
@@ -1773,6 +1790,12 @@ int ss_resort(ss_sim* s) {
         return SS_OK;
     }
     if (!s->sorted || s->pending) return SS_OK;
+    if (s->last_resort) {
+        // the last update re-sorted, so its direction counts do not describe
+        // the new order: re-sort after the next step instead
+        s->steps_since_sort = std::max<uint64_t>(s->steps_since_sort, (uint64_t)s->resort - 1);
+        return SS_OK;
+    }
     SS_HIP_CHECK(hipSetDevice(s->p.device));
     // offsets from the counts binned by the last update (or by ss_init); the
     // next step's update writes the particles in the new tile order

@@ -72,14 +72,38 @@ struct TileArgs {
     long long* main;        // [NM+1][ns] moments of each tile's own cells (count last)
     long long* inflow;      // [NM+1][ns] contributions from other tiles
     uint32_t* hist;         // optional [ns][na] counts (kept distribution)
-    // re-sort fused into the update: when dtoff is set, every particle is
-    // written to state d at a slot of the segment of the tile it occupied at
-    // the start of the step (offsets dtoff, from that tile's count)
+    // Every pass that does not re-sort counts, per tile, where its particles
+    // went: dcnt[tile][d] for the 27 neighbour directions d (DIR_HOME: the tile
+    // itself) and DIR_FAR (outside the tile's window).  A re-sort fused into
+    // the next update (dtoff set) writes every particle to state d in the
+    // segment of the tile its cell belongs to: the particles of tile S moving
+    // in direction d fill the block starting at boff[S][d] (an LDS counter per
+    // direction, as only S's workgroup fills it); far particles fill the far
+    // block of their tile (farbase, counter farctr).
+    uint32_t* dcnt;         // [ntiles][NDC]
     State<T> d;
     const uint32_t* dtoff;  // [ntiles + 1] or nullptr (update in place)
-    uint32_t* cross;        // [ntiles] particles placed from a segment's end
+    const uint32_t* boff;   // [ntiles][NDC]
+    const uint32_t* farbase;  // [ntiles]
+    uint32_t* farctr;       // [ntiles]
     int stage;              // 1: the window's cell records are staged in LDS (8-value records)
 };
+
+// Window-slot tables: entry c of an axis is the window coordinate of cell c
+// times the axis's slot stride, or WT_OUT when the cell lies outside the
+// window; the slot of a cell is the sum of its three entries, negative when
+// outside.
+constexpr int WT_OUT = -(1 << 24);
+
+// Neighbour directions of a tile: d = 9 (dx + 1) + 3 (dy + 1) + (dz + 1) for
+// the tile at offset (dx, dy, dz), dx, dy, dz in {-1, 0, 1}; DIR_FAR for a cell
+// outside the tile's window.  A window-slot table entry carries, above bit
+// 16, its axis's share of d (the digit dx + 1 times 9, 3 or 1), so the sum of
+// a cell's three entries is its slot + (d << 16).
+constexpr int DIR_HOME = 13;
+constexpr int DIR_FAR = 27;
+constexpr int NDC = 28;  // counters per tile
+constexpr int WT_SLOT_BITS = 16;
 
 // Moment kernels in LDS: one row of KS >= NM values per angle bin (an even
 // count, so that a row is read as 16-byte pairs).
@@ -90,20 +114,17 @@ constexpr int kf_stride() {
 
 // Dynamic LDS of the tile update: window accumulators [ws][nm+1], moment
 // kernels [na][KS], the window-slot tables of the three axes (naxes = nx + ny
-// + nz ints) followed by the re-sort's front counter (one int), and, when staged, the window's cell records [ws][8] of
+// + nz ints) followed by the direction counters and block offsets (2 NDC
+// ints), and, when staged, the window's cell records [ws][8] of
 // rec_bytes / 8 bytes per value (0: not staged).  Every region starts at a
 // multiple of 16 bytes.
 inline size_t tile_acc_bytes(int ws, int nm) { return ((size_t)ws * (nm + 1) * 8 + 15) / 16 * 16; }
 inline size_t tile_lds_bytes(int ws, int nm, int na, int naxes, size_t rec_bytes = 0) {
     const size_t ks = (size_t)((nm + 1) & ~1);
-    return tile_acc_bytes(ws, nm) + ks * na * 8 + ((size_t)(naxes + 1) * 4 + 15) / 16 * 16 + (size_t)ws * rec_bytes;
+    return tile_acc_bytes(ws, nm) + ks * na * 8 + ((size_t)(naxes + 2 * NDC) * 4 + 15) / 16 * 16 +
+           (size_t)ws * rec_bytes;
 }
 
-// Window-slot tables: entry c of an axis is the window coordinate of cell c
-// times the axis's slot stride, or WT_OUT when the cell lies outside the
-// window; the slot of a cell is the sum of its three entries, negative when
-// outside.
-constexpr int WT_OUT = -(1 << 24);
 
 // The particles in slots [q0, q1) of one tile: move them (MOVE) and bin them
 // into the next step's moments in the LDS window of the tile; MOVE = false
@@ -205,12 +226,21 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
         const int o = ax == 0 ? ox : (ax == 1 ? oy : oz), n = ax == 0 ? g.nx : (ax == 1 ? g.ny : g.nz);
         const int stride = ax == 0 ? tg.w[1] * tg.w[2] : (ax == 1 ? tg.w[2] : 1);
         const int w = window_coord(c, o, tg.t[ax], tg.h[ax], n);
-        wtab[k] = w >= 0 ? w * stride : WT_OUT;
+        // the neighbour digit: below the tile, in it (clipped at the grid's
+        // end), above it
+        const int ext = min(tg.t[ax], n - o), h = tg.h[ax];
+        const int dig = w < h ? 0 : (w - h < ext ? 1 : 2);
+        const int dmul = ax == 0 ? 9 : (ax == 1 ? 3 : 1);
+        wtab[k] = w >= 0 ? w * stride + ((dig * dmul) << WT_SLOT_BITS) : WT_OUT;
     }
-    // front counter of the re-sort: only this workgroup places particles at
-    // the front of this tile's new segment
-    lds_i32* home = (lds_i32*)wtab + naxes;
-    if (scatter && threadIdx.x == 0) *home = 0;
+    // per direction: particles counted (passes that do not re-sort) or placed
+    // (the re-sort), and the re-sort's block offsets
+    lds_i32* lcnt = (lds_i32*)wtab + naxes;
+    lds_i32* lboff = lcnt + NDC;
+    if (threadIdx.x < NDC) {
+        lcnt[threadIdx.x] = 0;
+        if (scatter) lboff[threadIdx.x] = (int)a.boff[(size_t)tile * NDC + threadIdx.x];
+    }
     const lds_i32* wt = (const lds_i32*)wtab;
     const lds_i32* wty = wt + g.nx;
     const lds_i32* wtz = wty + g.ny;
@@ -236,20 +266,22 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
     __syncthreads();
 
     const uint64_t step = MOVE ? *a.step + a.step_off : 0;
-    const uint32_t hbase = scatter ? a.dtoff[tile] : 0u;
     // Cell record of a particle: from the staged window, or (outside the
     // window, having drifted since the last re-sort, or records not staged)
     // from global memory; rp: the record's global address, ts: the tile of
     // its cell (re-sorting update).
-    auto fetch = [&](const Loaded<T>& q, T r[REC_FULL], const T*& rp, uint32_t& ts) {
+    auto fetch = [&](const Loaded<T>& q, T r[REC_FULL], const T*& rp, uint32_t& ts, int& dir) {
         const int cx = bin_x(g, q.x), cy = bin_y(g, q.y), cz = bin_z(g, q.z);
         rp = a.rec + (size_t)((cx * g.ny + cy) * g.nz + cz) * a.rs;
-        if constexpr (scatter) ts = (uint32_t)tile_of(tg, cx, cy, cz);
+        const int sum = wt[cx] + wty[cy] + wtz[cz];
+        if constexpr (scatter) {
+            ts = (uint32_t)tile_of(tg, cx, cy, cz);
+            dir = sum >= 0 ? sum >> WT_SLOT_BITS : DIR_FAR;
+        }
         bool from_lds = false;
         if (stage) {
-            const int slot = wt[cx] + wty[cy] + wtz[cz];
-            from_lds = slot >= 0;
-            staged_record(wrec, from_lds ? slot : 0, r);
+            from_lds = sum >= 0;
+            staged_record(wrec, from_lds ? (sum & ((1 << WT_SLOT_BITS) - 1)) : 0, r);
         }
         if (!from_lds) {
             if constexpr (sizeof(T) == 4) {
@@ -266,7 +298,8 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
     T rc[REC_FULL];
     const T* rpc = a.rec;
     uint32_t tsc = 0;
-    if (MOVE && i < p1) fetch(cur, rc, rpc, tsc);
+    int dirc = DIR_HOME;
+    if (MOVE && i < p1) fetch(cur, rc, rpc, tsc, dirc);
     for (; i < p1; i += blockDim.x) {
         T nx, ny, nz, ophi, otheta;
         Loaded<T> nxt;
@@ -298,23 +331,32 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
             State<T> out = a.s;
             uint32_t o = i;
             if constexpr (scatter) {
-                // particles still in this tile fill its new segment from the
-                // front (one LDS reservation per wavefront), the others fill
-                // their tile's segment from the back
-                const uint32_t ts = tsc;
-                const bool h = ts == (uint32_t)tile;
+                // particles still in this tile take consecutive slots of its
+                // block (one LDS reservation per wavefront), the others one
+                // slot each of the block of their direction (LDS counters),
+                // far particles of their tile's far block
+                const bool h = dirc == DIR_HOME;
                 const uint64_t m = __ballot(h);
                 const int lane = __lane_id();
                 if (m != 0) {
                     const int leader = __ffsll((unsigned long long)m) - 1;
                     uint32_t base = 0;
                     if (lane == leader)
-                        base = (uint32_t)__hip_atomic_fetch_add(home, (int)__popcll(m), __ATOMIC_RELAXED,
+                        base = (uint32_t)__hip_atomic_fetch_add(lcnt + DIR_HOME, (int)__popcll(m), __ATOMIC_RELAXED,
                                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
                     base = __shfl(base, leader);
-                    if (h) o = hbase + base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                    if (h) o = (uint32_t)lboff[DIR_HOME] + base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
                 }
-                if (!h) o = a.dtoff[ts + 1] - 1u - atomicAdd(a.cross + ts, 1u);
+                if (!h) {
+                    if (dirc != DIR_FAR)
+                        o = (uint32_t)lboff[dirc] +
+                            (uint32_t)__hip_atomic_fetch_add(lcnt + dirc, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    else
+                        o = a.farbase[tsc] + atomicAdd(a.farctr + tsc, 1u);
+                }
+                // the counts make every slot distinct and below n; the clamp
+                // only keeps the stores inside the arrays whatever the counts
+                o = min(o, (uint32_t)a.n - 1u);
                 out = a.d;
             }
             // (the slot's first use is the stores after the arithmetic, so
@@ -333,7 +375,17 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
         }
         const int ix = bin_x(g, nx), iy = bin_y(g, ny), iz = bin_z(g, nz);
         const int ang = angle_of(g, ophi, otheta);
-        const int wslot = wt[ix] + wty[iy] + wtz[iz];
+        const int wsum = wt[ix] + wty[iy] + wtz[iz];
+        const int wslot = wsum >= 0 ? wsum & ((1 << WT_SLOT_BITS) - 1) : -1;
+        if constexpr (!scatter) {
+            // where the particle went, for the next re-sort
+            const int d = wsum >= 0 ? wsum >> WT_SLOT_BITS : DIR_FAR;
+            const bool h = d == DIR_HOME;
+            const uint64_t m = __ballot(h);
+            if (m != 0 && __lane_id() == __ffsll((unsigned long long)m) - 1)
+                __hip_atomic_fetch_add(lcnt + DIR_HOME, (int)__popcll(m), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (!h) __hip_atomic_fetch_add(lcnt + d, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
         const lds_i64* kr = (const lds_i64*)kf + ang * KS;
         i64x2 kv[KS / 2];
         if (wslot >= 0) {
@@ -341,7 +393,7 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
             for (int q = 0; q < KS / 2; ++q) kv[q] = ((const lds_i64x2*)kr)[q];
         }
         if (MOVE) {
-            fetch(nxt, rc, rpc, tsc);
+            fetch(nxt, rc, rpc, tsc, dirc);
             cur = nxt;
             cid = nid;
             if constexpr (NZ == NZ_REC) ncur = nnxt;
@@ -368,6 +420,7 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
     }
     __syncthreads();
 
+    if (!scatter && threadIdx.x < NDC) a.dcnt[(size_t)tile * NDC + threadIdx.x] = (uint32_t)lcnt[threadIdx.x];
     // flush: own cells by plain stores when this pass owns the tile (every
     // cell of the grid is owned by exactly one tile, every tile by exactly
     // one pass), other cells with atomics when they received particles
@@ -406,7 +459,7 @@ struct TileLds {
         acc = reinterpret_cast<long long*>(base);
         kf = reinterpret_cast<long long*>(base + (((size_t)ws * (NM + 1) * 8 + 15) / 16 * 16));
         wtab = reinterpret_cast<int*>(kf + (size_t)na * kf_stride<NM>());
-        wrec = reinterpret_cast<T*>(reinterpret_cast<unsigned char*>(wtab) + ((size_t)(naxes + 1) * 4 + 15) / 16 * 16);
+        wrec = reinterpret_cast<T*>(reinterpret_cast<unsigned char*>(wtab) + ((size_t)(naxes + 2 * NDC) * 4 + 15) / 16 * 16);
     }
 };
 
@@ -484,27 +537,78 @@ __global__ void __launch_bounds__(256)
     if (rho) rho[c] = (T)((double)cntv * rho_scale);
 }
 
-// Particles per tile from the moment arrays' count component (the next
-// re-sort) -- one wavefront per tile, its lanes over the tile's cells.
-template <int NM>
+// Re-sort offsets from the direction counts of the last pass.
+//
+// Far particles per destination tile: the segments of the tiles whose last
+// pass counted far particles are scanned again (the same window test as the
+// pass: a cell outside the tile's window).
+template <typename T>
 __global__ void __launch_bounds__(256)
-    k_tile_counts(const long long* __restrict__ main, const long long* __restrict__ inflow, int nx, int ny,
-                  int nz, TileGeo tg, uint32_t* __restrict__ tcnt) {
-    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (t >= tg.ntiles) return;
+    k_far_count(State<T> s, const uint32_t* __restrict__ toff, Geo<T> g, TileGeo tg,
+                const uint32_t* __restrict__ dcnt, uint32_t* __restrict__ farcnt) {
+    const int t = blockIdx.x;
+    if (dcnt[(size_t)t * NDC + DIR_FAR] == 0) return;
     const int tz = t % tg.nt[2], ty = (t / tg.nt[2]) % tg.nt[1], tx = t / (tg.nt[2] * tg.nt[1]);
     const int ox = tx * tg.t[0], oy = ty * tg.t[1], oz = tz * tg.t[2];
-    const int ex = min(tg.t[0], nx - ox), ey = min(tg.t[1], ny - oy), ez = min(tg.t[2], nz - oz);
-    const int nc = ex * ey * ez;
-    const size_t cnt = (size_t)NM * nx * ny * nz;  // the count component
-    long long sum = 0;
-    for (int c = __lane_id(); c < nc; c += 64) {
-        const int k = c % ez, j = (c / ez) % ey, i = c / (ez * ey);
-        const size_t cell = (size_t)((ox + i) * ny + oy + j) * nz + oz + k;
-        sum += main[cnt + cell] + inflow[cnt + cell];
+    for (uint32_t i = toff[t] + threadIdx.x; i < toff[t + 1]; i += blockDim.x) {
+        const int cx = bin_x(g, at(s.x, i)), cy = bin_y(g, at(s.y, i)), cz = bin_z(g, at(s.z, i));
+        if (window_coord(cx, ox, tg.t[0], tg.h[0], g.nx) < 0 || window_coord(cy, oy, tg.t[1], tg.h[1], g.ny) < 0 ||
+            window_coord(cz, oz, tg.t[2], tg.h[2], g.nz) < 0)
+            atomicAdd(farcnt + tile_of(tg, cx, cy, cz), 1u);
     }
-    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
-    if (__lane_id() == 0) tcnt[t] = (uint32_t)sum;
+}
+
+// Tile at offset (dx, dy, dz) from tile t, periodic in tile coordinates.
+__device__ inline int tile_offset(const TileGeo& tg, int t, int dx, int dy, int dz) {
+    const int tz = t % tg.nt[2], ty = (t / tg.nt[2]) % tg.nt[1], tx = t / (tg.nt[2] * tg.nt[1]);
+    const int x = (tx + dx + tg.nt[0]) % tg.nt[0], y = (ty + dy + tg.nt[1]) % tg.nt[1],
+              z = (tz + dz + tg.nt[2]) % tg.nt[2];
+    return (x * tg.nt[1] + y) * tg.nt[2] + z;
+}
+
+// Particles of each tile after the re-sort: those moving into it from each
+// direction, and its far arrivals.  One wavefront per tile, lane d < 27 over
+// direction d.
+__global__ void __launch_bounds__(256)
+    k_dir_totals(const uint32_t* __restrict__ dcnt, const uint32_t* __restrict__ farcnt, TileGeo tg,
+                 uint32_t* __restrict__ tcnt) {
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= tg.ntiles) return;
+    const int d = __lane_id();
+    uint32_t v = 0;
+    if (d < DIR_FAR) v = dcnt[(size_t)tile_offset(tg, t, 1 - d / 9, 1 - (d / 3) % 3, 1 - d % 3) * NDC + d];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (d == 0) tcnt[t] = v + farcnt[t];
+}
+
+// The new segment of tile t (from toff2[t]) holds the blocks of the particles
+// arriving from direction d = 0 .. 26 (source tile t - offset(d)), then its
+// far arrivals; boff[source][d] is the start of each block.  Clears the far
+// placement counters.  One wavefront per tile, lane d over direction d.
+__global__ void __launch_bounds__(256)
+    k_dir_offsets(const uint32_t* __restrict__ dcnt, const uint32_t* __restrict__ toff2, TileGeo tg,
+                  uint32_t* __restrict__ boff, uint32_t* __restrict__ farbase, uint32_t* __restrict__ farctr) {
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= tg.ntiles) return;
+    const int d = __lane_id();
+    size_t k = 0;
+    uint32_t v = 0;
+    if (d < DIR_FAR) {
+        k = (size_t)tile_offset(tg, t, 1 - d / 9, 1 - (d / 3) % 3, 1 - d % 3) * NDC + d;
+        v = dcnt[k];
+    }
+    // inclusive scan over the lanes, in direction order
+    uint32_t s = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t u = __shfl_up(s, off);
+        if (d >= off) s += u;
+    }
+    const uint32_t o = toff2[t];
+    if (d < DIR_FAR) boff[k] = o + s - v;
+    if (d == DIR_FAR - 1) {
+        farbase[t] = o + s;
+        farctr[t] = 0u;
+    }
 }
 
 // Particles per tile from the particle state (initial sort).
