@@ -2215,6 +2215,31 @@ static int member_parity(const ss_sim* s) { return s->x == s->x_buf0 ? 0 : 1; }
 // Job tables for the members' current buffers (slot 0) and the swapped ones
 // (slot 1), each for an update in place and for an update that carries out
 // a pending re-sort.
+// The in-place persistent update that ens_launch_update runs for an
+// ensemble of this noise mode and physics.
+template <typename T, int NM>
+static const void* ens_update_kernel(bool ext, bool full) {
+    if (ext) return (const void*)k_tile_update_batched<T, NM, true, true, false>;
+    if (NM == 6 && !full) return (const void*)k_tile_update_batched<T, NM, true, NZ_PHILOX, false, false>;
+    return (const void*)k_tile_update_batched<T, NM, true, false, false>;
+}
+
+// Resident 256-thread workgroups per CU of a kernel with `lds` bytes of LDS
+// each: the lower of the LDS bound and the register bound (a workgroup puts
+// one wave on each of the CU's four SIMDs; a SIMD holds 512 VGPRs per lane
+// and at most eight waves).
+static int workgroups_per_cu(const void* kern, size_t lds, int device) {
+    int by_regs = 8;
+    hipFuncAttributes fa{};
+    if (hipFuncGetAttributes(&fa, kern) == hipSuccess && fa.numRegs > 0)
+        by_regs = std::min(8, 512 / ((fa.numRegs + 7) / 8 * 8));
+    int lds_cu = 0;
+    if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) != hipSuccess ||
+        lds_cu <= 0)
+        lds_cu = 65536;
+    return std::min(by_regs, lds > 0 ? (int)((size_t)lds_cu / lds) : by_regs);
+}
+
 template <typename T, int NM>
 static int ens_build_jobs(ss_ensemble* e) {
     const int n = (int)e->m.size();
@@ -2648,11 +2673,30 @@ int ss_ensemble_create(const ss_params* params, int nsys, ss_ensemble** out) {
         lds = std::max(lds, tile_lds_bytes(s->tg.ws, s->nm, s->na, s->nx + s->ny + s->nz));
         lds_staged = std::max(lds_staged, tile_lds_bytes(s->tg.ws, s->nm, s->na, s->nx + s->ny + s->nz, tile_rec_bytes(s)));
     }
-    // staged cell records only when they leave room for the member table
-    if ((lds_staged + 15) / 16 * 16 + sizeof(int) * (nsys + 1) <= (size_t)e->m[0]->lds_max)
-        lds = lds_staged;
-    else
-        for (ss_sim* s : e->m) s->no_stage = true;
+    // staged cell records only when they leave room for the member table and
+    // keep as many workgroups per CU: the persistent update sizes its LDS for
+    // the largest window of any member, and a workgroup fewer per CU costs
+    // more than the staging saves (48 members of 32^3 to 64^3 cells: 2.34 ms
+    // staged at two workgroups per CU against 1.88 ms unstaged at three)
+    {
+        bool full = false;
+        for (ss_sim* s : e->m)
+            full = full || (e->prec == SS_FP32 ? full_physics(update_params<float>(s)) : full_physics(update_params<double>(s)));
+        const void* kern = e->prec == SS_FP32
+                               ? (e->nm == 6 ? ens_update_kernel<float, 6>(e->ext, full)
+                                             : e->nm == 9 ? ens_update_kernel<float, 9>(e->ext, full)
+                                                          : ens_update_kernel<float, 12>(e->ext, full))
+                               : (e->nm == 6 ? ens_update_kernel<double, 6>(e->ext, full)
+                                             : e->nm == 9 ? ens_update_kernel<double, 9>(e->ext, full)
+                                                          : ens_update_kernel<double, 12>(e->ext, full));
+        const size_t table = sizeof(int) * (nsys + 1);
+        const size_t with = (lds_staged + 15) / 16 * 16 + table, without = (lds + 15) / 16 * 16 + table;
+        if (with <= (size_t)e->m[0]->lds_max &&
+            workgroups_per_cu(kern, with, s0->p.device) >= workgroups_per_cu(kern, without, s0->p.device))
+            lds = lds_staged;
+        else
+            for (ss_sim* s : e->m) s->no_stage = true;
+    }
     e->total_blocks = start.back();
     e->lds = (lds + 15) / 16 * 16;
     // the persistent update keeps the member table beside the largest window

@@ -69,7 +69,8 @@ struct TileArgs {
     uint32_t step_off;      // offset added to *step
     const T* noise;
     const long long* kfix;  // [NM][na] fixed-point moment kernels
-    long long* main;        // [NM+1][ns] moments of each tile's own cells (count last)
+    long long* main;        // [NM+1][ns] moments of each tile's own cells (count last,
+                            // summed only when the density is wanted: P.use_rho)
     long long* inflow;      // [NM+1][ns] contributions from other tiles
     uint32_t* hist;         // optional [ns][na] counts (kept distribution)
     // Every pass that does not re-sort counts, per tile, where its particles
@@ -195,11 +196,14 @@ __device__ inline void staged_record(const double* w, int slot, double r[REC_SMA
 
 template <typename T, int NM, bool MOVE, int NZ, bool KEEP, bool SCATTER, bool FULL = true, bool REUSE = true>
 __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0, uint32_t q1, bool owner,
-                                 long long* acc, const long long* kf, int* wtab, T* wrec) {
+                                 long long* acc, const long long* kf, int* wtab, T* wrec, uint32_t step_off) {
     constexpr int NC = NM + 1;
     constexpr int KS = kf_stride<NM>();
     const Geo<T>& g = a.g;
     const TileGeo& tg = a.tg;
+    // the count component (the density) is summed only when the density
+    // enters the step (volume_exclusion != 0): nothing else reads it
+    const bool counts = FULL && a.P.use_rho;
     const int tz = tile % tg.nt[2], ty = (tile / tg.nt[2]) % tg.nt[1], tx = tile / (tg.nt[2] * tg.nt[1]);
     const int ox = tx * tg.t[0], oy = ty * tg.t[1], oz = tz * tg.t[2];
 
@@ -221,14 +225,18 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
     for (int k = threadIdx.x; k < tg.ws * NC; k += blockDim.x) acc[k] = 0;
     const int naxes = g.nx + g.ny + g.nz;
     for (int k = threadIdx.x; k < naxes; k += blockDim.x) {
+        // (per-axis values by selects: a per-lane index into the tile
+        // geometry would be a memory access per lane)
         const int ax = k < g.nx ? 0 : (k < g.nx + g.ny ? 1 : 2);
         const int c = ax == 0 ? k : (ax == 1 ? k - g.nx : k - g.nx - g.ny);
         const int o = ax == 0 ? ox : (ax == 1 ? oy : oz), n = ax == 0 ? g.nx : (ax == 1 ? g.ny : g.nz);
         const int stride = ax == 0 ? tg.w[1] * tg.w[2] : (ax == 1 ? tg.w[2] : 1);
-        const int w = window_coord(c, o, tg.t[ax], tg.h[ax], n);
+        const int ta = ax == 0 ? tg.t[0] : (ax == 1 ? tg.t[1] : tg.t[2]);
+        const int h = ax == 0 ? tg.h[0] : (ax == 1 ? tg.h[1] : tg.h[2]);
+        const int w = window_coord(c, o, ta, h, n);
         // the neighbour digit: below the tile, in it (clipped at the grid's
         // end), above it
-        const int ext = min(tg.t[ax], n - o), h = tg.h[ax];
+        const int ext = min(ta, n - o);
         const int dig = w < h ? 0 : (w - h < ext ? 1 : 2);
         const int dmul = ax == 0 ? 9 : (ax == 1 ? 3 : 1);
         wtab[k] = w >= 0 ? w * stride + ((dig * dmul) << WT_SLOT_BITS) : WT_OUT;
@@ -265,7 +273,7 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
     }
     __syncthreads();
 
-    const uint64_t step = MOVE ? *a.step + a.step_off : 0;
+    const uint64_t step = MOVE ? *a.step + step_off : 0;
     // Cell record of a particle: from the staged window, or (outside the
     // window, having drifted since the last re-sort, or records not staged)
     // from global memory; rp: the record's global address, ts: the tile of
@@ -404,14 +412,14 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
             for (int q = 0; q < NM; ++q)
                 __hip_atomic_fetch_add(dst + q, (q & 1) ? kv[q / 2].y : kv[q / 2].x, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
-            __hip_atomic_fetch_add(dst + NM, 1ll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (counts) __hip_atomic_fetch_add(dst + NM, 1ll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         } else {
             const int cell = (ix * g.ny + iy) * g.nz + iz;
             long long* dst = a.inflow + cell;
 #pragma unroll
             for (int q = 0; q < NM; ++q)
                 atomicAdd(reinterpret_cast<unsigned long long*>(dst + (size_t)q * g.ns), (unsigned long long)kr[q]);
-            atomicAdd(reinterpret_cast<unsigned long long*>(dst + (size_t)NM * g.ns), 1ull);
+            if (counts) atomicAdd(reinterpret_cast<unsigned long long*>(dst + (size_t)NM * g.ns), 1ull);
         }
         if (KEEP) {
             const int cell = (ix * g.ny + iy) * g.nz + iz;
@@ -423,19 +431,30 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
     if (!scatter && threadIdx.x < NDC) a.dcnt[(size_t)tile * NDC + threadIdx.x] = (uint32_t)lcnt[threadIdx.x];
     // flush: own cells by plain stores when this pass owns the tile (every
     // cell of the grid is owned by exactly one tile, every tile by exactly
-    // one pass), other cells with atomics when they received particles
+    // one pass), other cells with atomics when they received particles (a
+    // cell whose sums are all zero is skipped whether or not it received
+    // any: adding zeros changes nothing)
     for (int slot = threadIdx.x; slot < tg.ws; slot += blockDim.x) {
         bool own;
         const size_t cell = window_cell(tg, g.nx, g.ny, g.nz, ox, oy, oz, slot, &own);
         const long long* src = acc + (size_t)slot * NC;
         if (own && owner) {
 #pragma unroll
-            for (int q = 0; q < NC; ++q) a.main[(size_t)q * g.ns + cell] = src[q];
-        } else if (src[NM] != 0) {
+            for (int q = 0; q < NM; ++q) a.main[(size_t)q * g.ns + cell] = src[q];
+            if (counts) a.main[(size_t)NM * g.ns + cell] = src[NM];
+        } else {
+            long long any = counts ? src[NM] : 0;
 #pragma unroll
-            for (int q = 0; q < NC; ++q)
-                atomicAdd(reinterpret_cast<unsigned long long*>(a.inflow + (size_t)q * g.ns + cell),
-                          (unsigned long long)src[q]);
+            for (int q = 0; q < NM; ++q) any |= src[q];
+            if (any != 0) {
+#pragma unroll
+                for (int q = 0; q < NM; ++q)
+                    atomicAdd(reinterpret_cast<unsigned long long*>(a.inflow + (size_t)q * g.ns + cell),
+                              (unsigned long long)src[q]);
+                if (counts)
+                    atomicAdd(reinterpret_cast<unsigned long long*>(a.inflow + (size_t)NM * g.ns + cell),
+                              (unsigned long long)src[NM]);
+            }
         }
     }
     if constexpr (REUSE) __syncthreads();
@@ -476,7 +495,7 @@ __global__ void __launch_bounds__(256) k_tile_update(TileArgs<T, NM> a) {
     const TileLds<T, NM> l(smem_raw, a.tg.ws, a.g.na, a.g.nx + a.g.ny + a.g.nz);
     load_kf<NM>(a.kfix, a.g.na, l.kf);
     tile_pass<T, NM, MOVE, NZ, KEEP, SCATTER, FULL, false>(a, tile, a.toff[tile], a.toff[tile + 1], true, l.acc, l.kf,
-                                                    l.wtab, l.wrec);
+                                                           l.wtab, l.wrec, a.step_off);
 }
 
 // Many systems in one persistent launch: work item w of the concatenated
@@ -505,15 +524,16 @@ __global__ void __launch_bounds__(256)
             else
                 hi = mid;
         }
-        lo = __builtin_amdgcn_readfirstlane(lo);  // uniform: keeps the job in scalar registers
-        TileArgs<T, NM> a = jobs[lo];
-        a.step_off = step_off;
+        lo = __builtin_amdgcn_readfirstlane(lo);  // uniform: the job's fields come by scalar loads
+        // a reference to the job in global memory, not a local copy: the
+        // copy of the whole structure ends up in scratch memory
+        const TileArgs<T, NM>& a = jobs[lo];
         const int tile = tile_for_block(a.tg, w - sstart[lo]);
         if (tile < a.tg.ntiles) {
             const TileLds<T, NM> l(smem_raw, a.tg.ws, a.g.na, a.g.nx + a.g.ny + a.g.nz);
             load_kf<NM>(a.kfix, a.g.na, l.kf);
             tile_pass<T, NM, MOVE, NZ, false, SCATTER, FULL>(a, tile, a.toff[tile], a.toff[tile + 1], true, l.acc,
-                                                             l.kf, l.wtab, l.wrec);
+                                                             l.kf, l.wtab, l.wrec, step_off);
         }
     }
 }
