@@ -23,9 +23,13 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-steps/s (1 GPU, 2^24 particles, 64³×16×16) + %HBM peak; FP32 vs FP64"
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per particle-step of the update kernel (FP32 + in-kernel
-# Philox: read x,y,z,phi,theta + 3 compensation words, write the same) and of
-# the FP64 plain update (read 40 + write 40; noise is generated in-kernel).
-UPDATE_BYTES = {"f32": 64.0, "f64": 80.0}
+# Philox: read x,y,z,phi,theta (20 B) + 3 compensation words of 16 bits (6 B),
+# write the same) and of the FP64 plain update (read 40 + write 40; noise is
+# generated in-kernel).  The byte accounting of the path's survey assumed
+# 32-bit compensation words, 64 B per FP32 particle-step; the fraction of the
+# HBM peak at that figure is reported beside (frac_at_64B).
+UPDATE_BYTES = {"f32": 52.0, "f64": 80.0}
+SURVEY_BYTES_F32 = 64.0
 # The plain path's bin stage reads the five FP64 coordinates (40 B per
 # particle) and writes the count grid (4 B per bin) and the moments (16 B per
 # cell and component); on the tiled path the binning of the new state is fused
@@ -497,6 +501,7 @@ def main():
         "traffic": traffic,
         "bytes_per_particle": UPDATE_BYTES["f32"],
         "launch_ms": upd_ms,
+        "frac_at_64B": SURVEY_BYTES_F32 * n / (upd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
     }
 
     mag = None

@@ -324,14 +324,42 @@ __global__ void __launch_bounds__(256)
 
 // ---------------------------------------------------------------------------
 // Particle state (structure of arrays).  lo* hold the low-order parts of the
-// FP32 positions (true position = x + lo), nullptr in FP64.
+// FP32 positions (true position = x + lo, lo_decode), nullptr in FP64.
 // ---------------------------------------------------------------------------
+typedef int16_t lo_t;
+
 template <typename T>
 struct State {
     T *x, *y, *z, *phi, *theta;
-    float *lx, *ly, *lz;
+    lo_t *lx, *ly, *lz;
     uint32_t* id;  // original index (sorted path); nullptr: slot == index
 };
+
+// The low-order part of an FP32 position hi >= 0 is stored as a 16-bit
+// integer k: the part is k u(hi), u(hi) = 2^(e - 39) for hi in [2^e, 2^(e+1)),
+// i.e. 2^-16 of hi's ulp (e clamped below at -10: positions near zero keep an
+// absolute step of 2^-49).  A normalized part, |lo| <= ulp(hi) / 2 =
+// 2^15 u(hi), is held to u(hi) / 2: 40 significant bits for the pair, the
+// precision the compensated sum needs over millions of steps, in 6 bytes per
+// particle instead of 12.
+__device__ inline float lo_unit(float hi) {
+    const uint32_t ex = (__float_as_uint(hi) >> 23) & 0xFFu;  // biased exponent of |hi|
+    return __uint_as_float(((ex < 117u ? 117u : ex) - 39u) << 23);
+}
+__device__ inline float lo_decode(float hi, lo_t k) { return (float)k * lo_unit(hi); }
+__device__ inline lo_t lo_encode(float hi, float lo) {
+    // lo / u(hi), exact: the reciprocal of a power of two
+    const uint32_t ex = (__float_as_uint(hi) >> 23) & 0xFFu;
+    const float inv = __uint_as_float((293u - (ex < 117u ? 117u : ex)) << 23);
+    const float k = __builtin_rintf(lo * inv);
+    return (lo_t)(int)__builtin_fminf(__builtin_fmaxf(k, -32768.0f), 32767.0f);
+}
+// The same from a double part (the exact residual p - hi of an imported
+// FP64 position), rounded once.
+__device__ inline lo_t lo_encode(float hi, double lo) {
+    const double k = __builtin_rint(lo / (double)lo_unit(hi));
+    return (lo_t)(int)__builtin_fmin(__builtin_fmax(k, -32768.0), 32767.0);
+}
 
 template <typename T>
 __device__ inline T add_wrap(T x, T d, T L, float* lo) {
@@ -360,6 +388,11 @@ __device__ inline float add_wrap<float>(float x, float d, float L, float* lo) {
             t = modulo(t, L);
             l = 0.0f;
         }
+        // the wrap can leave |l| above half an ulp of the new t: renormalize
+        // (Fast2Sum, |t| >= |l|), as the stored part must be (lo_encode)
+        const float s = t + l;
+        l = l - (s - t);
+        t = s;
     }
     *lo = l;
     return t;
@@ -394,9 +427,9 @@ __device__ inline void load_particle(const State<T>& s, I i, Loaded<T>& q) {
     q.phi = at(s.phi, i);
     q.theta = at(s.theta, i);
     if constexpr (sizeof(T) == 4) {
-        q.lx = at(s.lx, i);
-        q.ly = at(s.ly, i);
-        q.lz = at(s.lz, i);
+        q.lx = lo_decode(q.x, at(s.lx, i));
+        q.ly = lo_decode(q.y, at(s.ly, i));
+        q.lz = lo_decode(q.z, at(s.lz, i));
     } else {
         q.lx = q.ly = q.lz = 0.0f;
     }
@@ -418,9 +451,9 @@ __device__ inline void move_loaded(const State<T>& s, I i, const Loaded<T>& q, c
             nz = T(0);
             lz = 0.0f;
         }
-        at(s.lx, i) = lx;
-        at(s.ly, i) = ly;
-        at(s.lz, i) = lz;
+        at(s.lx, i) = lo_encode(nx, lx);
+        at(s.ly, i) = lo_encode(ny, ly);
+        at(s.lz, i) = lo_encode(nz, lz);
     } else {
         nx = add_wrap<T>(q.x, dp[0], P.box[0], nullptr);
         ny = add_wrap<T>(q.y, dp[1], P.box[1], nullptr);
@@ -650,10 +683,13 @@ __global__ void __launch_bounds__(256)
     T* dst[5] = {s.x, s.y, s.z, s.phi, s.theta};
     for (int k = 0; k < 5; ++k) dst[k][i] = (T)p[k];
     if (s.lx) {
-        float* lo[3] = {s.lx, s.ly, s.lz};
+        lo_t* lo[3] = {s.lx, s.ly, s.lz};
         for (int k = 0; k < 3; ++k) {
-            float hi = (float)p[k];
-            lo[k][i] = comp ? (float)comp[3 * i + k] : (float)(p[k] - (double)hi);
+            // hi = fl(p) and p - hi (exact) is a normalized part; a given
+            // compensation word (a snapshot's, which lo_decode made) encodes
+            // back to the same integer
+            const float hi = (float)p[k];
+            lo[k][i] = lo_encode(hi, comp ? comp[3 * i + k] : p[k] - (double)hi);
         }
     }
     if (s.id) s.id[i] = (uint32_t)i;
@@ -669,13 +705,13 @@ __global__ void __launch_bounds__(256)
     const T* src[5] = {s.x, s.y, s.z, s.phi, s.theta};
     for (int k = 0; k < 5; ++k) aos[5 * j + k] = (double)src[k][i];
     if (s.lx) {
-        const float* lo[3] = {s.lx, s.ly, s.lz};
+        const lo_t* lo[3] = {s.lx, s.ly, s.lz};
         if (comp) {
-            for (int k = 0; k < 3; ++k) comp[3 * j + k] = (double)lo[k][i];
+            for (int k = 0; k < 3; ++k) comp[3 * j + k] = (double)lo_decode((float)src[k][i], lo[k][i]);
         } else {
             const double L[3] = {box0, box1, box2};
             for (int k = 0; k < 3; ++k) {
-                double v = (double)src[k][i] + (double)lo[k][i];
+                double v = (double)src[k][i] + (double)lo_decode((float)src[k][i], lo[k][i]);
                 if (v < 0.0) v += L[k];
                 if (v >= L[k]) v -= L[k];
                 aos[5 * j + k] = v;

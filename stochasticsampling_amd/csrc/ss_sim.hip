@@ -195,7 +195,7 @@ struct ss_sim {
     // particle state, A and B (B only on the sorted path)
     void *x = nullptr, *y = nullptr, *z = nullptr, *phi = nullptr, *theta = nullptr;
     void *x2 = nullptr, *y2 = nullptr, *z2 = nullptr, *phi2 = nullptr, *theta2 = nullptr;
-    float *lx = nullptr, *ly = nullptr, *lz = nullptr, *lx2 = nullptr, *ly2 = nullptr, *lz2 = nullptr;
+    lo_t *lx = nullptr, *ly = nullptr, *lz = nullptr, *lx2 = nullptr, *ly2 = nullptr, *lz2 = nullptr;
     uint32_t *id = nullptr, *id2 = nullptr;
     // SS_NOISE_REFERENCE on the tiled path: slot of each particle id in the
     // current state, so that the noise records follow the storage order
@@ -1123,9 +1123,9 @@ static int create_impl(ss_sim* s) {
     A(s->phi, n * sizeof(T));
     A(s->theta, n * sizeof(T));
     if (s->prec == SS_FP32) {
-        A(s->lx, n * 4);
-        A(s->ly, n * 4);
-        A(s->lz, n * 4);
+        A(s->lx, n * sizeof(lo_t));
+        A(s->ly, n * sizeof(lo_t));
+        A(s->lz, n * sizeof(lo_t));
     }
     if (s->sorted || s->psort) {
         A(s->x2, n * sizeof(T));
@@ -1134,9 +1134,9 @@ static int create_impl(ss_sim* s) {
         A(s->phi2, n * sizeof(T));
         A(s->theta2, n * sizeof(T));
         if (s->prec == SS_FP32) {
-            A(s->lx2, n * 4);
-            A(s->ly2, n * 4);
-            A(s->lz2, n * 4);
+            A(s->lx2, n * sizeof(lo_t));
+            A(s->ly2, n * sizeof(lo_t));
+            A(s->lz2, n * sizeof(lo_t));
         }
         A(s->id, n * 4);
         A(s->id2, n * 4);
