@@ -258,18 +258,18 @@ def magnetic_case(parts, steps, warmup, profile_steps):
     sim.step(steps)
     sim.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
-    stages = sim.profile_steps(profile_steps)
+    stages = sim.profile_steps(profile_steps) if profile_steps else {}
     stages["plan"] = sim.get_info()
     diag = sim.get_diagnostics()
     sim.close()
     upd = stages.get("update")
+    achieved = UPDATE_BYTES["f32"] * n / (upd * 1e-3) / 1e9 if upd else None
     return {"value": n / (ms * 1e-3), "unit": "particle-steps/s", "ms_per_step": ms, "steps": steps,
             "dtype": "f32", "stages_ms": stages,
             "workload": "2^24 particles, 64^3x16x16, L=50, FP32, field strength 2 along z (torque), "
                         "mu_dd=0.5, drag=0.1, Philox seed 3",
-            "roofline": {"bound": "hbm", "achieved": UPDATE_BYTES["f32"] * n / (upd * 1e-3) / 1e9,
-                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": UPDATE_BYTES["f32"] * n / (upd * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS if achieved else None,
                          "bytes_per_particle": UPDATE_BYTES["f32"], "launch_ms": upd},
             "mean_polarisation": [float(v) for v in diag["mean_orientation"]]}
 

@@ -559,33 +559,43 @@ __global__ void __launch_bounds__(256)
 // particle order.
 constexpr int DIAG_N = 15;
 
+// A grid-stride loop: each thread sums its particles' fixed-point values in
+// registers, the workgroup combines its four wavefronts in LDS, and one
+// device atomic per workgroup and value adds the result (one atomic per
+// wavefront and value serialised on the 15 addresses: 25 ms at 2^24).
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_diag(State<T> s, uint64_t n, Geo<T> g, const double* __restrict__ table /*[12][na]*/, double scale,
            unsigned long long* __restrict__ acc) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ long long part[DIAG_N][4];
     long long v[DIAG_N];
 #pragma unroll
     for (int q = 0; q < DIAG_N; ++q) v[q] = 0;
-    if (i < n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         const double ph = (double)s.phi[i], th = (double)s.theta[i];
         double sp, cp, st, ct;
         ::sincos(ph, &sp, &cp);
         ::sincos(th, &st, &ct);
-        v[0] = llrint(st * cp * scale);
-        v[1] = llrint(st * sp * scale);
-        v[2] = llrint(ct * scale);
+        v[0] += llrint(st * cp * scale);
+        v[1] += llrint(st * sp * scale);
+        v[2] += llrint(ct * scale);
         const int a = angle_of(g, s.phi[i], s.theta[i]);
 #pragma unroll
-        for (int q = 0; q < 3; ++q) v[3 + q] = llrint(table[(size_t)(9 + q) * g.na + a] * scale);
+        for (int q = 0; q < 3; ++q) v[3 + q] += llrint(table[(size_t)(9 + q) * g.na + a] * scale);
 #pragma unroll
-        for (int q = 0; q < 9; ++q) v[6 + q] = llrint(table[(size_t)q * g.na + a] * scale);
+        for (int q = 0; q < 9; ++q) v[6 + q] += llrint(table[(size_t)q * g.na + a] * scale);
     }
+    const int wave = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < DIAG_N; ++q) {
         long long x = v[q];
         for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-        if (__lane_id() == 0) atomicAdd(acc + q, (unsigned long long)x);
+        if (__lane_id() == 0) part[q][wave] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < DIAG_N) {
+        const long long x = part[threadIdx.x][0] + part[threadIdx.x][1] + part[threadIdx.x][2] + part[threadIdx.x][3];
+        atomicAdd(acc + threadIdx.x, (unsigned long long)x);
     }
 }
 

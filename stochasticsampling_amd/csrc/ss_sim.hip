@@ -1701,7 +1701,9 @@ static int diagnostics(ss_sim* s, double* out) {
     const int shift = (int)std::floor(62.0 - std::log2((double)s->n * vmax));
     const double scale = std::ldexp(1.0, shift);
     SS_HIP_CHECK(hipMemsetAsync(s->diag_acc, 0, DIAG_N * 8, s->stream));
-    hipLaunchKernelGGL(k_diag<T>, dim3(blocks_for(s->n)), dim3(256), 0, s->stream, state_a<T>(s), s->n, geo<T>(s),
+    // a grid-stride launch of eight workgroups per CU at most
+    hipLaunchKernelGGL(k_diag<T>, dim3((unsigned)std::min<uint64_t>(blocks_for(s->n), 2048)), dim3(256), 0, s->stream,
+                       state_a<T>(s), s->n, geo<T>(s),
                        (const double*)s->diag_table, scale, s->diag_acc);
     SS_HIP_CHECK(hipGetLastError());
     long long acc[DIAG_N];
