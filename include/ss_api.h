@@ -234,6 +234,16 @@ int ss_profile_steps(ss_sim* sim, uint64_t nsteps, double* ms_per_step, char* na
    SS_ERR_STATE for a member of an ensemble. */
 int ss_resort(ss_sim* sim);
 
+/* Captures the graphs that the next ss_step(sim, nsteps) will replay (whole
+   re-sort periods, and the runs of steps between period boundaries that
+   ss_step otherwise captures only when a run of the same phase and length
+   is asked for a second time), without running anything or changing the
+   simulation's state: the set-up of a run of steps, as an FFT plan is the
+   set-up of its transforms.  Optional; the tiled path only (no effect with
+   the kept distribution, SS_NOISE_REFERENCE or elsewhere).  No reference
+   counterpart.  SS_ERR_STATE before ss_init or for a member of an ensemble. */
+int ss_prepare_steps(ss_sim* sim, uint64_t nsteps);
+
 /* Keep (on != 0) or stop keeping the full count grid of every step's
    binning, as SS_FLAG_KEEP_DISTRIBUTION does from ss_create, so that
    ss_get_distribution (Simulation::get_distribution, mod.rs:231-233) can be

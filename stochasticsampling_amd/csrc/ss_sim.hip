@@ -228,6 +228,14 @@ struct ss_sim {
     void* x_buf0 = nullptr;       // the first x buffer: tells which state buffer is current
     uint32_t* toff_buf0 = nullptr;  // the first offsets buffer (current with x_buf0)
     hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};  // [pending][parity]
+    // tiled path: captured runs of steps inside one re-sort period (phase,
+    // length, pending/parity key), most recently used last
+    struct Segment {
+        int phase, len, key;
+        hipGraphExec_t exec;
+    };
+    std::vector<Segment> segments;
+    std::vector<Segment> seen;  // shapes run once (eagerly); captured when asked for again
     int graph_steps = 0;
     void* kern = nullptr;
     double* diag_table = nullptr;       // [12][na] FP64: full stress kernel (9) + bin-centre n (3)
@@ -943,18 +951,18 @@ static int copy_real(ss_sim* s, const void* spec, int comp0, int ncomp, double* 
 // swapping the state buffers), 8 steps on the plain path.
 static int graph_period(const ss_sim* s) { return s->sorted ? s->resort : 8; }
 
-// Capture one period of steps into graph[key].  Capturing enqueues nothing;
-// it advances the host-side bookkeeping exactly as running the period would,
-// so the caller must launch the graph once afterwards.
+// Capture nsteps steps from the current state into *out.  Capturing
+// enqueues nothing; it advances the host-side bookkeeping exactly as running
+// the steps would (step_once), so the caller must launch the graph once
+// afterwards.
 template <typename T>
-static int capture_period(ss_sim* s, int key) {
-    const int K = graph_period(s);
+static int capture_steps(ss_sim* s, int nsteps, hipGraphExec_t* out) {
     SS_HIP_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     s->capturing = true;
     s->step_off = 0;
     int rc = SS_OK;
-    for (int k = 0; k < K && rc == SS_OK; ++k) rc = step_once<T>(s, nullptr);
-    if (rc == SS_OK) hipLaunchKernelGGL(k_add_step, dim3(1), dim3(64), 0, s->stream, s->d_step, (uint32_t)K);
+    for (int k = 0; k < nsteps && rc == SS_OK; ++k) rc = step_once<T>(s, nullptr);
+    if (rc == SS_OK) hipLaunchKernelGGL(k_add_step, dim3(1), dim3(64), 0, s->stream, s->d_step, (uint32_t)nsteps);
     s->capturing = false;
     s->step_off = 0;
     hipGraph_t g = nullptr;
@@ -964,18 +972,51 @@ static int capture_period(ss_sim* s, int key) {
         return rc;
     }
     if (e != hipSuccess) return fail(SS_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
-    e = hipGraphInstantiate(&s->graph[key], g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) {
-        s->graph[key] = nullptr;
+        *out = nullptr;
         return fail(SS_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
     }
-    s->graph_steps = K;
     return SS_OK;
 }
 
-// nsteps steps: eager steps up to the next period boundary, whole periods as
-// graph launches, eager steps for the rest.
+// One re-sort period (the tiled path) or 8 steps (the plain paths) as the
+// graph of period key `key`.
+template <typename T>
+static int capture_period(ss_sim* s, int key) {
+    const int K = graph_period(s);
+    const int rc = capture_steps<T>(s, K, &s->graph[key]);
+    if (rc == SS_OK) s->graph_steps = K;
+    return rc;
+}
+
+// The host-side bookkeeping of `len` steps of the tiled path from the start
+// of a re-sort period or inside one (what step_once does for them): only a
+// period's first update re-sorts (when one is pending), and a run that
+// reaches the end of the period computes the next re-sort's offsets.
+static void advance_tiled(ss_sim* s, int len) {
+    s->timestep += (uint64_t)len;
+    s->last_resort = len == 1 && s->pending;
+    if (s->pending) swap_sorted(s);
+    s->steps_since_sort += (uint64_t)len;
+    if (s->steps_since_sort >= (uint64_t)s->resort) {
+        s->pending = true;
+        s->steps_since_sort = 0;
+    }
+    s->have_fields = true;
+    s->spec_partial = s->fused;
+    s->spec_packed = s->fused && s->paired;
+}
+
+constexpr size_t MAX_SEGMENTS = 8;
+
+// nsteps steps: whole periods as graph launches; on the tiled path any other
+// run of steps inside one re-sort period as a launch of a captured segment
+// (captured the second time a run of its phase, length and buffer state is
+// asked for; the last MAX_SEGMENTS kept), so that a caller stepping a few
+// steps at a time pays one graph launch per call instead of five kernel
+// launches per step; the rest eager.
 template <typename T>
 static int run_steps(ss_sim* s, uint64_t nsteps) {
     const int K = graph_period(s);
@@ -1001,6 +1042,44 @@ static int run_steps(ss_sim* s, uint64_t nsteps) {
             }
             SS_HIP_CHECK(hipGraphLaunch(s->graph[key], s->stream));
             nsteps -= (uint64_t)K;
+        } else if (s->use_graph && s->sorted && s->hist_next == nullptr) {
+            // the run of steps up to the end of this re-sort period (or of
+            // the request) as one captured segment; the first request of a
+            // shape runs eagerly (a capture costs more than the launches it
+            // saves once), a repeated one is captured and kept
+            const int phase = (int)s->steps_since_sort;
+            const int len = (int)std::min<uint64_t>(nsteps, (uint64_t)(s->resort - phase));
+            const int key = (s->pending ? 2 : 0) + (s->x == s->x_buf0 ? 0 : 1);
+            auto same = [&](const ss_sim::Segment& g) { return g.phase == phase && g.len == len && g.key == key; };
+            auto it = std::find_if(s->segments.begin(), s->segments.end(), same);
+            if (it == s->segments.end() && std::find_if(s->seen.begin(), s->seen.end(), same) == s->seen.end()) {
+                if (s->seen.size() >= 4 * MAX_SEGMENTS) s->seen.erase(s->seen.begin());
+                s->seen.push_back({phase, len, key, nullptr});
+                for (int k = 0; k < len; ++k) {
+                    int rc = step_once<T>(s, nullptr);
+                    if (rc != SS_OK) return rc;
+                }
+                nsteps -= (uint64_t)len;
+                continue;
+            }
+            hipGraphExec_t exec = nullptr;
+            if (it == s->segments.end()) {
+                if (s->segments.size() >= MAX_SEGMENTS) {
+                    (void)hipGraphExecDestroy(s->segments.front().exec);
+                    s->segments.erase(s->segments.begin());
+                }
+                int rc = capture_steps<T>(s, len, &exec);
+                if (rc != SS_OK) return rc;
+                s->segments.push_back({phase, len, key, exec});
+            } else {
+                exec = it->exec;
+                const ss_sim::Segment g = *it;  // most recently used last
+                s->segments.erase(it);
+                s->segments.push_back(g);
+                advance_tiled(s, len);
+            }
+            SS_HIP_CHECK(hipGraphLaunch(exec, s->stream));
+            nsteps -= (uint64_t)len;
         } else {
             int rc = step_once<T>(s, nullptr);
             if (rc != SS_OK) return rc;
@@ -1008,6 +1087,107 @@ static int run_steps(ss_sim* s, uint64_t nsteps) {
         }
     }
     return SS_OK;
+}
+
+// The host-side state a step of the tiled path changes (step_once, the
+// re-sort its update may carry out, the next re-sort's offsets at the end of
+// a period), saved and restored around a capture that must not count.
+struct StepBook {
+    uint64_t timestep, steps_since_sort;
+    bool pending, last_resort, have_fields, have_hist, spec_partial, spec_packed, swap_hist;
+    void *x, *y, *z, *phi, *theta, *x2, *y2, *z2, *phi2, *theta2;
+    lo_t *lx, *ly, *lz, *lx2, *ly2, *lz2;
+    uint32_t *id, *id2, *toff, *toff2;
+};
+
+static StepBook save_book(const ss_sim* s) {
+    return {s->timestep, s->steps_since_sort, s->pending, s->last_resort, s->have_fields, s->have_hist,
+            s->spec_partial, s->spec_packed, s->swap_hist, s->x, s->y, s->z, s->phi, s->theta, s->x2, s->y2,
+            s->z2, s->phi2, s->theta2, s->lx, s->ly, s->lz, s->lx2, s->ly2, s->lz2, s->id, s->id2, s->toff,
+            s->toff2};
+}
+
+static void restore_book(ss_sim* s, const StepBook& b) {
+    s->timestep = b.timestep;
+    s->steps_since_sort = b.steps_since_sort;
+    s->pending = b.pending;
+    s->last_resort = b.last_resort;
+    s->have_fields = b.have_fields;
+    s->have_hist = b.have_hist;
+    s->spec_partial = b.spec_partial;
+    s->spec_packed = b.spec_packed;
+    s->swap_hist = b.swap_hist;
+    s->x = b.x;
+    s->y = b.y;
+    s->z = b.z;
+    s->phi = b.phi;
+    s->theta = b.theta;
+    s->x2 = b.x2;
+    s->y2 = b.y2;
+    s->z2 = b.z2;
+    s->phi2 = b.phi2;
+    s->theta2 = b.theta2;
+    s->lx = b.lx;
+    s->ly = b.ly;
+    s->lz = b.lz;
+    s->lx2 = b.lx2;
+    s->ly2 = b.ly2;
+    s->lz2 = b.lz2;
+    s->id = b.id;
+    s->id2 = b.id2;
+    s->toff = b.toff;
+    s->toff2 = b.toff2;
+}
+
+// Capture the graphs that the next run_steps(s, nsteps) replays (whole
+// periods and the runs of steps between period boundaries), without running
+// anything: the bookkeeping the captures advance is restored afterwards.
+// The tiled path with captured graphs only (not with a kept distribution or
+// the reference's noise stream, whose host-side generator state a capture
+// would advance); elsewhere nothing to do.
+template <typename T>
+static int prepare_steps(ss_sim* s, uint64_t nsteps) {
+    if (!s->use_graph || !s->sorted || s->hist_next != nullptr || s->refn) return SS_OK;
+    const StepBook book = save_book(s);
+    const int K = graph_period(s);
+    std::vector<ss_sim::Segment> made;
+    int rc = SS_OK;
+    while (nsteps > 0 && rc == SS_OK) {
+        const int key = (s->pending ? 2 : 0) + (s->x == s->x_buf0 ? 0 : 1);
+        if (s->steps_since_sort == 0 && nsteps >= (uint64_t)K) {
+            if (!s->graph[key])
+                rc = capture_period<T>(s, key);
+            else
+                advance_tiled(s, K);
+            nsteps -= (uint64_t)K;
+            continue;
+        }
+        const int phase = (int)s->steps_since_sort;
+        const int len = (int)std::min<uint64_t>(nsteps, (uint64_t)(s->resort - phase));
+        auto same = [&](const ss_sim::Segment& g) { return g.phase == phase && g.len == len && g.key == key; };
+        if (std::find_if(s->segments.begin(), s->segments.end(), same) != s->segments.end() ||
+            std::find_if(made.begin(), made.end(), same) != made.end()) {
+            advance_tiled(s, len);
+        } else {
+            hipGraphExec_t exec = nullptr;
+            rc = capture_steps<T>(s, len, &exec);
+            if (rc == SS_OK) made.push_back({phase, len, key, exec});
+        }
+        nsteps -= (uint64_t)len;
+    }
+    restore_book(s, book);
+    for (auto& g : made) {
+        if (rc != SS_OK) {
+            (void)hipGraphExecDestroy(g.exec);
+            continue;
+        }
+        if (s->segments.size() >= MAX_SEGMENTS) {
+            (void)hipGraphExecDestroy(s->segments.front().exec);
+            s->segments.erase(s->segments.begin());
+        }
+        s->segments.push_back(g);
+    }
+    return rc;
 }
 
 static int alloc(ss_sim* s, void** p, size_t bytes) {
@@ -1023,6 +1203,9 @@ static void drop_graphs(ss_sim* s) {
             (void)hipGraphExecDestroy(s->graph[k]);
             s->graph[k] = nullptr;
         }
+    for (auto& g : s->segments) (void)hipGraphExecDestroy(g.exec);
+    s->segments.clear();
+    s->seen.clear();
 }
 
 static void free_all(ss_sim* s) {
@@ -1531,6 +1714,14 @@ int ss_step(ss_sim* s, uint64_t nsteps, uint64_t* timestep_out) {
     SS_HIP_CHECK(hipStreamSynchronize(s->stream));
     if (timestep_out) *timestep_out = s->timestep;
     return SS_OK;
+}
+
+int ss_prepare_steps(ss_sim* s, uint64_t nsteps) {
+    if (!s) return fail(SS_ERR_INVALID, "NULL handle");
+    if (!s->initialised) return fail(SS_ERR_STATE, "ss_prepare_steps before ss_init");
+    if (s->in_ensemble) return fail(SS_ERR_STATE, "an ensemble member steps with ss_ensemble_step");
+    SS_HIP_CHECK(hipSetDevice(s->p.device));
+    return s->prec == SS_FP64 ? prepare_steps<double>(s, nsteps) : prepare_steps<float>(s, nsteps);
 }
 
 int ss_get_timestep(const ss_sim* s, uint64_t* t) {

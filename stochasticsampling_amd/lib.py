@@ -51,6 +51,7 @@ def _declare(L):
         "ss_profile_steps": (C.c_int, [vp, C.c_uint64, _d, C.c_char_p, C.c_int, C.c_int]),
         "ss_synchronize": (C.c_int, [vp]),
         "ss_resort": (C.c_int, [vp]),
+        "ss_prepare_steps": (C.c_int, [vp, C.c_uint64]),
         "ss_get_diagnostics": (C.c_int, [vp, _d]),
         "ss_get_info": (C.c_int, [vp, _P(Info)]),
         "ss_ensemble_create": (C.c_int, [pp, C.c_int, _P(vp)]),

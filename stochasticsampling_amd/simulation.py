@@ -149,6 +149,11 @@ class Simulation:
         """Start a new re-sort period at the next step (ss_resort)."""
         _l.check(self.L.ss_resort(self._h))
 
+    def prepare_steps(self, nsteps):
+        """Capture the graphs the next ``step(nsteps)`` replays, running
+        nothing (ss_prepare_steps): the set-up of a run of steps."""
+        _l.check(self.L.ss_prepare_steps(self._h, int(nsteps)))
+
     # getters ----------------------------------------------------------------
     def get_timestep(self):
         t = C.c_uint64()

@@ -722,3 +722,68 @@ def test_keep_distribution_switched_at_run_time(oracle):
     a.step(9)
     b.step(9)
     np.testing.assert_array_equal(a.get_particles(), b.get_particles())
+
+
+@pytest.mark.parametrize("precision", [ss.FP64, ss.FP32])
+def test_stepping_in_chunks_equals_one_call(precision):
+    """Runs of steps inside a re-sort period replay captured graphs once a
+    caller repeats them (the first request of each phase, length and buffer
+    state runs eagerly); the bookkeeping of a replay (the re-sort a run
+    starts with, the next re-sort's offsets a run ends with) must match
+    eager stepping: 45 steps in chunks of 3 and 5 with a re-sort every 8
+    steps give the same particles and fields bit for bit as one call."""
+    p = ss.make_params(box=(10.0, 10.0, 10.0), grid=(16, 16, 16, 8, 8), n=1 << 14, seed=7, precision=precision,
+                       binning=ss.BIN_SORTED, resort_interval=8)
+    parts = ss.initial_condition(ss.make_params(box=(10.0, 10.0, 10.0), grid=(16, 16, 16, 8, 8), n=1 << 14),
+                                 amplitude=0.05, seed=7)
+    a = ss.Simulation(p)
+    a.init(parts)
+    a.step(45)
+    b = ss.Simulation(p)
+    b.init(parts)
+    chunks = [3] * 5 + [5] * 3 + [3] * 5
+    assert sum(chunks) == 45
+    for c in chunks:
+        b.step(c)
+    assert a.get_timestep() == b.get_timestep() == 45
+    np.testing.assert_array_equal(a.get_particles(), b.get_particles())
+    ua, ga = a.get_flow_field(with_gradient=True)
+    ub, gb = b.get_flow_field(with_gradient=True)
+    np.testing.assert_array_equal(ua, ub)
+    np.testing.assert_array_equal(ga, gb)
+
+
+@pytest.mark.parametrize("precision", [ss.FP64, ss.FP32])
+def test_prepared_steps_equal_unprepared(precision):
+    """ss_prepare_steps captures the graphs of the next steps without running
+    anything: the state it leaves is the state it found (the timestep, the
+    particles), and the prepared steps give the same particles and fields
+    bit for bit as unprepared ones, from a pending re-sort and from the
+    middle of a period, across a period boundary."""
+    p = ss.make_params(box=(10.0, 10.0, 10.0), grid=(16, 16, 16, 8, 8), n=1 << 14, seed=9, precision=precision,
+                       binning=ss.BIN_SORTED, resort_interval=8)
+    parts = ss.initial_condition(ss.make_params(box=(10.0, 10.0, 10.0), grid=(16, 16, 16, 8, 8), n=1 << 14),
+                                 amplitude=0.05, seed=9)
+    runs = []
+    for prepare in (False, True):
+        sim = ss.Simulation(p)
+        sim.init(parts)
+        sim.step(3)
+        for n in (6, 11):
+            if prepare:
+                before = sim.get_particles()
+                sim.prepare_steps(n)
+                assert sim.get_timestep() in (3, 9)
+                np.testing.assert_array_equal(sim.get_particles(), before)
+            sim.step(n)
+        sim.resort()
+        if prepare:
+            sim.prepare_steps(20)
+        sim.step(20)
+        u, g = sim.get_flow_field(with_gradient=True)
+        runs.append((sim.get_timestep(), sim.get_particles(), u, g))
+    (ta, pa, ua, ga), (tb, pb, ub, gb) = runs
+    assert ta == tb == 40
+    np.testing.assert_array_equal(pa, pb)
+    np.testing.assert_array_equal(ua, ub)
+    np.testing.assert_array_equal(ga, gb)
