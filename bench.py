@@ -255,6 +255,7 @@ def magnetic_case(parts, steps, warmup, profile_steps):
     sim.init(parts)
     sim.step(warmup)
     sim.resort()
+    sim.prepare_steps(steps)
     sim.synchronize()
     t0 = time.perf_counter()
     sim.step(steps)
