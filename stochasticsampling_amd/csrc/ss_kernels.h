@@ -167,14 +167,19 @@ struct SolveFlags {
     int mag;      // P-hat present
     int full;
     int want_e, want_b, want_gb;
+    int zz_trace;  // flow_point: sigma_zz from the trace (the component is not binned)
 };
 
 // Stokes solve at one wave vector (spectral_solver.rs:133-189): v = sigma k,
 // v -= kn (kn . v), u = i v (1/k^2 + screening) / N.  sig(q) returns stress
 // component q: the 6 of a symmetric sigma (xx xy xz yy yz zz) or all 9.
+// zz_trace (6 components): sigma_zz = -(sigma_xx + sigma_yy), sig(5) not
+// read.  Every stress kernel of the path is traceless (the active nn - I/3,
+// the antisymmetric magnetic kernel, the rods kernel of stress.rs:144-159),
+// so the tiled path sums and transforms 5 of the 6 components.
 template <typename T, typename Sig>
 __device__ inline void flow_point(Sig sig, int nsig, const T k[3], T ki, T ik2, T screening, T norm, T ur[3],
-                                  T ui[3]) {
+                                  T ui[3], bool zz_trace = false) {
     // contractions fixed by the source: the solve kernel and the fused pencil
     // kernels inline this with different surroundings and must round alike
 #pragma clang fp contract(on)
@@ -187,7 +192,13 @@ __device__ inline void flow_point(Sig sig, int nsig, const T k[3], T ki, T ik2, 
         for (int j = 0; j < 3; ++j) {
             const int q = nsig == 6 ? (i <= j ? 3 * i + j - (i * (i + 1)) / 2 : 3 * j + i - (j * (j + 1)) / 2)
                                     : 3 * i + j;
-            const cplx<T> s = sig(q);
+            cplx<T> s;
+            if (zz_trace && nsig == 6 && q == 5) {
+                const cplx<T> a = sig(0), b = sig(3);
+                s = cplx<T>{-(a.x + b.x), -(a.y + b.y)};
+            } else {
+                s = sig(q);
+            }
             ar = ar + s.x * k[j];
             ai = ai + s.y * k[j];
         }
@@ -244,7 +255,7 @@ __global__ void __launch_bounds__(256)
     const T kn[3] = {k[0] * ki, k[1] * ki, k[2] * ki};
     T ur[3], ui[3];
     flow_point<T>([&](int q) { return in[(size_t)q * ns + idx]; }, f.nsig, k, ki, ik2[idx], screening, norm, ur,
-                  ui);
+                  ui, f.zz_trace != 0);
     int o = 0;
 #pragma unroll
     for (int i = 0; i < 3; ++i) out[(size_t)(o + i) * ns + idx] = cplx<T>{ur[i], ui[i]};

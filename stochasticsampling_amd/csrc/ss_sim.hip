@@ -166,6 +166,9 @@ struct ss_sim {
     // radices 2-5); spec_in then holds moments transformed along z and y only
     bool fused = false;
     bool spec_partial = false;
+    // tiled path, symmetric stress of 6 components, not paired: sigma_zz is
+    // not binned; the solve takes it from the trace (flow_point)
+    bool zz_trace = false;
     // paired fused stage: two real fields per complex transform; spec_in then
     // holds the moments packed in pairs (spec_packed)
     bool paired = false;
@@ -351,6 +354,7 @@ static SolveFlags solve_flags(const ss_sim* s, bool full) {
     f.want_e = s->use_e;
     f.want_b = s->use_b;
     f.want_gb = s->use_gb;
+    f.zz_trace = s->zz_trace ? 1 : 0;
     return f;
 }
 
@@ -440,13 +444,13 @@ static hipError_t launch_bin(ss_sim* s) {
         const dim3 grid(blocks_for(s->ns));
         if (s->nm == 6)
             hipLaunchKernelGGL((k_tile_convert<T, 6>), grid, dim3(256), 0, s->stream, s->mom_main, s->mom_inflow,
-                               s->ns, sc, rs, (cplx<T>*)s->spec_in, rho);
+                               s->ns, sc, rs, (cplx<T>*)s->spec_in, rho, s->zz_trace ? 1 : 0);
         else if (s->nm == 9)
             hipLaunchKernelGGL((k_tile_convert<T, 9>), grid, dim3(256), 0, s->stream, s->mom_main, s->mom_inflow,
-                               s->ns, sc, rs, (cplx<T>*)s->spec_in, rho);
+                               s->ns, sc, rs, (cplx<T>*)s->spec_in, rho, 0);
         else
             hipLaunchKernelGGL((k_tile_convert<T, 12>), grid, dim3(256), 0, s->stream, s->mom_main,
-                               s->mom_inflow, s->ns, sc, rs, (cplx<T>*)s->spec_in, rho);
+                               s->mom_inflow, s->ns, sc, rs, (cplx<T>*)s->spec_in, rho, 0);
     }
     return hipGetLastError();
 }
@@ -505,11 +509,11 @@ static hipError_t launch_spec_planes_fwd(ss_sim* s) {
         if (s->nm == 6)
             hipLaunchKernelGGL((k_spec_planes_fwd<T, 6, P>), grid, dim3(SPEC_PLANE_THREADS), lds, s->stream,
                                s->mom_main, s->mom_inflow, s->nx, s->fixed_scale * ms, ds, ay, az, twy, twz,
-                               (cplx<T>*)s->spec_in, rho, (T*)s->rec, s->rs);
+                               (cplx<T>*)s->spec_in, rho, (T*)s->rec, s->rs, s->zz_trace ? 1 : 0);
         else
             hipLaunchKernelGGL((k_spec_planes_fwd<T, 9, P>), grid, dim3(SPEC_PLANE_THREADS), lds, s->stream,
                                s->mom_main, s->mom_inflow, s->nx, s->fixed_scale * ms, ds, ay, az, twy, twz,
-                               (cplx<T>*)s->spec_in, rho, (T*)s->rec, s->rs);
+                               (cplx<T>*)s->spec_in, rho, (T*)s->rec, s->rs, s->zz_trace ? 1 : 0);
         return hipGetLastError();
     });
 }
@@ -526,7 +530,7 @@ static hipError_t launch_spec_pencils(ss_sim* s) {
                            (const cplx<T>*)s->spec_in, (cplx<T>*)s->spec_out, np, s->nz, s->pencils, s->nsig, nout,
                            spec_axis(s->fft.ax[0]), (const cplx<T>*)s->fft.ax[0].tw, (const T*)s->kx,
                            (const T*)s->ky, (const T*)s->kz, (const T*)s->ik2, (const T*)s->kinv,
-                           (T)s->p.hydro_screening, (T)(double)s->ns);
+                           (T)s->p.hydro_screening, (T)(double)s->ns, s->zz_trace ? 1 : 0);
         return hipGetLastError();
     });
 }
@@ -678,6 +682,7 @@ static TileArgs<T, NM> tile_args(ss_sim* s) {
     a.farbase = s->farbase;
     a.farctr = s->cursor;
     a.stage = tile_rec_bytes(s) > 0 ? 1 : 0;
+    a.zz_trace = s->zz_trace ? 1 : 0;
     return a;
 }
 
@@ -1350,6 +1355,8 @@ static int create_impl(ss_sim* s) {
     if (!s->sorted || (p.flags & SS_FLAG_KEEP_DISTRIBUTION)) A(s->hist, (size_t)s->ns * s->na * 4);
     const size_t cs = 2 * sizeof(T) * (size_t)s->ns;
     A(s->spec_in, cs * s->nm);
+    // (zeros in the components a step does not write: sigma_zz from the trace)
+    SS_HIP_CHECK(hipMemset(s->spec_in, 0, cs * s->nm));
     A(s->spec_out, cs * s->nout);
     A(s->spec_full, cs * (12 + (s->mag ? 12 : 0)));
     if (s->use_rho) A(s->rho, (size_t)s->ns * sizeof(T));
@@ -1425,7 +1432,9 @@ static int create_impl(ss_sim* s) {
     if (p.noise_mode == SS_NOISE_REFERENCE &&
         (rc = refnoise_create(&s->refn, s->n, p.noise_streams, p.seed)) != SS_OK)
         return rc;
-    return choose_spectral<T>(s);
+    rc = choose_spectral<T>(s);
+    s->zz_trace = s->sorted && s->nm == 6 && !s->paired;
+    return rc;
 }
 
 // Tile geometry of the tiled path.  Unless given, tiles start at 4^3 cells and
@@ -2607,6 +2616,7 @@ static int ens_build_spec(ss_ensemble* e) {
             a.rs = s->rs;
             a.nsig = s->nsig;
             a.nout = s->use_e ? 12 : 6;
+            a.zz_trace = s->zz_trace ? 1 : 0;
             // a batched launch has workgroups to spare: more pencils per
             // workgroup make the x-line reads longer runs of each row
             a.pb = s->pencils;

@@ -126,7 +126,7 @@ __device__ __forceinline__ void spec_planes_fwd_body(const long long* __restrict
                                             double scale, double rho_scale, const SpecAxis& ay, const SpecAxis& az,
                                             const cplx<T>* __restrict__ twy, const cplx<T>* __restrict__ twz,
                                             cplx<T>* __restrict__ spec, T* __restrict__ rho, T* __restrict__ rec,
-                                            int rs, int bid) {
+                                            int rs, int bid, int zz_trace) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     const int ny = P ? P : ay.n, nz = P ? P : az.n, np = ny * nz;
     cplx<T>* ty = reinterpret_cast<cplx<T>*>(smem_raw);
@@ -135,6 +135,7 @@ __device__ __forceinline__ void spec_planes_fwd_body(const long long* __restrict
     cplx<T>* b1 = b0 + np;
     const int x = bid % nx, q = bid / nx;
     const size_t ns = (size_t)nx * np, base = (size_t)x * np;
+    if (NM == 6 && zz_trace && q == 5) return;  // sigma_zz comes from the trace (flow_point)
     if (q == NM) {
         spec_count_plane<T, SPEC_PLANE_THREADS>(main + (size_t)NM * ns + base, inflow + (size_t)NM * ns + base, np,
                                                 rho_scale, rho, rec, base, rs);
@@ -161,9 +162,9 @@ __global__ void __launch_bounds__(SPEC_PLANE_THREADS)
     k_spec_planes_fwd(const long long* __restrict__ main, long long* __restrict__ inflow, int nx, double scale,
                       double rho_scale, SpecAxis ay, SpecAxis az, const cplx<T>* __restrict__ twy,
                       const cplx<T>* __restrict__ twz, cplx<T>* __restrict__ spec, T* __restrict__ rho,
-                      T* __restrict__ rec, int rs) {
+                      T* __restrict__ rec, int rs, int zz_trace) {
     spec_planes_fwd_body<T, NM, P>(main, inflow, nx, scale, rho_scale, ay, az, twy, twz, spec, rho, rec, rs,
-                                   (int)blockIdx.x);
+                                   (int)blockIdx.x, zz_trace);
 }
 
 // Wave vector, 1/|k| and 1/k^2 of bin (x, pencil p = y nz + z) from the
@@ -202,7 +203,7 @@ __device__ __forceinline__ void spec_pencils_body(const cplx<T>* __restrict__ in
                                          const cplx<T>* __restrict__ twx, const T* __restrict__ kx,
                                          const T* __restrict__ ky, const T* __restrict__ kz,
                                          const T* __restrict__ ik2, const T* __restrict__ kinv, T screening, T norm,
-                                         int bid, int groups) {
+                                         int bid, int groups, int zz_trace) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     const int nx = P ? P : ax.n, ls = nx + 1;
     const int nl = (nsig > nout ? nsig : nout) * pb_max;
@@ -219,9 +220,11 @@ __device__ __forceinline__ void spec_pencils_body(const cplx<T>* __restrict__ in
     for (int t = threadIdx.x; t < nx; t += SPEC_PENCIL_THREADS) tw[t] = twx[t];
     // line (c, p) at A[(c pb_max + p) ls]; lanes over p for coalesced loads
     // four loads at a time: eight take the kernel past 128 VGPRs (one
-    // workgroup per CU) where all axes have compile-time lengths
+    // workgroup per CU) where all axes have compile-time lengths; with
+    // zz_trace the last (zz) component is neither loaded nor transformed
+    const int nload = zz_trace && nsig == 6 ? 5 : nsig;
     batched_loop<SPEC_PENCIL_THREADS, 4>(
-        nsig * nx * pb_max,
+        nload * nx * pb_max,
         [&](int f) {
             const int p = f % pb_max, xx = (f / pb_max) % nx, c = f / (pb_max * nx);
             return p < pb ? in[(size_t)c * ns + (size_t)xx * np + p0 + p] : cplx<T>{T(0), T(0)};
@@ -231,7 +234,7 @@ __device__ __forceinline__ void spec_pencils_body(const cplx<T>* __restrict__ in
             A[(size_t)(c * pb_max + p) * ls + xx] = v;
         });
     __syncthreads();
-    cplx<T>* S = spec_lines<T, false, SPEC_PENCIL_THREADS, P, 0>(A, B, tw, ax, nsig * pb_max, ls, 1, -1);
+    cplx<T>* S = spec_lines<T, false, SPEC_PENCIL_THREADS, P, 0>(A, B, tw, ax, nload * pb_max, ls, 1, -1);
     cplx<T>* O = S == A ? B : A;
     for (int f = threadIdx.x; f < nx * pb_max; f += SPEC_PENCIL_THREADS) {
         const int p = f % pb_max, xx = f / pb_max;
@@ -243,7 +246,7 @@ __device__ __forceinline__ void spec_pencils_body(const cplx<T>* __restrict__ in
         const T* k = kp.k;
         T ur[3], ui[3];
         flow_point<T>([&](int q) { return S[(size_t)(q * pb_max + p) * ls + xx]; }, nsig, k, kp.ki, kp.ik2, screening,
-                      norm, ur, ui);
+                      norm, ur, ui, zz_trace != 0);
 #pragma unroll
         for (int i = 0; i < 3; ++i) O[(size_t)(i * pb_max + p) * ls + xx] = cplx<T>{ur[i], ui[i]};
 #pragma unroll
@@ -266,9 +269,9 @@ __global__ void __launch_bounds__(SPEC_PENCIL_THREADS)
     k_spec_pencils(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int np, int nz, int pb_max,
                    int nsig, int nout, SpecAxis ax, const cplx<T>* __restrict__ twx, const T* __restrict__ kx,
                    const T* __restrict__ ky, const T* __restrict__ kz, const T* __restrict__ ik2,
-                   const T* __restrict__ kinv, T screening, T norm) {
+                   const T* __restrict__ kinv, T screening, T norm, int zz_trace) {
     spec_pencils_body<T, P>(in, out, np, nz, pb_max, nsig, nout, ax, twx, kx, ky, kz, ik2, kinv, screening, norm,
-                            (int)blockIdx.x, (int)gridDim.x);
+                            (int)blockIdx.x, (int)gridDim.x, zz_trace);
 }
 
 // Inverse: field component o of plane x; the real part goes to record slot
@@ -323,6 +326,7 @@ struct SpecJob {
     T screening, norm;
     SpecAxis ax, ay, az;
     int nx, np, nz, rs, pb, nsig, nout;
+    int zz_trace;  // sigma_zz from the trace (flow_point)
 };
 
 // The system a block belongs to: the last m with first[m] <= b.
@@ -341,7 +345,7 @@ __global__ void __launch_bounds__(SPEC_PLANE_THREADS)
     const int m = spec_job_of(first, n, (int)blockIdx.x);
     const SpecJob<T>& a = jobs[m];
     spec_planes_fwd_body<T, NM, P>(a.main, a.inflow, a.nx, a.scale, a.rho_scale, a.ay, a.az, a.twy, a.twz,
-                                   a.spec_in, a.rho, a.rec, a.rs, (int)blockIdx.x - first[m]);
+                                   a.spec_in, a.rho, a.rec, a.rs, (int)blockIdx.x - first[m], a.zz_trace);
 }
 
 template <typename T, int P>
@@ -350,7 +354,8 @@ __global__ void __launch_bounds__(SPEC_PENCIL_THREADS)
     const int m = spec_job_of(first, n, (int)blockIdx.x);
     const SpecJob<T>& a = jobs[m];
     spec_pencils_body<T, P>(a.spec_in, a.spec_out, a.np, a.nz, a.pb, a.nsig, a.nout, a.ax, a.twx, a.kx, a.ky, a.kz,
-                            a.ik2, a.kinv, a.screening, a.norm, (int)blockIdx.x - first[m], first[m + 1] - first[m]);
+                            a.ik2, a.kinv, a.screening, a.norm, (int)blockIdx.x - first[m], first[m + 1] - first[m],
+                            a.zz_trace);
 }
 
 template <typename T, int P>

@@ -88,6 +88,7 @@ struct TileArgs {
     const uint32_t* farbase;  // [ntiles]
     uint32_t* farctr;       // [ntiles]
     int stage;              // 1: the window's cell records are staged in LDS (8-value records)
+    int zz_trace;           // NM = 6: sigma_zz is not summed (flow_point derives it from the trace)
 };
 
 // Window-slot tables: entry c of an axis is the window coordinate of cell c
@@ -204,6 +205,8 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
     // the count component (the density) is summed only when the density
     // enters the step (volume_exclusion != 0): nothing else reads it
     const bool counts = FULL && a.P.use_rho;
+    // sigma_zz (component 5 of the symmetric 6) from the trace instead
+    const bool skip_zz = NM == 6 && a.zz_trace;
     const int tz = tile % tg.nt[2], ty = (tile / tg.nt[2]) % tg.nt[1], tx = tile / (tg.nt[2] * tg.nt[1]);
     const int ox = tx * tg.t[0], oy = ty * tg.t[1], oz = tz * tg.t[2];
 
@@ -410,15 +413,17 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
             lds_i64* dst = (lds_i64*)acc + wslot * NC;
 #pragma unroll
             for (int q = 0; q < NM; ++q)
-                __hip_atomic_fetch_add(dst + q, (q & 1) ? kv[q / 2].y : kv[q / 2].x, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (!(skip_zz && q == 5))
+                    __hip_atomic_fetch_add(dst + q, (q & 1) ? kv[q / 2].y : kv[q / 2].x, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
             if (counts) __hip_atomic_fetch_add(dst + NM, 1ll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         } else {
             const int cell = (ix * g.ny + iy) * g.nz + iz;
             long long* dst = a.inflow + cell;
 #pragma unroll
             for (int q = 0; q < NM; ++q)
-                atomicAdd(reinterpret_cast<unsigned long long*>(dst + (size_t)q * g.ns), (unsigned long long)kr[q]);
+                if (!(skip_zz && q == 5))
+                    atomicAdd(reinterpret_cast<unsigned long long*>(dst + (size_t)q * g.ns), (unsigned long long)kr[q]);
             if (counts) atomicAdd(reinterpret_cast<unsigned long long*>(dst + (size_t)NM * g.ns), 1ull);
         }
         if (KEEP) {
@@ -440,7 +445,8 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
         const long long* src = acc + (size_t)slot * NC;
         if (own && owner) {
 #pragma unroll
-            for (int q = 0; q < NM; ++q) a.main[(size_t)q * g.ns + cell] = src[q];
+            for (int q = 0; q < NM; ++q)
+                if (!(skip_zz && q == 5)) a.main[(size_t)q * g.ns + cell] = src[q];
             if (counts) a.main[(size_t)NM * g.ns + cell] = src[NM];
         } else {
             long long any = counts ? src[NM] : 0;
@@ -449,8 +455,9 @@ __device__ inline void tile_pass(const TileArgs<T, NM>& a, int tile, uint32_t q0
             if (any != 0) {
 #pragma unroll
                 for (int q = 0; q < NM; ++q)
-                    atomicAdd(reinterpret_cast<unsigned long long*>(a.inflow + (size_t)q * g.ns + cell),
-                              (unsigned long long)src[q]);
+                    if (!(skip_zz && q == 5))
+                        atomicAdd(reinterpret_cast<unsigned long long*>(a.inflow + (size_t)q * g.ns + cell),
+                                  (unsigned long long)src[q]);
                 if (counts)
                     atomicAdd(reinterpret_cast<unsigned long long*>(a.inflow + (size_t)NM * g.ns + cell),
                               (unsigned long long)src[NM]);
@@ -543,12 +550,14 @@ __global__ void __launch_bounds__(256)
 template <typename T, int NM>
 __global__ void __launch_bounds__(256)
     k_tile_convert(const long long* __restrict__ main, long long* __restrict__ inflow, int ns, double scale,
-                   double rho_scale, cplx<T>* __restrict__ spec, T* __restrict__ rho) {
+                   double rho_scale, cplx<T>* __restrict__ spec, T* __restrict__ rho, int zz_trace) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ns) return;
 #pragma unroll
     for (int q = 0; q < NM; ++q) {
-        const long long v = main[(size_t)q * ns + c] + inflow[(size_t)q * ns + c];
+        // (sigma_zz from the trace: the component is not summed, 0 here)
+        const long long v =
+            NM == 6 && zz_trace && q == 5 ? 0 : main[(size_t)q * ns + c] + inflow[(size_t)q * ns + c];
         spec[(size_t)q * ns + c] = cplx<T>{(T)((double)v * scale), T(0)};
         inflow[(size_t)q * ns + c] = 0;
     }
