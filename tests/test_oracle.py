@@ -439,3 +439,31 @@ def test_quasi2d_clamp(oracle):
     sim.step(2)
     out = sim.particles()
     assert np.all(out[:, 2] == 0.0) and np.all(out[:, 4] == PI / 2)
+
+
+@pytest.mark.parametrize("alpha,beta,shape,direction", [
+    (-1.0, 0.0, 0.0, (0.0, 0.0, 0.0)),
+    (0.0, 1.0, 0.0, (0.0, 0.0, 0.0)),
+    (0.0, 1.0, 0.0, (0.3, -0.5, 0.8)),
+    (0.0, 0.0, 1.0, (0.0, 0.0, 0.0)),
+    (1.7, -0.6, 2.3, (0.0, 0.6, 0.8)),
+])
+def test_stress_kernels_are_traceless(oracle, alpha, beta, shape, direction):
+    """Every stress kernel of the step is traceless: the active nn - I/3, the
+    antisymmetric magnetic kernel and the rods kernel (stress.rs:103-159).
+    The tiled path relies on it: it takes sigma_zz = -(sigma_xx + sigma_yy)
+    instead of summing and transforming the zz component.  Checked on the
+    bin centres of a 16 x 12 angle grid to FP64 rounding."""
+    p = make_params(box=(10.0, 10.0, 10.0), grid=(4, 4, 4, 16, 12), n=8, stress_active=alpha,
+                    stress_magnetic=beta, shape=shape, field_direction=direction)
+    K = oracle.stress_kernel(p)  # [3][3][nphi][ntheta]
+    trace = K[0, 0] + K[1, 1] + K[2, 2]
+    scale = np.abs(K).max()
+    assert scale > 0.1
+    assert np.abs(trace).max() < 1e-15 * scale * 8
+    # a kernel with a trace would fail: the active kernel without the -I/3
+    K2 = K.copy()
+    for a in range(3):
+        K2[a, a] += alpha / 3.0
+    if alpha != 0.0:
+        assert np.abs(K2[0, 0] + K2[1, 1] + K2[2, 2]).max() > 0.5 * abs(alpha)
