@@ -29,7 +29,7 @@ struct Math<double> {
     __device__ static inline double floor(double a) { return ::floor(a); }
     __device__ static inline double fmod(double a, double b) { return ::fmod(a, b); }
     __device__ static inline double fabs(double a) { return ::fabs(a); }
-    __device__ static inline void sincos_angle(double a, double* s, double* c) { ::sincos(a, s, c); }
+    __device__ static inline void sincos_angle(double a, double* s, double* c);
     __device__ static inline double atan2_angle(double y, double x) { return ::atan2(y, x); }
 };
 
@@ -84,6 +84,95 @@ __device__ inline float atan2_angle_f32(float y, float x) {
     if (__builtin_signbit(x)) r = 3.14159265358979324f - r;
     return __builtin_signbit(y) ? -r : r;
 }
+
+// sin and cos of an angle in binary64, for the angles of the path (particle
+// angles, axis angles, half rotation angles, 2 pi u of the noise; |x| up to
+// a few times 2 pi): quadrant k = rint(x 2/pi) and r = x - k pi/2 as a
+// double-length pair rh + rl, with pi/2 split in three parts (Cody and
+// Waite; the first two of 31 and 32 significant bits, so that k P1 and k P2
+// are exact for |k| < 2^21, the second difference taken exactly by
+// Knuth's two-sum); then the Taylor polynomials of sin (to r^17) and cos (to
+// r^16) on |r| <= pi/4, whose truncation errors (8e-20, 3e-18) are far below
+// an ulp, with the first-order term of rl added to each.  Within 1 ulp of
+// the exact values.  Beyond |x| = 2^20 (never reached by the path) the
+// library.
+__host__ __device__ inline void sincos_angle_f64(double x, double* s, double* c) {
+#pragma clang fp contract(on)
+    if (!(__builtin_fabs(x) < 1048576.0)) {
+        ::sincos(x, s, c);
+        return;
+    }
+    const double k = __builtin_rint(x * 0.6366197723675814);
+    const double r1 = __builtin_fma(-k, 1.5707963267341256, x);  // exact
+    const double w = k * 6.077100506303966e-11;                    // exact
+    const double rh = r1 - w;
+    const double bb = rh - r1;
+    const double rl = ((r1 - (rh - bb)) - (w + bb)) - k * 2.0222662487959506e-21;
+    const double z = rh * rh;
+    double ps = __builtin_fma(z, 2.8114572543455206e-15, -7.647163731819816e-13);
+    ps = __builtin_fma(z, ps, 1.6059043836821613e-10);
+    ps = __builtin_fma(z, ps, -2.505210838544172e-08);
+    ps = __builtin_fma(z, ps, 2.7557319223985893e-06);
+    ps = __builtin_fma(z, ps, -0.0001984126984126984);
+    ps = __builtin_fma(z, ps, 0.008333333333333333);
+    ps = __builtin_fma(z, ps, -0.16666666666666666);
+    // sin(rh + rl) = sin rh + rl cos rh; cos rh = 1 - z/2 to the order kept
+    const double sr = rh + __builtin_fma(rh * z, ps, __builtin_fma(-0.5 * z, rl, rl));
+    double pc = __builtin_fma(z, 4.779477332387385e-14, -1.1470745597729725e-11);
+    pc = __builtin_fma(z, pc, 2.08767569878681e-09);
+    pc = __builtin_fma(z, pc, -2.755731922398589e-07);
+    pc = __builtin_fma(z, pc, 2.48015873015873e-05);
+    pc = __builtin_fma(z, pc, -0.001388888888888889);
+    pc = __builtin_fma(z, pc, 0.041666666666666664);
+    // cos(rh + rl) = cos rh - rl sin rh; 1 - z/2 split so that the large
+    // part is exact: hz = z/2 rounded, its rounding kept in the tail
+    const double hz = 0.5 * z;
+    const double one_hz = 1.0 - hz;
+    const double tail = ((1.0 - one_hz) - hz) + (__builtin_fma(rh, rh, -z) * -0.5);
+    const double cr = one_hz + (__builtin_fma(z * z, pc, tail) - rl * rh);
+    const int q = (int)(long long)k;
+    const double a = (q & 1) ? cr : sr, b = (q & 1) ? sr : cr;
+    *s = x == 0.0 ? x : ((q & 2) ? -a : a);
+    *c = ((q + 1) & 2) ? -b : b;
+}
+
+// ln u for u in (0, 1] (the uniforms of the noise, at least 2^-53): u = 2^e m
+// with m = 1 + f in [sqrt(1/2), sqrt(2)) (f exact), s = f/(2 + f) in
+// [-0.172, 0.172], ln m = 2 atanh s = 2s + s Q(s^2), Q = 2s^2/3 + 2s^4/5 +
+// ... to s^20 (truncation 2e-19), and with 2s = f - s f, ln m = f - s (f -
+// Q): the exact f carries the result and the rounded s enters only a
+// correction a sixth its size.  ln 2 split in two parts (the first of 42
+// bits, so that e ln2_hi is exact).  Within 1 ulp.  Other arguments: the
+// library.
+__host__ __device__ inline double log_unit_f64(double u) {
+#pragma clang fp contract(on)
+    if (!(u >= 2.2250738585072014e-308 && u <= 1.0)) return ::log(u);
+    const unsigned long long bits = __builtin_bit_cast(unsigned long long, u);
+    int e = (int)((bits >> 52) & 0x7FFull) - 1022;
+    double m = __builtin_bit_cast(double, (bits & 0x000FFFFFFFFFFFFFull) | 0x3FE0000000000000ull);
+    if (m < 0.7071067811865476) {
+        m = m * 2.0;
+        e -= 1;
+    }
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    double p = __builtin_fma(z, 0.09523809523809523, 0.10526315789473684);
+    p = __builtin_fma(z, p, 0.11764705882352941);
+    p = __builtin_fma(z, p, 0.13333333333333333);
+    p = __builtin_fma(z, p, 0.15384615384615385);
+    p = __builtin_fma(z, p, 0.18181818181818182);
+    p = __builtin_fma(z, p, 0.2222222222222222);
+    p = __builtin_fma(z, p, 0.2857142857142857);
+    p = __builtin_fma(z, p, 0.4);
+    p = __builtin_fma(z, p, 0.6666666666666666);
+    const double Q = z * p;
+    const double de = (double)e;
+    const double lm = f - (s * (f - Q) - de * 5.497923018708371e-14);
+    return __builtin_fma(de, 0.6931471805598903, lm);
+}
+
+__device__ inline void Math<double>::sincos_angle(double a, double* s, double* c) { sincos_angle_f64(a, s, c); }
 
 template <>
 struct Math<float> {
@@ -194,13 +283,16 @@ __device__ inline void philox_noise(uint64_t seed, uint64_t step, uint32_t id, d
         u[k] = ((double)m + 0.5) * (1.0 / 4503599627370496.0);
     }
     const double TWOPI = Math<double>::TWOPI;
-    const double r0 = ::sqrt(-2.0 * ::log(u[0]));
-    const double r1 = ::sqrt(-2.0 * ::log(u[2]));
-    nz.x = r0 * ::cos(TWOPI * u[1]);
-    nz.y = r0 * ::sin(TWOPI * u[1]);
-    nz.z = r1 * ::cos(TWOPI * u[3]);
-    ::sincos(TWOPI * u[4], &nz.sa, &nz.ca);
-    nz.rot = dr * ::sqrt(-2.0 * ::log(1.0 - u[5]));
+    const double r0 = ::sqrt(-2.0 * log_unit_f64(u[0]));
+    const double r1 = ::sqrt(-2.0 * log_unit_f64(u[2]));
+    double sn1, cs1, sn3, cs3;
+    sincos_angle_f64(TWOPI * u[1], &sn1, &cs1);
+    sincos_angle_f64(TWOPI * u[3], &sn3, &cs3);
+    nz.x = r0 * cs1;
+    nz.y = r0 * sn1;
+    nz.z = r1 * cs3;
+    sincos_angle_f64(TWOPI * u[4], &nz.sa, &nz.ca);
+    nz.rot = dr * ::sqrt(-2.0 * log_unit_f64(1.0 - u[5]));
 }
 
 // (m + 1/2) 2^-52 for m = f 2^31 + t (f: 21 bits, t: 31 bits) in binary32:
@@ -380,10 +472,15 @@ __host__ __device__ inline bool full_physics(const UpdateParams<T>& P) {
 // Outputs the position increment dp (to be added with pbc) and the new
 // orientation angles after ang_pbc.
 // sin and cos of the half rotation angle: a short Taylor polynomial in FP32
-// (|x| < 0.5, error below 1e-8), the library functions otherwise.
+// (|x| < 0.5, error below 1e-8), the library functions otherwise; in FP64
+// the reduced-range sin and cos above.
 template <typename T>
 __device__ inline void half_angle(T x, T* s, T* c) {
 #pragma clang fp contract(on)
+    if constexpr (sizeof(T) == 8) {
+        sincos_angle_f64(x, s, c);
+        return;
+    }
     if constexpr (sizeof(T) == 4) {
         if (x < 0.5f && x > -0.5f) {
             const float x2 = x * x;
