@@ -1016,6 +1016,15 @@ static void advance_tiled(ss_sim* s, int len) {
 
 constexpr size_t MAX_SEGMENTS = 8;
 
+// Destroy a captured graph that may have a launch still queued or running on
+// `st`: the executable owns the kernel arguments of its nodes, so it is only
+// released once the stream has drained.
+static void destroy_exec(hipStream_t st, hipGraphExec_t g) {
+    if (!g) return;
+    if (st) (void)hipStreamSynchronize(st);
+    (void)hipGraphExecDestroy(g);
+}
+
 // nsteps steps: whole periods as graph launches; on the tiled path any other
 // run of steps inside one re-sort period as a launch of a captured segment
 // (captured the second time a run of its phase, length and buffer state is
@@ -1070,7 +1079,7 @@ static int run_steps(ss_sim* s, uint64_t nsteps) {
             hipGraphExec_t exec = nullptr;
             if (it == s->segments.end()) {
                 if (s->segments.size() >= MAX_SEGMENTS) {
-                    (void)hipGraphExecDestroy(s->segments.front().exec);
+                    destroy_exec(s->stream, s->segments.front().exec);
                     s->segments.erase(s->segments.begin());
                 }
                 int rc = capture_steps<T>(s, len, &exec);
@@ -1187,7 +1196,7 @@ static int prepare_steps(ss_sim* s, uint64_t nsteps) {
             continue;
         }
         if (s->segments.size() >= MAX_SEGMENTS) {
-            (void)hipGraphExecDestroy(s->segments.front().exec);
+            destroy_exec(s->stream, s->segments.front().exec);
             s->segments.erase(s->segments.begin());
         }
         s->segments.push_back(g);
@@ -1205,10 +1214,10 @@ static int alloc(ss_sim* s, void** p, size_t bytes) {
 static void drop_graphs(ss_sim* s) {
     for (int k = 0; k < 4; ++k)
         if (s->graph[k]) {
-            (void)hipGraphExecDestroy(s->graph[k]);
+            destroy_exec(s->stream, s->graph[k]);
             s->graph[k] = nullptr;
         }
-    for (auto& g : s->segments) (void)hipGraphExecDestroy(g.exec);
+    for (auto& g : s->segments) destroy_exec(s->stream, g.exec);
     s->segments.clear();
     s->seen.clear();
 }
@@ -2757,7 +2766,7 @@ static int ens_jobs_for_current(ss_ensemble* e) {
     if (flipped) return 1;
     for (int k = 0; k < 4; ++k)
         if (e->graph[k]) {
-            (void)hipGraphExecDestroy(e->graph[k]);
+            destroy_exec(e->stream, e->graph[k]);
             e->graph[k] = nullptr;
         }
     int rc = e->nm == 6 ? ens_build_jobs<T, 6>(e) : e->nm == 9 ? ens_build_jobs<T, 9>(e) : ens_build_jobs<T, 12>(e);

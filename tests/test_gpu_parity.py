@@ -754,6 +754,35 @@ def test_stepping_in_chunks_equals_one_call(precision):
 
 
 @pytest.mark.parametrize("precision", [ss.FP64, ss.FP32])
+def test_more_segment_shapes_than_kept(precision):
+    """Four ways of splitting a re-sort period of 16 steps, each used for
+    four periods in a row, make about thirty distinct runs (phase, length,
+    buffer state) that are asked for twice and captured, against eight
+    captured segments kept, so most captured segments are evicted again;
+    256 steps in such chunks equal one call bit for bit."""
+    p = ss.make_params(box=(10.0, 10.0, 10.0), grid=(16, 16, 16, 8, 8), n=1 << 14, seed=5, precision=precision,
+                       binning=ss.BIN_SORTED, resort_interval=16)
+    parts = ss.initial_condition(ss.make_params(box=(10.0, 10.0, 10.0), grid=(16, 16, 16, 8, 8), n=1 << 14),
+                                 amplitude=0.05, seed=5)
+    a = ss.Simulation(p)
+    a.init(parts)
+    a.step(256)
+    b = ss.Simulation(p)
+    b.init(parts)
+    splits = ([1, 2, 3, 4, 6], [6, 4, 3, 2, 1], [5, 5, 5, 1], [2, 7, 7])
+    chunks = [c for split in splits for _ in range(4) for c in split]
+    assert sum(chunks) == 256
+    for c in chunks:
+        b.step(c)
+    assert a.get_timestep() == b.get_timestep() == 256
+    np.testing.assert_array_equal(a.get_particles(), b.get_particles())
+    ua, ga = a.get_flow_field(with_gradient=True)
+    ub, gb = b.get_flow_field(with_gradient=True)
+    np.testing.assert_array_equal(ua, ub)
+    np.testing.assert_array_equal(ga, gb)
+
+
+@pytest.mark.parametrize("precision", [ss.FP64, ss.FP32])
 def test_prepared_steps_equal_unprepared(precision):
     """ss_prepare_steps captures the graphs of the next steps without running
     anything: the state it leaves is the state it found (the timestep, the
