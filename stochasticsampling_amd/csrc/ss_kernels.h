@@ -217,6 +217,34 @@ __device__ inline void flow_point(Sig sig, int nsig, const T k[3], T ki, T ik2, 
     }
 }
 
+// Magnetic solve at one wave vector (magnetic_solver.rs:88-94): b = (P -
+// kn (kn . P)) / N, the k = 0 mode kept.  pq(q) returns P-hat component q.
+// Shared by the solve kernel and the fused pencil kernel, which must round
+// alike.
+template <typename T, typename Pq>
+__device__ inline void mag_point(Pq pq, const T kn[3], T norm, T br[3], T bi[3]) {
+#pragma clang fp contract(on)
+    T kr = T(0), kim = T(0);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const cplx<T> v = pq(q);
+        kr = kr + kn[q] * v.x;
+        kim = kim + kn[q] * v.y;
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const cplx<T> v = pq(q);
+        br[q] = (v.x - kn[q] * kr) / norm;
+        bi[q] = (v.y - kn[q] * kim) / norm;
+    }
+}
+
+// grad b_ab = i k_a b_b
+template <typename T>
+__device__ inline cplx<T> grad_b(const T k[3], const T br[3], const T bi[3], int a, int b) {
+    return cplx<T>{-(k[a] * bi[b]), k[a] * br[b]};
+}
+
 // G_ab = i k_a u_b (spectral_solver.rs:192-229)
 template <typename T>
 __device__ inline cplx<T> grad_u(const T k[3], const T ur[3], const T ui[3], int a, int b) {
@@ -275,22 +303,9 @@ __global__ void __launch_bounds__(256)
         }
     }
     if (f.mag && (f.full || f.want_b || f.want_gb)) {
-        // b = (P - kn (kn . P)) / N, k = 0 kept (magnetic_solver.rs:88-94)
         T br[3], bi[3];
         const cplx<T>* P = in + (size_t)f.nsig * ns;
-        T kr = T(0), kim = T(0);
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const cplx<T> pq = P[(size_t)q * ns + idx];
-            kr = kr + kn[q] * pq.x;
-            kim = kim + kn[q] * pq.y;
-        }
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const cplx<T> pq = P[(size_t)q * ns + idx];
-            br[q] = (pq.x - kn[q] * kr) / norm;
-            bi[q] = (pq.y - kn[q] * kim) / norm;
-        }
+        mag_point<T>([&](int q) { return P[(size_t)q * ns + idx]; }, kn, norm, br, bi);
         if (f.full || f.want_b) {
 #pragma unroll
             for (int q = 0; q < 3; ++q) out[(size_t)(o + q) * ns + idx] = cplx<T>{br[q], bi[q]};
@@ -298,8 +313,7 @@ __global__ void __launch_bounds__(256)
         }
         if (f.full || f.want_gb) {
             for (int a = 0; a < 3; ++a)
-                for (int b = 0; b < 3; ++b)
-                    out[(size_t)(o + 3 * a + b) * ns + idx] = cplx<T>{-(k[a] * bi[b]), k[a] * br[b]};
+                for (int b = 0; b < 3; ++b) out[(size_t)(o + 3 * a + b) * ns + idx] = grad_b(k, br, bi, a, b);
             o += 9;
         }
     }

@@ -518,11 +518,16 @@ static hipError_t launch_spec_planes_fwd(ss_sim* s) {
     });
 }
 
+// The fields after u and W that the fused stage produces (SPEC_OUT_*).
+static int spec_outs(const ss_sim* s) {
+    return (s->use_e ? SPEC_OUT_E : 0) | (s->mag && s->use_b ? SPEC_OUT_B : 0) | (s->mag && s->use_gb ? SPEC_OUT_GB : 0);
+}
+
 template <typename T>
 static hipError_t launch_spec_pencils(ss_sim* s) {
     const int np = s->ny * s->nz;
-    const int nout = s->use_e ? 12 : 6;
-    const size_t lds = spec_pencil_lds(s->nx, std::max(s->nsig, nout) * s->pencils, sizeof(T));
+    const int nout = s->nout, nmag = s->mag ? 3 : 0;
+    const size_t lds = spec_pencil_lds(s->nx, std::max(s->nsig + nmag, nout) * s->pencils, sizeof(T));
     const dim3 grid((unsigned)((np + s->pencils - 1) / s->pencils));
     return with_spec_p(s->spec_p, [&](auto pc) {
         constexpr int P = decltype(pc)::value;
@@ -530,7 +535,7 @@ static hipError_t launch_spec_pencils(ss_sim* s) {
                            (const cplx<T>*)s->spec_in, (cplx<T>*)s->spec_out, np, s->nz, s->pencils, s->nsig, nout,
                            spec_axis(s->fft.ax[0]), (const cplx<T>*)s->fft.ax[0].tw, (const T*)s->kx,
                            (const T*)s->ky, (const T*)s->kz, (const T*)s->ik2, (const T*)s->kinv,
-                           (T)s->p.hydro_screening, (T)(double)s->ns, s->zz_trace ? 1 : 0);
+                           (T)s->p.hydro_screening, (T)(double)s->ns, s->zz_trace ? 1 : 0, nmag, spec_outs(s));
         return hipGetLastError();
     });
 }
@@ -594,14 +599,15 @@ static hipError_t launch_spec_planes_inv2(ss_sim* s) {
 
 template <typename T>
 static hipError_t launch_spec_planes_inv(ss_sim* s) {
-    const int nout = s->use_e ? 12 : 6;
+    const int nout = s->nout;
     const size_t lds = spec_plane_lds(s->ny, s->nz, sizeof(T), s->spec_p > 0);
     const dim3 grid((unsigned)(s->nx * nout));
     return with_spec_p(s->spec_p, [&](auto pc) {
         constexpr int P = decltype(pc)::value;
         hipLaunchKernelGGL((k_spec_planes_inv<T, P>), grid, dim3(SPEC_PLANE_THREADS), lds, s->stream,
                            (const cplx<T>*)s->spec_out, s->nx, spec_axis(s->fft.ax[1]), spec_axis(s->fft.ax[2]),
-                           (const cplx<T>*)s->fft.ax[1].tw, (const cplx<T>*)s->fft.ax[2].tw, (T*)s->rec, s->rs);
+                           (const cplx<T>*)s->fft.ax[1].tw, (const cplx<T>*)s->fft.ax[2].tw, (T*)s->rec, s->rs,
+                           spec_outs(s), (T)s->p.magnetic_dipole_dipole);
         return hipGetLastError();
     });
 }
@@ -1233,16 +1239,18 @@ static void free_all(ss_sim* s) {
     s->stream = nullptr;
 }
 
-// Fused spectral stage when it applies: tiled path, no magnetic solve, FFT
-// radices 2-5, a y-z plane (two buffers) within the workgroup LDS limit (at
-// most 144 KB).  The pencil group is the largest power of two up to 16 whose
-// buffers fit in 64 KB.
+// Fused spectral stage when it applies: tiled path, FFT radices 2-5, a y-z
+// plane (two buffers) within the workgroup LDS limit (at most 144 KB); with
+// the magnetic solve only for a symmetric stress (6 stress and 3
+// polarisation moments) and without pairing.  The pencil group is the
+// largest power of two up to 4 whose buffers fit in 64 KB.
 template <typename T>
 static int choose_spectral(ss_sim* s) {
     s->fused = false;
-    if (!s->sorted || s->mag || (s->p.flags & SS_FLAG_AXIS_PASSES) || s->fft.variant != SS_FFT_RADIX ||
+    if (!s->sorted || (s->p.flags & SS_FLAG_AXIS_PASSES) || s->fft.variant != SS_FFT_RADIX ||
         !s->fft.small_radices())
         return SS_OK;
+    if (s->mag && (s->nsig != 6 || (s->p.flags & SS_FLAG_PAIRED_SPECTRAL))) return SS_OK;
     int lim = 0;
     SS_HIP_CHECK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, s->p.device));
     const size_t cap = std::min<size_t>((size_t)lim, 144 * 1024);
@@ -1251,7 +1259,7 @@ static int choose_spectral(ss_sim* s) {
     for (int P : {16, 32, 48, 64})
         if (ct_plan(a0, P) && ct_plan(a1, P) && ct_plan(a2, P)) s->spec_p = P;
     const size_t plane = spec_plane_lds(s->ny, s->nz, sizeof(T), s->spec_p > 0);
-    const int nmax = std::max(s->nsig, s->nout);
+    const int nmax = std::max(s->nsig + (s->mag ? 3 : 0), s->nout);
     // four pencils (pencil pairs) per workgroup: at 64^3 more workgroups of
     // fewer lines finish sooner than fewer larger ones, down to this size
     int pb = 4;
@@ -2525,7 +2533,7 @@ static int ens_build_spec(ss_ensemble* e) {
     if (e->nm != 6 && e->nm != 9) return SS_OK;
     for (int j = 0; j < n; ++j) {
         const ss_sim* s = e->m[j];
-        if (!s->fused || s->paired) continue;
+        if (!s->fused || s->paired || s->mag) continue;  // the batched kernels have no magnetic solve
         ss_ensemble::SpecGroup* g = nullptr;
         for (auto& q : e->sgroups)
             if (q.p == s->spec_p) g = &q;

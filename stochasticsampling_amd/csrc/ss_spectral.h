@@ -31,6 +31,11 @@ constexpr int SPEC_PLANE_THREADS = 512;
 constexpr int SPEC_PENCIL_THREADS = 512;
 constexpr int SPEC_MAX_PENCILS = 16;  // pencils (or pencil pairs) per workgroup
 
+// Fields of the pencil output and of the inverse planes after u (3) and W
+// (3), present as flagged, in this order (the order of the solve kernel's
+// hot outputs): E (6), b (3), grad b (9).
+constexpr int SPEC_OUT_E = 1, SPEC_OUT_B = 2, SPEC_OUT_GB = 4;
+
 // A strided loop over t < n whose global loads are issued U iterations at a
 // time, the stores after them: a loop that consumes each load before the next
 // one waits for the memory latency once per iteration.
@@ -192,21 +197,23 @@ inline size_t spec_pencil_lds(int nx, int nl, size_t esz) {
     return esz * 2 * ((size_t)nx + 2 * (size_t)nl * (nx + 1));
 }
 
-// Pencils p0 .. p0+pb-1 (p = y nz + z) of the nsig stress components: forward
-// x transform, flow solve, inverse x transform of the nout = 6 (u, W) or 12
-// (u, W, E) outputs.
+// Pencils p0 .. p0+pb-1 (p = y nz + z) of the nsig stress components and,
+// with nmag = 3, the polarisation P-hat after them: forward x transform, flow
+// solve (and magnetic solve), inverse x transform of the nout outputs: u, W
+// and the fields `outs` flags (SPEC_OUT_*).
 // bid, groups: the block's index and the number of blocks in the launch of
-// one system.
+// one system.  zz_trace and nmag are not used together.
 template <typename T, int P>
 __device__ __forceinline__ void spec_pencils_body(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int np, int nz,
                                          int pb_max, int nsig, int nout, const SpecAxis& ax,
                                          const cplx<T>* __restrict__ twx, const T* __restrict__ kx,
                                          const T* __restrict__ ky, const T* __restrict__ kz,
                                          const T* __restrict__ ik2, const T* __restrict__ kinv, T screening, T norm,
-                                         int bid, int groups, int zz_trace) {
+                                         int bid, int groups, int zz_trace, int nmag, int outs) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     const int nx = P ? P : ax.n, ls = nx + 1;
-    const int nl = (nsig > nout ? nsig : nout) * pb_max;
+    const int nin = nsig + nmag;
+    const int nl = (nin > nout ? nin : nout) * pb_max;
     cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* A = tw + nx;
     cplx<T>* B = A + (size_t)nl * ls;
@@ -222,7 +229,7 @@ __device__ __forceinline__ void spec_pencils_body(const cplx<T>* __restrict__ in
     // four loads at a time: eight take the kernel past 128 VGPRs (one
     // workgroup per CU) where all axes have compile-time lengths; with
     // zz_trace the last (zz) component is neither loaded nor transformed
-    const int nload = zz_trace && nsig == 6 ? 5 : nsig;
+    const int nload = zz_trace && nsig == 6 ? 5 : nin;
     batched_loop<SPEC_PENCIL_THREADS, 4>(
         nload * nx * pb_max,
         [&](int f) {
@@ -251,9 +258,28 @@ __device__ __forceinline__ void spec_pencils_body(const cplx<T>* __restrict__ in
         for (int i = 0; i < 3; ++i) O[(size_t)(i * pb_max + p) * ls + xx] = cplx<T>{ur[i], ui[i]};
 #pragma unroll
         for (int w = 0; w < 3; ++w) O[(size_t)((3 + w) * pb_max + p) * ls + xx] = vort_part(k, ur, ui, w);
-        if (nout == 12) {
+        int o = 6;
+        if (outs & SPEC_OUT_E) {
 #pragma unroll
-            for (int w = 0; w < 6; ++w) O[(size_t)((6 + w) * pb_max + p) * ls + xx] = strain_part(k, ur, ui, w);
+            for (int w = 0; w < 6; ++w) O[(size_t)((o + w) * pb_max + p) * ls + xx] = strain_part(k, ur, ui, w);
+            o += 6;
+        }
+        if (nmag) {
+            const T kn[3] = {k[0] * kp.ki, k[1] * kp.ki, k[2] * kp.ki};
+            T br[3], bi[3];
+            mag_point<T>([&](int q) { return S[(size_t)((nsig + q) * pb_max + p) * ls + xx]; }, kn, norm, br, bi);
+            if (outs & SPEC_OUT_B) {
+#pragma unroll
+                for (int q = 0; q < 3; ++q) O[(size_t)((o + q) * pb_max + p) * ls + xx] = cplx<T>{br[q], bi[q]};
+                o += 3;
+            }
+            if (outs & SPEC_OUT_GB) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int b = 0; b < 3; ++b)
+                        O[(size_t)((o + 3 * a + b) * pb_max + p) * ls + xx] = grad_b(k, br, bi, a, b);
+            }
         }
     }
     __syncthreads();
@@ -269,17 +295,19 @@ __global__ void __launch_bounds__(SPEC_PENCIL_THREADS)
     k_spec_pencils(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int np, int nz, int pb_max,
                    int nsig, int nout, SpecAxis ax, const cplx<T>* __restrict__ twx, const T* __restrict__ kx,
                    const T* __restrict__ ky, const T* __restrict__ kz, const T* __restrict__ ik2,
-                   const T* __restrict__ kinv, T screening, T norm, int zz_trace) {
+                   const T* __restrict__ kinv, T screening, T norm, int zz_trace, int nmag, int outs) {
     spec_pencils_body<T, P>(in, out, np, nz, pb_max, nsig, nout, ax, twx, kx, ky, kz, ik2, kinv, screening, norm,
-                            (int)blockIdx.x, (int)gridDim.x, zz_trace);
+                            (int)blockIdx.x, (int)gridDim.x, zz_trace, nmag, outs);
 }
 
 // Inverse: field component o of plane x; the real part goes to record slot
-// o (u, W) or REC_E + o - 6 (E).
+// o (u, W), then to the slots of E, b (times mu_dd) and grad b, those `outs`
+// flags, in that order (the solve kernel and k_pack's layout).
 template <typename T, int P>
 __device__ __forceinline__ void spec_planes_inv_body(const cplx<T>* __restrict__ spec, int nx, const SpecAxis& ay,
                                             const SpecAxis& az, const cplx<T>* __restrict__ twy,
-                                            const cplx<T>* __restrict__ twz, T* __restrict__ rec, int rs, int bid) {
+                                            const cplx<T>* __restrict__ twz, T* __restrict__ rec, int rs, int bid,
+                                            int outs, T mu_dd) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     const int ny = P ? P : ay.n, nz = P ? P : az.n, np = ny * nz;
     cplx<T>* ty = reinterpret_cast<cplx<T>*>(smem_raw);
@@ -294,21 +322,40 @@ __device__ __forceinline__ void spec_planes_inv_body(const cplx<T>* __restrict__
     batched_loop<SPEC_PLANE_THREADS>(np, [&](int c) { return src[c]; }, [&](int c, cplx<T> v) { b0[c] = v; });
     __syncthreads();
     cplx<T>* r2 = spec_plane_inv<T, P>(b0, b1, ty, tz, ay, az, ny, nz);
-    const int slot = o < 6 ? o : REC_E + o - 6;
-    for (int c = threadIdx.x; c < np; c += SPEC_PLANE_THREADS) rec[(base + c) * rs + slot] = r2[c].x;
+    int slot = o, r = o - 6;
+    bool times_mu = false;
+    if (o >= 6) {
+        if ((outs & SPEC_OUT_E) && r < 6) {
+            slot = REC_E + r;
+        } else {
+            if (outs & SPEC_OUT_E) r -= 6;
+            if ((outs & SPEC_OUT_B) && r < 3) {
+                slot = REC_B + r;
+                times_mu = true;
+            } else {
+                if (outs & SPEC_OUT_B) r -= 3;
+                slot = REC_GB + r;
+            }
+        }
+    }
+    if (times_mu)
+        for (int c = threadIdx.x; c < np; c += SPEC_PLANE_THREADS) rec[(base + c) * rs + slot] = r2[c].x * mu_dd;
+    else
+        for (int c = threadIdx.x; c < np; c += SPEC_PLANE_THREADS) rec[(base + c) * rs + slot] = r2[c].x;
 }
 
 template <typename T, int P>
 __global__ void __launch_bounds__(SPEC_PLANE_THREADS)
     k_spec_planes_inv(const cplx<T>* __restrict__ spec, int nx, SpecAxis ay, SpecAxis az,
                       const cplx<T>* __restrict__ twy, const cplx<T>* __restrict__ twz, T* __restrict__ rec,
-                      int rs) {
-    spec_planes_inv_body<T, P>(spec, nx, ay, az, twy, twz, rec, rs, (int)blockIdx.x);
+                      int rs, int outs, T mu_dd) {
+    spec_planes_inv_body<T, P>(spec, nx, ay, az, twy, twz, rec, rs, (int)blockIdx.x, outs, mu_dd);
 }
 
 // ---------------------------------------------------------------------------
 // Batched launches: the spectral stages of several systems (an ensemble's
-// members of one axis-length class) in one launch of each kernel.  first[m]
+// members of one axis-length class, without the magnetic solve) in one
+// launch of each kernel.  first[m]
 // is the first block of system m (first[n] the total), so every block runs
 // the body of its own system with its index within that system.
 // ---------------------------------------------------------------------------
@@ -355,7 +402,7 @@ __global__ void __launch_bounds__(SPEC_PENCIL_THREADS)
     const SpecJob<T>& a = jobs[m];
     spec_pencils_body<T, P>(a.spec_in, a.spec_out, a.np, a.nz, a.pb, a.nsig, a.nout, a.ax, a.twx, a.kx, a.ky, a.kz,
                             a.ik2, a.kinv, a.screening, a.norm, (int)blockIdx.x - first[m], first[m + 1] - first[m],
-                            a.zz_trace);
+                            a.zz_trace, 0, a.nout == 12 ? SPEC_OUT_E : 0);
 }
 
 template <typename T, int P>
@@ -363,7 +410,8 @@ __global__ void __launch_bounds__(SPEC_PLANE_THREADS)
     k_spec_planes_inv_batched(const SpecJob<T>* __restrict__ jobs, const int* __restrict__ first, int n) {
     const int m = spec_job_of(first, n, (int)blockIdx.x);
     const SpecJob<T>& a = jobs[m];
-    spec_planes_inv_body<T, P>(a.spec_out, a.nx, a.ay, a.az, a.twy, a.twz, a.rec, a.rs, (int)blockIdx.x - first[m]);
+    spec_planes_inv_body<T, P>(a.spec_out, a.nx, a.ay, a.az, a.twy, a.twz, a.rec, a.rs, (int)blockIdx.x - first[m],
+                               a.nout == 12 ? SPEC_OUT_E : 0, T(1));
 }
 
 // ---------------------------------------------------------------------------

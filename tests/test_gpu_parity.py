@@ -421,6 +421,10 @@ def test_graph_replays_match_eager_steps(precision, binning):
         ((12, 20, 15, 6, 4), dict(shape=0.3, volume_exclusion=0.2)),
         ((16, 8, 10, 8, 8), dict(stress_magnetic=0.7)),
         ((12, 10, 1, 6, 4), dict(flags=ss.FLAG_QUASI2D)),
+        ((16, 16, 16, 8, 8), dict(magnetic_reorientation=2.0, field_direction=(0.0, 0.0, 1.0),
+                                  magnetic_dipole_dipole=0.5, magnetic_drag=0.1)),
+        ((12, 10, 8, 6, 4), dict(magnetic_dipole_dipole=0.4, shape=0.3)),
+        ((16, 8, 10, 6, 6), dict(magnetic_drag=0.2, volume_exclusion=0.2)),
     ],
 )
 def test_fused_spectral_stage_equals_axis_passes(precision, grid, extra):
@@ -428,7 +432,8 @@ def test_fused_spectral_stage_equals_axis_passes(precision, grid, extra):
     stages, axis order and solve as convert + one pass per FFT axis + solve +
     pack, so particles and fields agree bit for bit -- also where all axes are
     16, 32, 48 or 64 long and the stages have their sizes fixed at compile
-    time."""
+    time, and with the magnetic solve (b times mu_dd and grad b in the
+    records, alone or with E and the density)."""
     extra = dict(extra)
     flags = extra.pop("flags", 0)
     box = tuple(0.8 * g for g in grid[:3])
@@ -444,6 +449,8 @@ def test_fused_spectral_stage_equals_axis_passes(precision, grid, extra):
         sim.step(7)
         u, g = sim.get_flow_field(with_gradient=True)
         out.append((sim.get_particles(), u, g))
+        if "magnetic_dipole_dipole" in extra or "magnetic_drag" in extra:
+            out[-1] += sim.get_magnetic_field(with_gradient=True)
         # a second read of the fields finds the same (the x transform of the
         # moments is completed once)
         np.testing.assert_array_equal(sim.get_flow_field(), u)
