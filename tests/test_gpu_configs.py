@@ -274,7 +274,7 @@ OPTIONAL = dict(shape=0.6, volume_exclusion=0.3, hydro_screening=0.2, stress_mag
 
 
 @pytest.mark.parametrize("binning", [ss.BIN_HISTOGRAM, ss.BIN_SORTED, ss.BIN_PLAIN_TILED])
-@pytest.mark.parametrize("variant", ["y_field", "z_field", "small_alpha"])
+@pytest.mark.parametrize("variant", ["y_field", "z_field", "small_alpha", "symmetric_stress"])
 def test_optional_physics_whole_steps_fp64(oracle, binning, variant):
     """The Langevin chain of mod.rs:323-363 with Jeffery strain, density-
     dependent diffusion, screening, magnetic stress, field torque along the
@@ -282,7 +282,9 @@ def test_optional_physics_whole_steps_fp64(oracle, binning, variant):
     the stress of mod.rs:100-104 with all three kernels: counts bit-exact,
     u, grad u, B, grad B <= 1e-10 relative, all five state columns <= 1e-9.
     "small_alpha" runs active stress 1e-3 alone (the fixed-point moments of
-    the tiled path must keep their resolution for small coefficients)."""
+    the tiled path must keep their resolution for small coefficients);
+    "symmetric_stress" leaves the magnetic stress out, so that the tiled path
+    solves the flow and the magnetic field in its fused spectral stage."""
     box = (10.0, 9.0, 8.0)
     grid = (12, 10, 8, 6, 6)
     n = 6000
@@ -292,9 +294,13 @@ def test_optional_physics_whole_steps_fp64(oracle, binning, variant):
         kw = dict(OPTIONAL)
         if variant == "z_field":
             kw["field_direction"] = (0.0, 0.0, 1.0)
+        if variant == "symmetric_stress":
+            del kw["stress_magnetic"]
     p = _params(box, grid, n, 5, binning=binning, flags=ss.FLAG_KEEP_DISTRIBUTION, **kw)
     parts = ss.initial_condition(_params(box, grid, n, 5), amplitude=0.2, seed=5)
     sim, osim = _run_pair(oracle, p, parts, 4, _params(box, grid, n, 5, **kw))
+    if variant == "symmetric_stress" and binning == ss.BIN_SORTED:
+        assert sim.get_info()["fused_spectral"]
     np.testing.assert_array_equal(sim.get_counts(), osim.counts())
     u, g = sim.get_flow_field(with_gradient=True)
     ue, ge = osim.flow_field()
