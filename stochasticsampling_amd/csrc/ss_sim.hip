@@ -162,8 +162,9 @@ struct ss_sim {
     bool psort = false;
     int psort_lds = 0;
     uint32_t *strays = nullptr, *scnt = nullptr;  // drifted particles' keys per tile stretch, and their number
-    // spectral stage as three fused kernels (tiled path, no magnetic solve,
-    // radices 2-5); spec_in then holds moments transformed along z and y only
+    // spectral stage as three fused kernels (tiled path, radices 2-5; with
+    // the magnetic solve only for a symmetric stress and unpaired); spec_in
+    // then holds moments transformed along z and y only
     bool fused = false;
     bool spec_partial = false;
     // tiled path, symmetric stress of 6 components, not paired: sigma_zz is
