@@ -500,7 +500,9 @@ static hipError_t launch_spec_planes_fwd(ss_sim* s) {
     double ms, ds;
     scales(s, ms, ds);
     const size_t lds = spec_plane_lds(s->ny, s->nz, sizeof(T), s->spec_p > 0);
-    const dim3 grid((unsigned)(s->nx * (s->nm + 1)));  // + the count component
+    // + the count component when the density is wanted (otherwise the update
+    // never adds to its inflow, and there is nothing to convert or clear)
+    const dim3 grid((unsigned)(s->nx * (s->nm + (s->use_rho ? 1 : 0))));
     T* rho = s->use_rho ? (T*)s->rho : nullptr;
     const SpecAxis ay = spec_axis(s->fft.ax[1]), az = spec_axis(s->fft.ax[2]);
     const cplx<T>* twy = (const cplx<T>*)s->fft.ax[1].tw;
